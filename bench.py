@@ -1,0 +1,169 @@
+"""Benchmark: points/s of the triangulation path on one MI355X.
+
+Workload: 1,000,000 points i.i.d. U[0,1)^3, f32, seed 1 (numpy PCG64), already
+resident in HBM when the timed region starts; the result (tets + adjacency)
+stays in HBM (wtet_delaunay_resident).  One step = one whole triangulation:
+normalize + sort, insertion rounds, flips, exact audit, compaction, and the
+host repair of any face the GPU phase left non-Delaunay.
+
+Reported: the filtered path (f32 filter -> f64 filter -> exact) as ``value``
+and the plain exact-only path beside it, the live roofline of the dominant
+kernel (HIP events on the library's stream), and the CPU oracle (serial
+Bowyer-Watson) timed on this host as the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# algorithmic bytes per unit (minimal SoA traffic):
+#   flip_check: per active tet checked -- own quad + adjacency 32, four
+#   neighbour quads 64, eight vertex xyz 96, vote atomic 8
+BYTES_PER_UNIT = {"flip_check": 200}
+
+
+def job_bytes(st):
+    """Whole-job algorithmic bytes from the run's own counters:
+    12N (read xyz) + 128 per uninserted point per round + 224N (splits)
+    + 200 per checked tet + 256 per flip + 32 per output tet."""
+    n = st["num_points"]
+    return (12 * n + 128 * st["sum_uninserted"] + 224 * n + 200 * st["sum_active"]
+            + 256 * st["sum_flips"] + 32 * st["num_tets_out"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the exact-only comparison")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "the path is single-device and no multi-GPU figure is produced.", file=sys.stderr)
+        sys.exit(1)
+
+    import torch
+
+    import wtet_amd
+    from wtet_amd import workloads as W
+
+    pts = W.uniform(args.n, args.seed)
+    d_pts = torch.from_numpy(pts).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = wtet_amd.Context(0)
+    filt = wtet_amd.Config(predicate_mode="filtered")
+    exact = wtet_amd.Config(predicate_mode="exact")
+
+    # counters and parity run (untimed)
+    res = ctx.delaunay(pts, wtet_amd.Config(predicate_mode="filtered", collect_stats=True))
+    st = res.stats
+    pred = st["pred"]
+    o3 = max(1, sum(pred[0:4]))
+    ins = max(1, sum(pred[4:8]))
+    stages = {
+        "orient3d": {"f32": pred[0] / o3, "f64": pred[1] / o3, "exact": pred[2] / o3, "sos": pred[3] / o3},
+        "insphere": {"f32": pred[4] / ins, "f64": pred[5] / ins, "exact": pred[6] / ins, "sos": pred[7] / ins},
+    }
+
+    def run(cfg, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            ctx.delaunay_resident(d_pts.data_ptr(), args.n, cfg)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / max(k, 1)
+
+    run(filt, args.warmup)
+    ms = run(filt, args.steps)
+    value = args.n / (ms / 1e3)
+    print(f"filtered: {ms:.2f} ms/step, {value:.4g} points/s", file=sys.stderr, flush=True)
+    st_t = ctx.stats()
+
+    # live per-kernel timing (HIP events on the library's stream)
+    ctx.set_profiling(True)
+    ctx.delaunay_resident(d_pts.data_ptr(), args.n, filt)
+    kt = ctx.kernel_times()
+    st_p = ctx.stats()
+    ctx.set_profiling(False)
+    dom = "flip_check"
+    launches, tot_ms = kt[dom]
+    bytes_dom = BYTES_PER_UNIT[dom] * st_p["sum_active"]
+    achieved = bytes_dom / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_flip_check", "launches": launches,
+                "avg_launch_ms": round(tot_ms / max(launches, 1), 4),
+                "bytes_per_launch_avg": int(bytes_dom / max(launches, 1))}
+
+    plain = None
+    if not args.no_plain:
+        run(exact, 1)
+        ms_plain = run(exact, max(1, min(args.steps, 3)))
+        plain = {"value": args.n / (ms_plain / 1e3), "ms_per_step": round(ms_plain, 3)}
+        print(f"exact-only: {ms_plain:.2f} ms/step", file=sys.stderr, flush=True)
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        pn, _, _ = O.normalize_points(pts)
+        t0 = time.perf_counter()
+        tets = O.delaunay(pn)
+        dt = time.perf_counter() - t0
+        cpu = {"value": args.n / dt, "unit": "points/s", "cores": 1, "kind": "port",
+               "sample": f"the whole {args.n}-point input, serial Bowyer-Watson of the oracle "
+                         f"({dt:.1f} s, exact predicates + SoS)"}
+        ref = O.canonical_real(tets, args.n)
+        got = O.canonical_real(res.tets, args.n)
+        parity = bool(ref.shape == got.shape and np.array_equal(ref, got))
+
+    kern = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in kt.items() if v[0]}
+    out = {
+        "metric": "points/s @1M uniform f32: plain vs filtered+batched, same GPU, bit-exact tets",
+        "value": round(value, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"uniform_{args.n}_seed{args.seed}", "points": args.n,
+                   "distribution": "i.i.d. U[0,1)^3 f32, numpy PCG64", "predicates": "f32->f64->exact",
+                   "batched": False},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_exact": plain,
+        "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
+        "parity_with_oracle": parity,
+        "stage_fractions": stages,
+        "ms_device": round(st_t["ms_device"], 3),
+        "ms_repair": round(st_t["ms_repair"], 3),
+        "violations_gpu": st["violations_gpu"],
+        "repair_regions": st["repair_regions"],
+        "rounds": st["rounds"],
+        "flip_iterations": st["flip_iterations"],
+        "job_algorithmic_bytes": job_bytes(st),
+        "job_achieved_GBs": round(job_bytes(st) / (ms / 1e3) / 1e9, 1),
+        "kernel_ms": kern,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * wtet_amd -- C ABI of the MI355X Delaunay triangulation path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns a status code
+ * (never aborts the process); wtet_last_error() describes the last failure
+ * of a context.  Each declaration names the reference interface it
+ * replaces (paths relative to the reference repository).
+ */
+#ifndef WTET_AMD_H
+#define WTET_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WTET_INVALID 0xFFFFFFFFu /* src/types.rs:177 INVALID */
+
+typedef enum wtet_status {
+  WTET_OK = 0,
+  WTET_ERR_INVALID_ARGUMENT = 1, /* bad shape / null pointer / bad option    */
+  WTET_ERR_TOO_FEW_POINTS = 2,   /* n < 4 (src/python.rs:182-187, lib.rs:167) */
+  WTET_ERR_DEVICE = 3,           /* HIP runtime failure or no device          */
+  WTET_ERR_OUT_OF_MEMORY = 4,    /* device or host allocation failed          */
+  WTET_ERR_NOT_CONVERGED = 5,    /* could not complete the triangulation      */
+  WTET_ERR_INTERNAL = 6          /* invariant violated inside the library     */
+} wtet_status;
+
+/* GDelConfig, src/types.rs:32-61 (defaults :49-61); the two trailing
+ * fields are additions of this library. */
+typedef struct wtet_config {
+  int32_t insertion_rule;         /* 0 circumcenter, 1 centroid (types.rs:17-28) */
+  int32_t enable_flipping;        /* 1 */
+  int32_t enable_sorting;         /* 0: Morton order requested                 */
+  int32_t enable_hilbert_sorting; /* 0: Hilbert + stratified order requested    */
+  int32_t enable_splaying;        /* 1: repair any face the GPU leaves non-Delaunay */
+  uint32_t max_insert_iterations; /* 100 */
+  uint32_t max_flip_iterations;   /* 10: flip iterations per insertion round   */
+  int32_t predicate_mode;         /* 0: f32 filter -> f64 filter -> exact; 1: exact only */
+  int32_t collect_stats;          /* 1: count predicate stages (small cost)    */
+} wtet_config;
+
+/* DelaunayResult, src/types.rs:69-79, owned by the caller after the call;
+ * release with wtet_result_free. */
+typedef struct wtet_result {
+  uint32_t num_tets;
+  uint32_t *tets;      /* num_tets*4 vertex ids: input ids, super n..n+3, infinity n+4 */
+  uint32_t *adjacency; /* num_tets*4: (tet << 5) | face of the neighbour across face f,
+                          face f opposite vertex f; WTET_INVALID when none */
+  uint32_t num_failed;
+  uint32_t *failed_verts; /* input ids of vertices opposite faces the GPU phase left
+                             non-Delaunay (before the host repair), sorted */
+} wtet_result;
+
+typedef struct wtet_stats {
+  uint64_t pred[8]; /* orient3d: f32, f64, exact nonzero, exact zero (SoS);
+                       insphere: f32, f64, exact nonzero, exact zero (SoS) */
+  uint64_t sum_uninserted; /* sum over rounds of uninserted points            */
+  uint64_t sum_active;     /* sum over flip iterations of tets checked         */
+  uint64_t sum_flips;      /* flips executed                                  */
+  uint64_t num_points;
+  uint64_t num_tets_out;   /* tets in the result (incl. super/infinity tets)   */
+  uint64_t tet_slots;      /* tet slots used on the device                     */
+  uint32_t rounds;
+  uint32_t flip_iterations;
+  uint32_t violations_gpu; /* non-Delaunay faces left by the GPU phase          */
+  uint32_t repair_regions; /* regions re-triangulated on the host             */
+  uint32_t walk_failures;
+  uint32_t grow_events;    /* device buffer growths                           */
+  double ms_device;        /* normalized points in HBM -> tets in HBM          */
+  double ms_repair;        /* host repair                                      */
+  double ms_total;         /* whole call                                       */
+} wtet_stats;
+
+typedef struct wtet_ctx wtet_ctx;
+
+/* Device singleton of src/python.rs:13-47 becomes an explicit context. */
+wtet_status wtet_create(int device, wtet_ctx **out);
+void wtet_destroy(wtet_ctx *ctx);
+const char *wtet_last_error(const wtet_ctx *ctx);
+
+void wtet_config_default(wtet_config *cfg);
+
+/* delaunay_3d(device, queue, points, config), src/lib.rs:160-310;
+ * xyz: n*3 row-major f32 in host memory. */
+wtet_status wtet_delaunay(wtet_ctx *ctx, const float *xyz, uint32_t n,
+                          const wtet_config *cfg, wtet_result *out);
+
+/* Same with the points already resident in device memory (d_xyz: n*3 f32). */
+wtet_status wtet_delaunay_device(wtet_ctx *ctx, const float *d_xyz, uint32_t n,
+                                 const wtet_config *cfg, wtet_result *out);
+
+void wtet_result_free(wtet_result *res);
+
+/* Result left in device memory owned by the context (valid until the next
+ * call on the same context): same layout as wtet_result. */
+typedef struct wtet_device_result {
+  uint32_t num_tets;
+  uint32_t *d_tets;
+  uint32_t *d_adjacency;
+  uint32_t num_failed;
+} wtet_device_result;
+
+/* Points and result both resident in device memory: no host copy of the
+ * mesh unless the host repair runs (it then downloads the mesh, repairs it
+ * and uploads the result).  No counterpart in the reference, whose
+ * delaunay_3d always reads back (src/lib.rs:181, src/gpu/mod.rs:207-332). */
+wtet_status wtet_delaunay_resident(wtet_ctx *ctx, const float *d_xyz, uint32_t n,
+                                   const wtet_config *cfg, wtet_device_result *out);
+
+/* Counters of the last wtet_delaunay* call on this context. */
+wtet_status wtet_get_stats(const wtet_ctx *ctx, wtet_stats *out);
+
+/* check_quality(points, result), src/python.rs:282-339 (check_delaunay_quality,
+ * src/lib.rs:25-78): insphere violations among tet pairs whose vertices are
+ * all real, and the number of such interior tets.  Host code. */
+wtet_status wtet_check_quality(const float *xyz, uint32_t n, const uint32_t *tets,
+                               const uint32_t *adjacency, uint32_t num_tets,
+                               uint64_t *violations, uint64_t *interior_tets);
+
+/* encode_adjacency / decode_adjacency, src/python.rs:256-272 */
+uint32_t wtet_encode_adjacency(uint32_t tet, uint32_t face);
+void wtet_decode_adjacency(uint32_t packed, uint32_t *tet, uint32_t *face);
+
+/* __version__ and gpu_info(), src/python.rs:355-371, 397-399 */
+const char *wtet_version(void);
+wtet_status wtet_device_info(wtet_ctx *ctx, char *buf, uint32_t len);
+
+/* Per-kernel-group timing with HIP events on the context's stream (the
+ * reference's GpuProfiler/CpuProfiler scopes, src/profiler.rs:5-224).
+ * wtet_set_profiling resets the totals; kinds 0..11 are prepare_points,
+ * vote, pick, split, split_points, flip_check, flip_decide, flip_exec,
+ * flip_update, relocate, audit, output.  Totals accumulate over calls. */
+wtet_status wtet_set_profiling(wtet_ctx *ctx, int on);
+wtet_status wtet_kernel_times(const wtet_ctx *ctx, uint32_t kind, const char **name,
+                              uint32_t *launches, double *total_ms);
+
+/* Host phase 2 alone (src/lib.rs:227-307: cpu_flip_violations ->
+ * full_bw_reconstruction -> splay, src/phase2.rs): turns a valid
+ * triangulation of the n normalized points (ids 0..n-1, super vertices
+ * n..n+3, optional infinity tets with id n+4) into the exact Delaunay one.
+ * *tets (num_tets*4) and *adjacency (same layout as wtet_result; may be
+ * null: rebuilt by face matching) are read, then replaced by new malloc'd
+ * arrays the caller frees with free().  regions (may be null) receives the
+ * number of re-triangulated regions. */
+wtet_status wtet_repair(const float *xyzn, uint32_t n, uint32_t **tets, uint32_t **adjacency,
+                        uint32_t *num_tets, int mode, uint32_t *regions);
+
+/* Predicate batches (orient3d/insphere staging and SoS, src/predicates.rs).
+ * kind: 0 orient3d_sos (4 pts, 4 ids), 1 orient4d (5 pts, 5 ids),
+ *       2 orient3d raw sign (4 pts), 3 insphere raw sign (5 pts).
+ * mode: 0 filtered, 1 exact only.  pts f32 xyz triples, one query after
+ * another.  The _host variant runs the host copy used by the repair stage;
+ * the _device variant runs the kernels' code on the GPU (host buffers in,
+ * host buffers out) and adds stage counters to stats8 when non-null. */
+wtet_status wtet_predicates_host(int kind, const float *pts, const uint32_t *ids, uint32_t n,
+                                 int mode, int32_t *out);
+wtet_status wtet_predicates_device(wtet_ctx *ctx, int kind, const float *pts,
+                                   const uint32_t *ids, uint32_t n, int mode, int32_t *out,
+                                   uint64_t *stats8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WTET_AMD_H */

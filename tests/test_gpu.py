@@ -1,0 +1,196 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the oracle.
+
+Bar: the canonical real-only tet set (sorted quads, sorted list) equals the
+oracle's exactly; the device predicates equal the oracle's signs exactly.
+At sizes the oracle cannot finish quickly the size-independent properties
+are checked instead (no insphere violation on any face, exact counts).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import wtet_amd
+from wtet_amd import _core
+from wtet_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "inputs_sha256.json")))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = wtet_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _check(oracle, ctx, pts, mode="filtered", **kw):
+    n = len(pts)
+    res = ctx.delaunay(pts, wtet_amd.Config(predicate_mode=mode, collect_stats=True, **kw))
+    pn, _, _ = oracle.normalize_points(pts)
+    ref = oracle.canonical_real(oracle.delaunay(pn), n)
+    got = oracle.canonical_real(res.tets, n)
+    assert got.shape == ref.shape, (got.shape, ref.shape, res.stats)
+    assert np.array_equal(got, ref), res.stats
+    # adjacency is symmetric and every face has a neighbour (infinity tets close the mesh)
+    adj = res.adjacency
+    t = np.repeat(np.arange(len(adj)), 4)
+    f = np.tile(np.arange(4), len(adj))
+    a = adj.ravel()
+    assert np.all(a != wtet_amd.INVALID)
+    back = adj[a >> 5, a & 3]
+    assert np.array_equal(back, (t << 5) | f)
+    return res
+
+
+# -- predicates ------------------------------------------------------------------
+
+def _pred_cases(rng, npts, k):
+    A = rng.random((k, npts, 3), dtype=np.float32)
+    base = rng.random((k, 1, 3), dtype=np.float32)
+    B = (base + rng.standard_normal((k, npts, 3)).astype(np.float32) * np.float32(1e-6)).astype(np.float32)
+    C = (rng.integers(0, 4, (k, npts, 3)) / np.float32(3)).astype(np.float32)
+    D = (rng.integers(0, 128, (k, npts, 3)).astype(np.float32) / np.float32(127)).astype(np.float32)
+    S = rng.random((k, npts, 3), dtype=np.float32)
+    sup = np.array([[-100, -100, -100], [400, -100, -100], [-100, 400, -100], [-100, -100, 400]], np.float32)
+    S[:, 0, :] = sup[rng.integers(0, 4, k)]
+    T = (rng.random((k, npts, 3), dtype=np.float32) * np.float32(1e-30)).astype(np.float32)
+    T[:, :2] = rng.random((k, 2, 3), dtype=np.float32)
+    return np.concatenate([A, B, C, D, S, T])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_device_predicates_match_oracle(oracle, ctx, mode):
+    rng = np.random.default_rng(31)
+    for kind, npts, ofn in [(0, 4, oracle.orient3d_sos), (1, 5, oracle.orient4d)]:
+        q = _pred_cases(rng, npts, 20000)
+        idx = np.array([rng.permutation(40)[:npts] for _ in range(len(q))], np.uint32)
+        ref = ofn(q.astype(np.float64), idx)
+        got, st = ctx.predicates(kind, q, idx, mode)
+        assert np.array_equal(got, ref), int((got != ref).sum())
+        base = 0 if kind == 0 else 4
+        assert int(st[base:base + 4].sum()) == len(q)  # every query counted once
+        if mode == 1:
+            assert st[base] == 0 and st[base + 1] == 0  # exact-only never filters
+    for kind, npts, ofn in [(2, 4, oracle.orient3d_sign), (3, 5, oracle.insphere_sign)]:
+        q = _pred_cases(rng, npts, 10000)
+        got, _ = ctx.predicates(kind, q, None, mode)
+        assert np.array_equal(got, ofn(q.astype(np.float64)))
+
+
+# -- triangulations ----------------------------------------------------------------
+
+def test_config1_uniform_10k_exact_path(oracle, ctx):
+    """10,000 U[0,1)^3 f32 seed 0, exact predicates only."""
+    p = W.uniform(10000, 0)
+    assert W.sha256(p) == GOLD["uniform_10000_seed0"]
+    res = _check(oracle, ctx, p, mode="exact")
+    can = oracle.canonical_real(res.tets, len(p))
+    g = GOLD["oracle_canonical_uniform_10000_seed0"]
+    assert len(can) == g["count"]
+    assert hashlib.sha256(can.astype("<u4").tobytes()).hexdigest() == g["sha256"]
+    st = res.stats
+    assert st["pred"][0] == st["pred"][1] == st["pred"][4] == st["pred"][5] == 0
+
+
+@pytest.mark.parametrize("n,seed", [(4, 1), (5, 2), (20, 3), (100, 4), (1000, 5), (10000, 0), (50000, 7)])
+def test_uniform_filtered(oracle, ctx, n, seed):
+    _check(oracle, ctx, W.uniform(n, seed))
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 8, 16])
+def test_lattice_degenerate(oracle, ctx, k):
+    res = _check(oracle, ctx, W.lattice(k))
+    if k == 16:
+        can = oracle.canonical_real(res.tets, k ** 3)
+        g = GOLD["oracle_canonical_lattice_16"]
+        assert len(can) == g["count"]
+        assert hashlib.sha256(can.astype("<u4").tobytes()).hexdigest() == g["sha256"]
+
+
+@pytest.mark.parametrize("kind", ["sphere", "shell", "clusters", "duplicates", "plane", "line",
+                                  "scaled", "cospherical"])
+def test_special_sets(oracle, ctx, kind):
+    rng = np.random.default_rng(17)
+    if kind == "sphere":
+        p = W.sphere(4000, 1)
+    elif kind == "shell":
+        p = W.shell(3000, 2)
+    elif kind == "clusters":
+        p = W.clusters(5000, 3)
+    elif kind == "duplicates":
+        q = rng.random((400, 3), dtype=np.float32)
+        p = np.vstack([q, q[:200]])
+    elif kind == "plane":
+        p = np.hstack([rng.random((600, 2), dtype=np.float32), np.full((600, 1), 0.25, np.float32)])
+    elif kind == "line":
+        p = np.hstack([rng.random((60, 1), dtype=np.float32), np.zeros((60, 2), np.float32)])
+    elif kind == "scaled":
+        p = (rng.standard_normal((3000, 3)) * 1e4 + 5e5).astype(np.float32)
+    else:
+        p = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                      [0.6, 0.8, 0], [0, 0.6, 0.8], [0.8, 0, 0.6]], np.float32)
+    _check(oracle, ctx, p)
+
+
+def test_exact_and_filtered_agree_on_lattice(oracle, ctx):
+    p = W.lattice(12)
+    a = ctx.delaunay(p, wtet_amd.Config(predicate_mode="exact"))
+    b = ctx.delaunay(p, wtet_amd.Config(predicate_mode="filtered"))
+    assert np.array_equal(oracle.canonical_real(a.tets, len(p)), oracle.canonical_real(b.tets, len(p)))
+
+
+def test_uniform_200k_matches_oracle(oracle, ctx):
+    _check(oracle, ctx, W.uniform(200000, 11))
+
+
+def test_uniform_1m_properties(ctx):
+    """configs[1] shape (1,000,000 U[0,1)^3 f32 seed 1): no face fails the
+    insphere test and the counts are consistent (size-independent checks)."""
+    p = W.uniform(1000000, 1)
+    assert W.sha256(p) == GOLD["uniform_1000000_seed1"]
+    res = ctx.delaunay(p, wtet_amd.Config(collect_stats=True))
+    v, interior = wtet_amd.check_quality(p, res)
+    assert v == 0
+    assert interior > 6 * len(p)
+    ids = np.unique(res.tets)
+    assert ids[0] == 0 and len(ids) == len(p) + 5  # every point, 4 super, infinity
+
+
+def test_context_reuse_and_smaller_second_call(oracle, ctx):
+    _check(oracle, ctx, W.uniform(20000, 21))
+    _check(oracle, ctx, W.uniform(3000, 22))
+
+
+def test_module_level_api(oracle):
+    p = W.uniform(2000, 5)
+    r = wtet_amd.delaunay(p)
+    assert r.tets.dtype == np.uint32 and r.adjacency.shape == r.tets.shape
+    assert r.failed_verts.dtype == np.uint32
+    assert "GPU" in wtet_amd.initialize_gpu()
+    assert "GPU Device" in wtet_amd.gpu_info()
+    v, _ = wtet_amd.check_quality(p, r)
+    assert v == 0
+
+
+def test_resident_matches_host_result(oracle, ctx):
+    """Device-resident entry point: same mesh as the host-result call."""
+    torch = pytest.importorskip("torch")
+    p = W.uniform(30000, 8)
+    t = torch.from_numpy(p).to("cuda:0")
+    torch.cuda.synchronize()
+    m, dt, da = ctx.delaunay_resident(t.data_ptr(), len(p))
+
+    class _Dev:
+        def __init__(self, ptr, n):
+            self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<i4", "data": (ptr, False),
+                                             "version": 3}
+
+    tets = torch.as_tensor(_Dev(dt, m * 4), device="cuda:0").cpu().numpy().view(np.uint32).reshape(m, 4)
+    ref = oracle.canonical_real(ctx.delaunay(p).tets, len(p))
+    assert np.array_equal(oracle.canonical_real(tets, len(p)), ref)

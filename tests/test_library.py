@@ -1,0 +1,163 @@
+"""CPU-side checks of the product library: it loads, exports every symbol
+include/wtet.h declares, and its host code (predicates used by the repair
+stage, the repair itself, check_quality, the Python surface) agrees with
+the oracle.  No device calls."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import wtet_amd
+from wtet_amd import _core
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "wtet.h")).read()
+    return sorted(set(re.findall(r"\b(wtet_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _core.lib()
+    syms = _declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def test_adjacency_encoding_matches_reference_known_answers():
+    import json
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_known_answers.json")))
+    L = _core.lib()
+    import ctypes
+    for c in gold["encode_opp"]:
+        p = wtet_amd.encode_adjacency(c["tet"], c["face"])
+        assert wtet_amd.decode_adjacency(p) == (c["tet"], c["face"])
+        assert L.wtet_encode_adjacency(c["tet"], c["face"]) == p
+        t, f = ctypes.c_uint32(), ctypes.c_uint32()
+        L.wtet_decode_adjacency(p, ctypes.byref(t), ctypes.byref(f))
+        assert (t.value, f.value) == (c["tet"], c["face"])
+    assert wtet_amd.INVALID == 2 ** 32 - 1
+
+
+def _pred_cases(rng, npts, k):
+    A = rng.random((k, npts, 3), dtype=np.float32)
+    base = rng.random((k, 1, 3), dtype=np.float32)
+    B = (base + rng.standard_normal((k, npts, 3)).astype(np.float32) * np.float32(1e-6)).astype(np.float32)
+    C = (rng.integers(0, 4, (k, npts, 3)) / np.float32(3)).astype(np.float32)
+    D = (rng.integers(0, 128, (k, npts, 3)).astype(np.float32) / np.float32(127)).astype(np.float32)
+    S = rng.random((k, npts, 3), dtype=np.float32)
+    sup = np.array([[-100, -100, -100], [400, -100, -100], [-100, 400, -100], [-100, -100, 400]], np.float32)
+    S[:, 0, :] = sup[rng.integers(0, 4, k)]
+    T = (rng.random((k, npts, 3), dtype=np.float32) * np.float32(1e-30)).astype(np.float32)
+    T[:, :2] = rng.random((k, 2, 3), dtype=np.float32)
+    return np.concatenate([A, B, C, D, S, T])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_host_predicates_match_oracle(oracle, mode):
+    rng = np.random.default_rng(21)
+    for kind, npts, ofn in [(0, 4, oracle.orient3d_sos), (1, 5, oracle.orient4d)]:
+        q = _pred_cases(rng, npts, 3000)
+        idx = np.array([rng.permutation(40)[:npts] for _ in range(len(q))], np.uint32)
+        ref = ofn(q.astype(np.float64), idx)
+        got = _core.predicates_host(kind, q, idx, mode)
+        assert np.array_equal(got, ref)
+    for kind, npts, ofn in [(2, 4, oracle.orient3d_sign), (3, 5, oracle.insphere_sign)]:
+        q = _pred_cases(rng, npts, 2000)
+        assert np.array_equal(_core.predicates_host(kind, q, None, mode), ofn(q.astype(np.float64)))
+
+
+def _random_valid_flips(tets, rounds, rng):
+    """Apply random 2-3 flips to a triangulation (valid, no longer Delaunay)."""
+    from collections import defaultdict
+    tets = [list(t) for t in tets]
+    for _ in range(rounds):
+        faces = defaultdict(list)
+        for i, t in enumerate(tets):
+            for f in range(4):
+                faces[tuple(sorted(t[:f] + t[f + 1:]))].append(i)
+        shared = [v for v in faces.values() if len(v) == 2]
+        i, j = shared[rng.integers(len(shared))]
+        yield i, j, tets
+
+
+def test_repair_restores_delaunay_from_perturbed_mesh(oracle):
+    """Flip random interior faces of the exact triangulation with 2-3 flips
+    wherever geometrically valid (a non-Delaunay but valid mesh), then the
+    host repair must return exactly the oracle's tet set."""
+    rng = np.random.default_rng(3)
+    p = rng.random((400, 3), dtype=np.float32)
+    pn, _, _ = oracle.normalize_points(p)
+    n = len(pn)
+    ref = oracle.delaunay(pn)
+    ext = oracle.extended_points(pn)
+    tets = [list(map(int, t)) for t in ref]
+    flipped = 0
+    for _ in range(400):
+        if flipped >= 40:
+            break
+        adj = oracle.rebuild_adjacency(np.array(tets, np.uint32))
+        t = int(rng.integers(len(tets)))
+        f = int(rng.integers(4))
+        a = int(adj[t, f])
+        if a == 0xFFFFFFFF:
+            continue
+        u, g = a >> 5, a & 3
+        T, U = tets[t], tets[u]
+        apex, o = T[f], U[g]
+        face = [T[k] for k in ((1, 3, 2), (0, 2, 3), (0, 3, 1), (0, 1, 2))[f]]
+        ok = True
+        new = []
+        for k in range(3):
+            x, y = face[k], face[(k + 1) % 3]
+            s = oracle.orient3d_sos(ext[[x, y, apex, o]][None], np.array([[x, y, apex, o]], np.uint32))[0]
+            if s != -1:
+                ok = False
+                break
+            new.append([x, y, o, apex])
+        if not ok:
+            continue
+        for idx in sorted([t, u], reverse=True):
+            tets.pop(idx)
+        tets.extend(new)
+        flipped += 1
+    assert flipped >= 10
+    bad = np.array(tets, np.uint32)
+    adjb = oracle.rebuild_adjacency(bad)
+    assert oracle.count_violations(pn, bad, adjb, only_real=False) > 0
+    out_t, out_a, regions = _core.repair(pn, bad, None)
+    assert regions >= 1
+    got = oracle.canonical_real(out_t, n)
+    assert np.array_equal(got, oracle.canonical_real(ref, n))
+    assert oracle.count_violations(pn, out_t, oracle.rebuild_adjacency(out_t), only_real=False) == 0
+
+
+def test_check_quality_counts_like_reference(oracle):
+    rng = np.random.default_rng(9)
+    p = rng.random((300, 3), dtype=np.float32)
+    pn, _, _ = oracle.normalize_points(p)
+    tets = oracle.delaunay(pn)
+    adj = oracle.rebuild_adjacency(tets)
+    res = wtet_amd.DelaunayResult(tets, adj, np.zeros(0, np.uint32))
+    v, interior = wtet_amd.check_quality(p, res)
+    assert v == 0
+    assert interior == len(oracle.canonical_real(tets, len(p)))
+
+
+def test_python_surface_errors_and_reprs():
+    with pytest.raises(ValueError, match=r"Points must have shape \(N, 3\), got \[5, 2\]"):
+        wtet_amd.delaunay(np.zeros((5, 2), np.float32))
+    with pytest.raises(ValueError, match="Need at least 4 points for 3D Delaunay, got 3"):
+        wtet_amd.delaunay(np.zeros((3, 3), np.float32))
+    with pytest.raises(ValueError, match="Invalid insertion_rule 'midpoint'"):
+        wtet_amd.Config(insertion_rule="midpoint")
+    c = wtet_amd.Config()
+    assert repr(c) == ("Config(insertion_rule='circumcenter', flipping=true, sorting=false, "
+                       "hilbert=false, splaying=true, max_insert_iter=100, max_flip_iter=10)")
+    r = wtet_amd.DelaunayResult(np.zeros((7, 4), np.uint32), np.zeros((7, 4), np.uint32),
+                                np.zeros(0, np.uint32))
+    assert repr(r) == "DelaunayResult(num_tets=7, failed_verts=0)"
+    assert wtet_amd.__version__ == "0.1.0"

@@ -1,0 +1,422 @@
+"""ctypes binding of libwtet_amd.so, mirroring the reference's ``wtet._core``
+PyO3 module (src/python.rs:13-402): ``delaunay``, ``Config``,
+``DelaunayResult``, ``check_quality``, ``initialize_gpu``, ``gpu_info``,
+``encode_adjacency``, ``decode_adjacency``, ``INVALID``, ``__version__``.
+
+The HIP library is required: importing works without it, but every call
+that needs the device raises ``RuntimeError`` naming the missing file.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libwtet_amd.so")
+
+INVALID = 0xFFFFFFFF
+__version__ = "0.1.0"
+
+_STATUS = {
+    0: "ok",
+    1: "invalid argument",
+    2: "too few points",
+    3: "device error",
+    4: "out of memory",
+    5: "not converged",
+    6: "internal error",
+}
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("insertion_rule", ctypes.c_int32),
+        ("enable_flipping", ctypes.c_int32),
+        ("enable_sorting", ctypes.c_int32),
+        ("enable_hilbert_sorting", ctypes.c_int32),
+        ("enable_splaying", ctypes.c_int32),
+        ("max_insert_iterations", ctypes.c_uint32),
+        ("max_flip_iterations", ctypes.c_uint32),
+        ("predicate_mode", ctypes.c_int32),
+        ("collect_stats", ctypes.c_int32),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("num_tets", ctypes.c_uint32),
+        ("tets", ctypes.POINTER(ctypes.c_uint32)),
+        ("adjacency", ctypes.POINTER(ctypes.c_uint32)),
+        ("num_failed", ctypes.c_uint32),
+        ("failed_verts", ctypes.POINTER(ctypes.c_uint32)),
+    ]
+
+
+class _DevResult(ctypes.Structure):
+    _fields_ = [
+        ("num_tets", ctypes.c_uint32),
+        ("d_tets", ctypes.c_void_p),
+        ("d_adjacency", ctypes.c_void_p),
+        ("num_failed", ctypes.c_uint32),
+    ]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [
+        ("pred", ctypes.c_uint64 * 8),
+        ("sum_uninserted", ctypes.c_uint64),
+        ("sum_active", ctypes.c_uint64),
+        ("sum_flips", ctypes.c_uint64),
+        ("num_points", ctypes.c_uint64),
+        ("num_tets_out", ctypes.c_uint64),
+        ("tet_slots", ctypes.c_uint64),
+        ("rounds", ctypes.c_uint32),
+        ("flip_iterations", ctypes.c_uint32),
+        ("violations_gpu", ctypes.c_uint32),
+        ("repair_regions", ctypes.c_uint32),
+        ("walk_failures", ctypes.c_uint32),
+        ("grow_events", ctypes.c_uint32),
+        ("ms_device", ctypes.c_double),
+        ("ms_repair", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def lib():
+    """The loaded HIP library (raises RuntimeError when it is not built)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"wtet_amd HIP library not found at {LIB_PATH}; build it with "
+                    "`make -C wtet_amd/csrc` (hipcc, gfx950)")
+            L = ctypes.CDLL(LIB_PATH)
+            P = ctypes.c_void_p
+            u32 = ctypes.c_uint32
+            L.wtet_create.argtypes = [ctypes.c_int, ctypes.POINTER(P)]
+            L.wtet_destroy.argtypes = [P]
+            L.wtet_last_error.argtypes = [P]
+            L.wtet_last_error.restype = ctypes.c_char_p
+            L.wtet_config_default.argtypes = [ctypes.POINTER(_Config)]
+            L.wtet_delaunay.argtypes = [P, P, u32, ctypes.POINTER(_Config), ctypes.POINTER(_Result)]
+            L.wtet_delaunay_device.argtypes = [P, P, u32, ctypes.POINTER(_Config),
+                                               ctypes.POINTER(_Result)]
+            L.wtet_result_free.argtypes = [ctypes.POINTER(_Result)]
+            L.wtet_delaunay_resident.argtypes = [P, P, u32, ctypes.POINTER(_Config),
+                                                 ctypes.POINTER(_DevResult)]
+            L.wtet_get_stats.argtypes = [P, ctypes.POINTER(_Stats)]
+            L.wtet_check_quality.argtypes = [P, u32, P, P, u32, P, P]
+            L.wtet_encode_adjacency.argtypes = [u32, u32]
+            L.wtet_encode_adjacency.restype = u32
+            L.wtet_decode_adjacency.argtypes = [u32, P, P]
+            L.wtet_version.restype = ctypes.c_char_p
+            L.wtet_device_info.argtypes = [P, ctypes.c_char_p, u32]
+            L.wtet_set_profiling.argtypes = [P, ctypes.c_int]
+            L.wtet_kernel_times.argtypes = [P, u32, ctypes.POINTER(ctypes.c_char_p),
+                                            ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)]
+            L.wtet_predicates_host.argtypes = [ctypes.c_int, P, P, u32, ctypes.c_int, P]
+            L.wtet_predicates_device.argtypes = [P, ctypes.c_int, P, P, u32, ctypes.c_int, P, P]
+            L.wtet_repair.argtypes = [P, u32, ctypes.POINTER(ctypes.POINTER(u32)),
+                                      ctypes.POINTER(ctypes.POINTER(u32)), ctypes.POINTER(u32),
+                                      ctypes.c_int, P]
+            _lib = L
+        return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One HIP device context (stream + persistent device buffers)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        st = L.wtet_create(device, ctypes.byref(h))
+        if st != 0:
+            raise RuntimeError(f"wtet_create(device={device}) failed: {_STATUS.get(st, st)}")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().wtet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self) -> str:
+        return lib().wtet_last_error(self._h).decode()
+
+    def stats(self) -> dict:
+        s = _Stats()
+        lib().wtet_get_stats(self._h, ctypes.byref(s))
+        d = {k: getattr(s, k) for k, _ in _Stats._fields_ if k != "pred"}
+        d["pred"] = list(s.pred)
+        return d
+
+    def set_profiling(self, on: bool = True):
+        lib().wtet_set_profiling(self._h, int(on))
+
+    def kernel_times(self) -> dict:
+        """{kernel group: (launches, total ms)} accumulated since set_profiling."""
+        out = {}
+        for k in range(12):
+            nm = ctypes.c_char_p()
+            cnt = ctypes.c_uint32()
+            ms = ctypes.c_double()
+            lib().wtet_kernel_times(self._h, k, ctypes.byref(nm), ctypes.byref(cnt), ctypes.byref(ms))
+            out[nm.value.decode()] = (cnt.value, ms.value)
+        return out
+
+    def info(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        lib().wtet_device_info(self._h, buf, 512)
+        return buf.value.decode()
+
+    def _run(self, fn, ptr, n, cfg):
+        res = _Result()
+        c = cfg._c() if cfg is not None else Config()._c()
+        st = fn(self._h, ptr, n, ctypes.byref(c), ctypes.byref(res))
+        if st != 0:
+            msg = self.error()
+            if st in (1, 2):
+                raise ValueError(msg)
+            raise RuntimeError(f"wtet_delaunay failed ({_STATUS.get(st, st)}): {msg}")
+        try:
+            m = res.num_tets
+            tets = np.ctypeslib.as_array(res.tets, shape=(m * 4,)).copy().reshape(m, 4) if m else \
+                np.zeros((0, 4), np.uint32)
+            adj = np.ctypeslib.as_array(res.adjacency, shape=(m * 4,)).copy().reshape(m, 4) if m else \
+                np.zeros((0, 4), np.uint32)
+            k = res.num_failed
+            failed = np.ctypeslib.as_array(res.failed_verts, shape=(k,)).copy() if k else \
+                np.zeros((0,), np.uint32)
+        finally:
+            lib().wtet_result_free(ctypes.byref(res))
+        return DelaunayResult(tets, adj, failed, self.stats())
+
+    def delaunay(self, points: np.ndarray, config: "Config | None" = None) -> "DelaunayResult":
+        pts = _validate(points)
+        return self._run(lib().wtet_delaunay, _p(pts), len(pts), config)
+
+    def delaunay_device(self, ptr: int, n: int, config: "Config | None" = None) -> "DelaunayResult":
+        """Points already in device memory (n*3 f32 at ``ptr``), e.g. a torch
+        tensor's ``data_ptr()``; the caller synchronizes its own stream first."""
+        if n < 4:
+            raise ValueError(f"Need at least 4 points for 3D Delaunay, got {n}")
+        return self._run(lib().wtet_delaunay_device, ctypes.c_void_p(ptr), n, config)
+
+    def delaunay_resident(self, ptr: int, n: int, config: "Config | None" = None):
+        """Points and result in device memory.  Returns (num_tets, d_tets,
+        d_adjacency) device addresses owned by the context (valid until the
+        next call)."""
+        if n < 4:
+            raise ValueError(f"Need at least 4 points for 3D Delaunay, got {n}")
+        res = _DevResult()
+        c = config._c() if config is not None else Config()._c()
+        st = lib().wtet_delaunay_resident(self._h, ctypes.c_void_p(ptr), n, ctypes.byref(c),
+                                          ctypes.byref(res))
+        if st != 0:
+            raise RuntimeError(f"wtet_delaunay_resident failed ({_STATUS.get(st, st)}): {self.error()}")
+        return res.num_tets, res.d_tets, res.d_adjacency
+
+    def predicates(self, kind: int, pts: np.ndarray, ids: np.ndarray | None, mode: int = 0):
+        """Device predicate batch; returns (signs, stage counters)."""
+        pts = np.ascontiguousarray(pts, np.float32)
+        per = 4 if kind in (0, 2) else 5
+        n = pts.size // (3 * per)
+        out = np.empty(n, np.int32)
+        st8 = np.zeros(8, np.uint64)
+        idp = None
+        if ids is not None:
+            ids = np.ascontiguousarray(ids, np.uint32)
+            idp = _p(ids)
+        s = lib().wtet_predicates_device(self._h, kind, _p(pts), idp, n, mode, _p(out), _p(st8))
+        if s != 0:
+            raise RuntimeError(f"device predicates failed: {self.error()}")
+        return out, st8
+
+
+def predicates_host(kind: int, pts: np.ndarray, ids: np.ndarray | None, mode: int = 0):
+    pts = np.ascontiguousarray(pts, np.float32)
+    per = 4 if kind in (0, 2) else 5
+    n = pts.size // (3 * per)
+    out = np.empty(n, np.int32)
+    idp = None
+    if ids is not None:
+        ids = np.ascontiguousarray(ids, np.uint32)
+        idp = _p(ids)
+    if lib().wtet_predicates_host(kind, _p(pts), idp, n, mode, _p(out)) != 0:
+        raise ValueError("invalid predicate batch")
+    return out
+
+
+def repair(points_normalized: np.ndarray, tets: np.ndarray, adjacency: np.ndarray | None = None,
+           mode: int = 0):
+    """Host phase 2 on a valid triangulation (ids: points, super n..n+3,
+    infinity n+4): returns (tets, adjacency) of the exact Delaunay mesh."""
+    L = lib()
+    pts = np.ascontiguousarray(points_normalized, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tets, np.uint32).reshape(-1, 4)
+    tp = (ctypes.c_uint32 * t.size).from_buffer_copy(t.tobytes())
+    ap = None
+    if adjacency is not None:
+        a = np.ascontiguousarray(adjacency, np.uint32).reshape(-1, 4)
+        ap = (ctypes.c_uint32 * a.size).from_buffer_copy(a.tobytes())
+    tptr = ctypes.cast(tp, ctypes.POINTER(ctypes.c_uint32))
+    aptr = ctypes.cast(ap, ctypes.POINTER(ctypes.c_uint32)) if ap is not None else \
+        ctypes.POINTER(ctypes.c_uint32)()
+    m = ctypes.c_uint32(len(t))
+    regions = np.zeros(2, np.uint32)
+    st = L.wtet_repair(_p(pts), len(pts), ctypes.byref(tptr), ctypes.byref(aptr), ctypes.byref(m),
+                       mode, _p(regions))
+    if st != 0:
+        raise RuntimeError(f"wtet_repair failed ({_STATUS.get(st, st)})")
+    k = m.value
+    out_t = np.ctypeslib.as_array(tptr, shape=(k * 4,)).copy().reshape(k, 4)
+    out_a = np.ctypeslib.as_array(aptr, shape=(k * 4,)).copy().reshape(k, 4)
+    L.wtet_result_free(ctypes.byref(_Result(k, tptr, aptr, 0, ctypes.POINTER(ctypes.c_uint32)())))
+    return out_t, out_a, int(regions[0])
+
+
+def _validate(points) -> np.ndarray:
+    a = np.asarray(points)
+    if a.ndim != 2 or a.shape[1] != 3:
+        raise ValueError(f"Points must have shape (N, 3), got {list(a.shape)}")
+    if a.shape[0] < 4:
+        raise ValueError(f"Need at least 4 points for 3D Delaunay, got {a.shape[0]}")
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Config:
+    """Triangulation options (src/python.rs:53-120; GDelConfig src/types.rs:32-61).
+
+    ``predicate_mode`` ("filtered" | "exact") and ``collect_stats`` are
+    additions of this library."""
+
+    def __init__(self, insertion_rule: str = "circumcenter", enable_flipping: bool = True,
+                 enable_sorting: bool = False, enable_hilbert_sorting: bool = False,
+                 enable_splaying: bool = True, max_insert_iterations: int = 100,
+                 max_flip_iterations: int = 10, predicate_mode: str = "filtered",
+                 collect_stats: bool = False):
+        rule = str(insertion_rule).lower()
+        if rule not in ("circumcenter", "centroid"):
+            raise ValueError(
+                f"Invalid insertion_rule '{insertion_rule}'. Must be 'circumcenter' or 'centroid'.")
+        if predicate_mode not in ("filtered", "exact"):
+            raise ValueError(f"Invalid predicate_mode '{predicate_mode}'. Must be 'filtered' or 'exact'.")
+        self.insertion_rule = rule
+        self.enable_flipping = bool(enable_flipping)
+        self.enable_sorting = bool(enable_sorting)
+        self.enable_hilbert_sorting = bool(enable_hilbert_sorting)
+        self.enable_splaying = bool(enable_splaying)
+        self.max_insert_iterations = int(max_insert_iterations)
+        self.max_flip_iterations = int(max_flip_iterations)
+        self.predicate_mode = predicate_mode
+        self.collect_stats = bool(collect_stats)
+
+    def _c(self) -> _Config:
+        return _Config(0 if self.insertion_rule == "circumcenter" else 1, int(self.enable_flipping),
+                       int(self.enable_sorting), int(self.enable_hilbert_sorting),
+                       int(self.enable_splaying), self.max_insert_iterations,
+                       self.max_flip_iterations, 1 if self.predicate_mode == "exact" else 0,
+                       int(self.collect_stats))
+
+    def __repr__(self) -> str:
+        b = lambda v: "true" if v else "false"  # noqa: E731 (Rust bool formatting)
+        return (f"Config(insertion_rule='{self.insertion_rule}', flipping={b(self.enable_flipping)}, "
+                f"sorting={b(self.enable_sorting)}, hilbert={b(self.enable_hilbert_sorting)}, "
+                f"splaying={b(self.enable_splaying)}, max_insert_iter={self.max_insert_iterations}, "
+                f"max_flip_iter={self.max_flip_iterations})")
+
+
+class DelaunayResult:
+    """src/python.rs:126-160: ``tets`` (M,4) uint32, ``adjacency`` (M,4)
+    uint32 packed ``(tet << 5) | face``, ``failed_verts`` (K,) uint32."""
+
+    def __init__(self, tets, adjacency, failed_verts, stats=None):
+        self.tets = tets
+        self.adjacency = adjacency
+        self.failed_verts = failed_verts
+        self.stats = stats or {}
+
+    @property
+    def num_tets(self) -> int:
+        return int(self.tets.shape[0])
+
+    @property
+    def num_failed(self) -> int:
+        return int(self.failed_verts.shape[0])
+
+    def __repr__(self) -> str:
+        return f"DelaunayResult(num_tets={self.num_tets}, failed_verts={self.num_failed})"
+
+
+_default = None
+_default_lock = threading.Lock()
+
+
+def _ctx() -> Context:
+    global _default
+    with _default_lock:
+        if _default is None:
+            _default = Context(0)
+        return _default
+
+
+def delaunay(points, config: Config | None = None) -> DelaunayResult:
+    """3D Delaunay triangulation of (N, 3) float32 points (src/python.rs:166-250)."""
+    pts = _validate(points)
+    return _ctx().delaunay(pts, config)
+
+
+def check_quality(points, result: DelaunayResult):
+    """(violations, interior_tets) among all-real tet pairs (src/python.rs:282-339)."""
+    a = np.asarray(points)
+    if a.ndim != 2 or a.shape[1] != 3:
+        raise ValueError(f"Points must have shape (N, 3), got {list(a.shape)}")
+    pts = np.ascontiguousarray(a, np.float32)
+    t = np.ascontiguousarray(result.tets, np.uint32)
+    adj = np.ascontiguousarray(result.adjacency, np.uint32)
+    v = np.zeros(1, np.uint64)
+    i = np.zeros(1, np.uint64)
+    st = lib().wtet_check_quality(_p(pts), len(pts), _p(t), _p(adj), len(t), _p(v), _p(i))
+    if st != 0:
+        raise RuntimeError("check_quality: malformed adjacency")
+    return int(v[0]), int(i[0])
+
+
+def initialize_gpu() -> str:
+    """Create the default device context; returns a description (src/python.rs:344-353)."""
+    c = _ctx()
+    return "GPU initialized: " + c.info().replace("\n", "; ")
+
+
+def gpu_info() -> str:
+    """src/python.rs:358-371."""
+    if _default is None:
+        return "GPU not yet initialized. Call initialize_gpu() or run delaunay() to initialize."
+    return _default.info()
+
+
+def encode_adjacency(tet_idx: int, face_idx: int) -> int:
+    return ((int(tet_idx) << 5) | (int(face_idx) & 3)) & 0xFFFFFFFF
+
+
+def decode_adjacency(packed: int):
+    p = int(packed) & 0xFFFFFFFF
+    return (p >> 5, p & 3)

@@ -1,0 +1,826 @@
+// Host orchestration of the triangulation loop and the C ABI (include/wtet.h).
+//
+// One context owns a HIP stream and device buffers that persist across calls
+// and only grow.  A call runs: normalize + Morton sort on the device, then
+// insertion rounds (vote, pick, split, point split, flip iterations,
+// relocation) until every point is inserted, then flips until none is
+// possible, an exact audit of every face, compaction of the live tets, and
+// -- when the audit found non-Delaunay faces -- the host repair.
+// Reference loop: src/phase1.rs:16-1431; façade: src/lib.rs:160-310.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/wtet.h"
+#include "kernels.h"
+#include "repair.h"
+
+using namespace wtet;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct wtet_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  wtet_stats stats = {};
+  std::vector<DevBuf*> all;
+
+  // point preparation
+  DevBuf xyz_in, xyzn, keys, ids, keys2, ids2, sort_tmp, mm6, pts;
+  // mesh (per tet slot)
+  DevBuf tet, opp, tinfo, tvote, fvote, tsplit, fit, fround, tmsg, tfwd;
+  // lists
+  DevBuf uvert[2], utet[2], wlist, act[2], prop, flips, fout, blk, slot2idx, rec;
+  // counters
+  DevBuf ctr, gstats;
+  uint32_t* h_ctr = nullptr;  // pinned mirror of the counters
+  uint64_t cap_tets = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  // per-kernel timing with HIP events on the context's stream
+  // (src/profiler.rs GpuProfiler scopes, here actually recorded)
+  bool profiling = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
+  std::vector<int> evkind;
+  size_t evused = 0;
+  double ktime[16] = {};
+  uint32_t kcount[16] = {};
+
+  wtet_ctx() {
+    DevBuf* l[] = {&xyz_in, &xyzn,   &keys,   &ids,   &keys2,  &ids2,     &sort_tmp, &mm6,
+                   &pts,    &tet,    &opp,    &tinfo, &tvote,  &fvote,    &tsplit,   &fit,
+                   &fround, &tmsg,   &tfwd,   &uvert[0], &uvert[1], &utet[0], &utet[1],
+                   &wlist,  &act[0], &act[1], &prop,  &flips,  &fout,     &blk,      &slot2idx,
+                   &rec,    &ctr,    &gstats};
+    all.assign(std::begin(l), std::end(l));
+  }
+};
+
+namespace {
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define HIP_CHECK(x)                          \
+  do {                                        \
+    hipError_t e_ = (x);                      \
+    if (e_ != hipSuccess) throw HipError{e_, #x}; \
+  } while (0)
+
+struct Fail {
+  wtet_status st;
+  std::string msg;
+};
+
+void ensure(wtet_ctx* c, DevBuf& b, size_t bytes, bool keep = false) {
+  if (b.bytes >= bytes && b.p) return;
+  size_t nb = std::max(bytes, b.bytes + b.bytes / 2);
+  void* p = nullptr;
+  if (hipMalloc(&p, nb) != hipSuccess) {
+    (void)hipGetLastError();
+    throw Fail{WTET_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(nb) + " bytes failed"};
+  }
+  if (keep && b.p && b.bytes) HIP_CHECK(hipMemcpyAsync(p, b.p, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+  if (b.p) {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    HIP_CHECK(hipFree(b.p));
+  }
+  b.p = p;
+  b.bytes = nb;
+}
+
+template <typename T>
+T* P(DevBuf& b) {
+  return reinterpret_cast<T*>(b.p);
+}
+
+// per-tet-slot arrays sized for `cap` slots; keep=true preserves contents
+void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
+  if (cap <= c->cap_tets) return;
+  if (cap >= (1ull << 30)) throw Fail{WTET_ERR_OUT_OF_MEMORY, "more than 2^30 tet slots required"};
+  ensure(c, c->tet, cap * 16, keep);
+  ensure(c, c->opp, cap * 16, keep);
+  ensure(c, c->tinfo, cap * 4, keep);
+  ensure(c, c->tvote, cap * 8, keep);
+  ensure(c, c->fvote, cap * 8, keep);
+  ensure(c, c->tsplit, cap * 8, keep);
+  ensure(c, c->fit, cap * 4, keep);
+  ensure(c, c->fround, cap * 4, keep);
+  ensure(c, c->tmsg, cap * 16, keep);
+  ensure(c, c->tfwd, cap * 4, keep);
+  ensure(c, c->act[0], cap * 4, keep);
+  ensure(c, c->act[1], cap * 4, keep);
+  ensure(c, c->prop, cap * 16, false);
+  ensure(c, c->flips, cap * 16, false);
+  ensure(c, c->fout, cap * 16, false);
+  ensure(c, c->slot2idx, cap * 4, false);
+  ensure(c, c->blk, (cap / kBlock + 2) * 4, false);
+  if (keep) c->stats.grow_events++;
+  c->cap_tets = std::min<uint64_t>(c->tet.bytes / 16, (1ull << 30) - 1);
+}
+
+enum KernelKind : int {
+  K_PREP = 0, K_VOTE, K_PICK, K_SPLIT, K_SPLIT_POINTS, K_FLIP_CHECK, K_FLIP_DECIDE,
+  K_FLIP_EXEC, K_FLIP_UPDATE, K_RELOCATE, K_AUDIT, K_OUTPUT, K_NKINDS
+};
+const char* kKernelNames[K_NKINDS] = {"prepare_points", "vote", "pick", "split", "split_points",
+                                      "flip_check", "flip_decide", "flip_exec", "flip_update",
+                                      "relocate", "audit", "output"};
+
+// brackets the launches issued by fn with an event pair when profiling
+template <typename Fn>
+void timed(wtet_ctx* c, int kind, Fn fn) {
+  if (!c->profiling) {
+    fn();
+    return;
+  }
+  if (c->evused == c->evpool.size()) {
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    c->evpool.emplace_back(a, b);
+    c->evkind.push_back(0);
+  }
+  auto& pr = c->evpool[c->evused];
+  c->evkind[c->evused] = kind;
+  ++c->evused;
+  HIP_CHECK(hipEventRecord(pr.first, c->stream));
+  fn();
+  HIP_CHECK(hipEventRecord(pr.second, c->stream));
+}
+
+void collect_times(wtet_ctx* c) {
+  if (!c->profiling) return;
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < c->evused; ++i) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, c->evpool[i].first, c->evpool[i].second));
+    c->ktime[c->evkind[i]] += ms;
+    c->kcount[c->evkind[i]] += 1;
+  }
+  c->evused = 0;
+}
+
+Mesh mesh_of(wtet_ctx* c, uint32_t n, int mode) {
+  Mesh M;
+  M.pts = P<float4>(c->pts);
+  M.tet = P<uint4>(c->tet);
+  M.opp = P<uint4>(c->opp);
+  M.tinfo = P<uint32_t>(c->tinfo);
+  M.tvote = P<uint64_t>(c->tvote);
+  M.fvote = P<uint64_t>(c->fvote);
+  M.tsplit = P<uint2>(c->tsplit);
+  M.fit = P<uint32_t>(c->fit);
+  M.fround = P<uint32_t>(c->fround);
+  M.tmsg = P<uint4>(c->tmsg);
+  M.tfwd = P<uint32_t>(c->tfwd);
+  M.ctr = P<uint32_t>(c->ctr);
+  M.inf = n + 4;
+  M.mode = mode;
+  return M;
+}
+
+void read_ctr(wtet_ctx* c) {
+  HIP_CHECK(hipMemcpyAsync(c->h_ctr, c->ctr.p, C_COUNT * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+// The five starting tets (src/shaders/init.wgsl:87-148): the super tet
+// (n..n+3, raw orient3d < 0) and one tet joining each of its faces to the
+// infinity vertex n+4, so every face of the mesh has a neighbour.
+void upload_initial_tets(wtet_ctx* c, uint32_t n) {
+  uint32_t s[4] = {n, n + 1, n + 2, n + 3}, inf = n + 4;
+  std::vector<uint32_t> tv(5 * 4), to(5 * 4);
+  auto fv = [](int f, int k) { return face_vi(f, k); };
+  for (int k = 0; k < 4; ++k) tv[k] = s[k];
+  for (int f = 0; f < 4; ++f) {
+    // reversed face + infinity: the infinity tet sees the face from outside
+    uint32_t x = s[fv(f, 0)], y = s[fv(f, 1)], z = s[fv(f, 2)];
+    uint32_t* t = &tv[4 * (1 + f)];
+    t[0] = y; t[1] = x; t[2] = z; t[3] = inf;
+  }
+  // adjacency by face matching over the five tets
+  for (int t = 0; t < 5; ++t)
+    for (int f = 0; f < 4; ++f) {
+      uint32_t a[3];
+      int j = 0;
+      for (int v = 0; v < 4; ++v)
+        if (v != f) a[j++] = tv[4 * t + v];
+      std::sort(a, a + 3);
+      to[4 * t + f] = kInvalid;
+      for (int u = 0; u < 5 && to[4 * t + f] == kInvalid; ++u) {
+        if (u == t) continue;
+        for (int g = 0; g < 4; ++g) {
+          uint32_t b[3];
+          int i = 0;
+          for (int v = 0; v < 4; ++v)
+            if (v != g) b[i++] = tv[4 * u + v];
+          std::sort(b, b + 3);
+          if (a[0] == b[0] && a[1] == b[1] && a[2] == b[2]) {
+            to[4 * t + f] = enc_opp((uint32_t)u, (uint32_t)g);
+            break;
+          }
+        }
+      }
+      if (to[4 * t + f] == kInvalid) throw Fail{WTET_ERR_INTERNAL, "initial tets not closed"};
+    }
+  std::vector<uint32_t> info(5, kAlive), zero32(5, 0), zero64(10, 0);
+  std::vector<uint64_t> none(5, kVoteNone);
+  HIP_CHECK(hipMemcpyAsync(c->tet.p, tv.data(), 80, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->opp.p, to.data(), 80, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->tinfo.p, info.data(), 20, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->tvote.p, none.data(), 40, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->fvote.p, none.data(), 40, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->tsplit.p, zero64.data(), 40, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->fit.p, zero32.data(), 20, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->fround.p, zero32.data(), 20, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void upload_points_tail(wtet_ctx* c, uint32_t n) {
+  // super vertices src/gpu/mod.rs:152-162 and the infinity vertex; w = id
+  const float sv[5][3] = {{-100.f, -100.f, -100.f}, {400.f, -100.f, -100.f},
+                          {-100.f, 400.f, -100.f}, {-100.f, -100.f, 400.f},
+                          {1000.f, 1000.f, 1000.f}};
+  float4 tail[5];
+  for (int k = 0; k < 5; ++k) {
+    uint32_t id = n + (uint32_t)k;
+    float w;
+    std::memcpy(&w, &id, 4);
+    tail[k] = make_float4(sv[k][0], sv[k][1], sv[k][2], w);
+  }
+  HIP_CHECK(hipMemcpyAsync(P<float4>(c->pts) + n, tail, sizeof(tail), hipMemcpyHostToDevice,
+                           c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+struct Counters {
+  uint64_t sumU = 0, sumA = 0, sumF = 0;
+  uint32_t rounds = 0, iters = 0;
+};
+
+// One flip iteration over the active list act[ac] (length A).
+// Returns the flips executed; A becomes the next list's length.
+void collect_times(wtet_ctx* c);
+
+uint32_t flip_iteration(wtet_ctx* c, uint32_t n, int mode, uint32_t& ac, uint32_t& A,
+                        uint32_t& it, uint32_t round, Counters& K, uint64_t* st) {
+  if (c->evused > 4096) collect_times(c);
+  // worst case every proposal wins and takes one fresh slot
+  read_ctr(c);
+  uint64_t need = (uint64_t)c->h_ctr[C_TETS] + A + 64;
+  if (need > c->cap_tets) ensure_tets(c, need + need / 2, true);
+  Mesh M = mesh_of(c, n, mode);
+  ++it;
+  launch_set_ctr(M.ctr, C_ACT_NEXT, 0u, C_FLIPS, 0u, c->stream);
+  uint32_t* cur = P<uint32_t>(c->act[ac]);
+  uint32_t* nxt = P<uint32_t>(c->act[ac ^ 1]);
+  timed(c, K_FLIP_CHECK, [&] { launch_flip_check(M, cur, P<uint4>(c->prop), A, it, st, c->stream); });
+  timed(c, K_FLIP_DECIDE,
+        [&] { launch_flip_decide(M, P<uint4>(c->prop), A, it, P<uint4>(c->flips), nxt, c->stream); });
+  timed(c, K_FLIP_EXEC, [&] {
+    launch_flip_exec(M, P<uint4>(c->flips), A, it, round, P<uint4>(c->fout), nxt, c->stream);
+  });
+  timed(c, K_FLIP_UPDATE, [&] { launch_flip_update(M, P<uint4>(c->fout), A, it, c->stream); });
+  HIP_CHECK(hipGetLastError());
+  read_ctr(c);
+  uint32_t F = c->h_ctr[C_FLIPS];
+  K.sumA += A;
+  K.sumF += F;
+  K.iters++;
+  A = c->h_ctr[C_ACT_NEXT];
+  ac ^= 1;
+  return F;
+}
+
+void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& cfg, Counters& K,
+                std::vector<uint32_t>& out_tets, std::vector<uint32_t>& out_adj,
+                std::vector<uint32_t>& viol_rec, uint32_t& nviol, bool resident,
+                uint32_t& mout) {
+  hipStream_t s = c->stream;
+  int mode = cfg.predicate_mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
+  uint64_t* st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
+
+  ensure(c, c->xyzn, (size_t)n * 12);
+  ensure(c, c->keys, (size_t)n * 4);
+  ensure(c, c->ids, (size_t)n * 4);
+  ensure(c, c->keys2, (size_t)n * 4);
+  ensure(c, c->ids2, (size_t)n * 4);
+  ensure(c, c->sort_tmp, std::max<size_t>(sort_temp_bytes(n), 16));
+  ensure(c, c->mm6, 64);
+  ensure(c, c->pts, ((size_t)n + 5) * 16);
+  for (int b = 0; b < 2; ++b) {
+    ensure(c, c->uvert[b], (size_t)n * 4);
+    ensure(c, c->utet[b], (size_t)n * 4);
+  }
+  ensure(c, c->wlist, (size_t)n * 8);
+  ensure(c, c->ctr, C_COUNT * 4);
+  ensure(c, c->gstats, 64);
+  ensure(c, c->rec, (1u << 20) * 4);
+  ensure_tets(c, 7ull * n + 4096, false);
+
+  HIP_CHECK(hipEventRecord(c->ev0, s));
+  // points: normalize (f32, as the reference), Morton keys, sort, gather
+  uint32_t* mm6 = P<uint32_t>(c->mm6);
+  launch_fill_u32(mm6, 0xFFFFFFFFu, 3, s);
+  launch_fill_u32(mm6 + 3, 0u, 3, s);
+  launch_fill_u32(P<uint32_t>(c->ctr), 0u, C_COUNT, s);
+  launch_fill_u64(P<uint64_t>(c->gstats), 0ull, 8, s);
+  timed(c, K_PREP, [&] {
+    launch_minmax(d_xyz, n, mm6, s);
+    launch_normalize(d_xyz, n, mm6, P<float>(c->xyzn), P<uint32_t>(c->keys), P<uint32_t>(c->ids), s);
+    sort_pairs(P<uint32_t>(c->keys), P<uint32_t>(c->ids), P<uint32_t>(c->keys2),
+               P<uint32_t>(c->ids2), n, c->sort_tmp.p, c->sort_tmp.bytes, s);
+    launch_gather_points(P<float>(c->xyzn), P<uint32_t>(c->ids2), n, P<float4>(c->pts), s);
+  });
+  HIP_CHECK(hipGetLastError());
+  upload_points_tail(c, n);
+  upload_initial_tets(c, n);
+  launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u, -1, 0u, s);
+
+  // uninserted list: every point in the super tet (slot 0)
+  {
+    std::vector<uint32_t> iota(n);
+    for (uint32_t i = 0; i < n; ++i) iota[i] = i;
+    HIP_CHECK(hipMemcpyAsync(c->uvert[0].p, iota.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    launch_fill_u32(P<uint32_t>(c->utet[0]), 0u, n, s);
+  }
+
+  uint32_t U = n, cur = 0, ac = 0, A = 0, it = 0, round = 0;
+  uint32_t max_rounds = std::max<uint32_t>(cfg.max_insert_iterations, 1u);
+  uint32_t max_fi = std::max<uint32_t>(cfg.max_flip_iterations, 1u);
+  bool flipping = cfg.enable_flipping != 0;
+  while (U > 0) {
+    ++round;
+    if (round > std::max<uint32_t>(max_rounds, 4096u))
+      throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
+    Mesh M = mesh_of(c, n, mode);
+    timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
+    timed(c, K_PICK, [&] {
+      launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round,
+                  P<uint2>(c->wlist), P<uint32_t>(c->uvert[cur ^ 1]),
+                  P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), s);
+    });
+    HIP_CHECK(hipGetLastError());
+    read_ctr(c);
+    uint32_t W = c->h_ctr[C_WIN];
+    if (c->h_ctr[C_BADTET]) throw Fail{WTET_ERR_INTERNAL, "a point was located outside the super tet"};
+    if (W == 0) throw Fail{WTET_ERR_INTERNAL, "no insertion possible in a round"};
+    uint64_t need = (uint64_t)c->h_ctr[C_TETS] + 64;
+    if (need > c->cap_tets) {
+      ensure_tets(c, need + need / 2, true);
+      M = mesh_of(c, n, mode);
+    }
+    K.sumU += U;
+    K.rounds++;
+    // splits append their children after the active entries carried over
+    launch_set_ctr(M.ctr, C_ACT_NEXT, A, -1, 0u, s);
+    timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), W, round, P<uint32_t>(c->act[ac]), s); });
+    U -= W;
+    cur ^= 1;
+    timed(c, K_SPLIT_POINTS, [&] {
+      launch_split_points(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st, s);
+    });
+    HIP_CHECK(hipGetLastError());
+    read_ctr(c);
+    A = c->h_ctr[C_ACT_NEXT];
+    if (flipping) {
+      for (uint32_t k = 0; k < max_fi && A > 0; ++k)
+        if (flip_iteration(c, n, mode, ac, A, it, round, K, st) == 0) break;
+      if (U > 0) {
+        M = mesh_of(c, n, mode);
+        timed(c, K_RELOCATE, [&] {
+          launch_relocate(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st, s);
+        });
+        HIP_CHECK(hipGetLastError());
+      }
+    } else {
+      A = 0;
+    }
+  }
+  // finish flipping
+  if (flipping) {
+    uint32_t guard = 0;
+    while (A > 0) {
+      if (++guard > 100000) throw Fail{WTET_ERR_NOT_CONVERGED, "flipping did not terminate"};
+      if (flip_iteration(c, n, mode, ac, A, it, round, K, st) == 0) break;
+    }
+  }
+  // audit every finite face
+  read_ctr(c);
+  uint32_t T = c->h_ctr[C_TETS];
+  Mesh M = mesh_of(c, n, mode);
+  launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, s);
+  const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
+  timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, st, s); });
+  // compaction
+  timed(c, K_OUTPUT, [&] {
+    launch_output(M, T, n, P<uint32_t>(c->slot2idx), P<uint32_t>(c->blk), P<uint32_t>(c->prop),
+                  P<uint32_t>(c->fout), s);
+  });
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(c->ev1, s));
+  read_ctr(c);
+  collect_times(c);
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->stats.ms_device = ms;
+  c->stats.walk_failures = c->h_ctr[C_WALK_FAIL];
+  if (c->h_ctr[C_WALK_FAIL]) throw Fail{WTET_ERR_INTERNAL, "point relocation walk did not terminate"};
+  uint32_t Mout = c->h_ctr[C_ALIVE];
+  mout = Mout;
+  nviol = c->h_ctr[C_VIOL];
+  if (!resident || nviol > 0) {
+    out_tets.resize((size_t)Mout * 4);
+    out_adj.resize((size_t)Mout * 4);
+    HIP_CHECK(hipMemcpyAsync(out_tets.data(), c->prop.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(out_adj.data(), c->fout.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
+  }
+  uint32_t nrec = std::min(c->h_ctr[C_FAILREC], rec_cap);
+  viol_rec.resize(nrec);
+  if (nrec) {
+    // records are slot-based; map to output indices on the host
+    std::vector<uint32_t> s2i(T);
+    HIP_CHECK(hipMemcpyAsync(viol_rec.data(), c->rec.p, (size_t)nrec * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(s2i.data(), c->slot2idx.p, (size_t)T * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (auto& r : viol_rec) r = (s2i[r >> 2] << 2) | (r & 3u);
+  }
+  if (cfg.collect_stats)
+    HIP_CHECK(hipMemcpyAsync(c->stats.pred, c->gstats.p, 64, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->stats.tet_slots = T;
+}
+
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+namespace {
+
+wtet_status run_guarded(wtet_ctx* c, void (*fn)(void*), void* arg) {
+  try {
+    fn(arg);
+    return WTET_OK;
+  } catch (const Fail& f) {
+    if (c) c->err = f.msg;
+    return f.st;
+  } catch (const HipError& h) {
+    if (c) c->err = std::string("HIP error ") + hipGetErrorString(h.e) + " in " + h.what;
+    return WTET_ERR_DEVICE;
+  } catch (const std::bad_alloc&) {
+    if (c) c->err = "host allocation failed";
+    return WTET_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    if (c) c->err = "unexpected exception";
+    return WTET_ERR_INTERNAL;
+  }
+}
+
+struct DelaunayArgs {
+  wtet_ctx* c;
+  const float* xyz;
+  bool on_device;
+  uint32_t n;
+  wtet_config cfg;
+  wtet_result* out;           // host result, or
+  wtet_device_result* dout;   // device-resident result
+};
+
+void delaunay_impl(void* a_) {
+  DelaunayArgs& a = *static_cast<DelaunayArgs*>(a_);
+  wtet_ctx* c = a.c;
+  auto t0 = std::chrono::steady_clock::now();
+  HIP_CHECK(hipSetDevice(c->device));
+  std::memset(&c->stats, 0, sizeof(c->stats));
+  const float* d_xyz = a.xyz;
+  if (!a.on_device) {
+    ensure(c, c->xyz_in, (size_t)a.n * 12);
+    HIP_CHECK(hipMemcpyAsync(c->xyz_in.p, a.xyz, (size_t)a.n * 12, hipMemcpyHostToDevice, c->stream));
+    d_xyz = P<float>(c->xyz_in);
+  }
+  Counters K;
+  std::vector<uint32_t> tets, adj, rec;
+  uint32_t nviol = 0, mout = 0;
+  bool resident = a.dout != nullptr;
+  run_device(c, d_xyz, a.n, a.cfg, K, tets, adj, rec, nviol, resident, mout);
+  c->stats.violations_gpu = nviol;
+  // failed_verts: the vertex opposite each non-Delaunay real-only face
+  std::vector<uint32_t> failed;
+  for (uint32_t r : rec) {
+    if (tets.empty()) break;
+    uint32_t t = r >> 2, f = r & 3u;
+    uint32_t e = adj[4 * (size_t)t + f];
+    if (e == kInvalid) continue;
+    uint32_t u = e >> 5, g = e & 3u;
+    bool real = true;
+    for (int k = 0; k < 4; ++k)
+      real = real && tets[4 * (size_t)t + k] < a.n && tets[4 * (size_t)u + k] < a.n;
+    if (real) failed.push_back(tets[4 * (size_t)u + g]);
+  }
+  std::sort(failed.begin(), failed.end());
+  failed.erase(std::unique(failed.begin(), failed.end()), failed.end());
+  auto t1 = std::chrono::steady_clock::now();
+  if (nviol > 0 && a.cfg.enable_splaying) {
+    // host repair needs the normalized coordinates of the input ids
+    std::vector<float> xyzn((size_t)a.n * 3);
+    HIP_CHECK(hipMemcpy(xyzn.data(), c->xyzn.p, (size_t)a.n * 12, hipMemcpyDeviceToHost));
+    RepairReport rr;
+    wtet_status rs = repair_delaunay(xyzn.data(), a.n, tets, adj, rec, a.cfg.predicate_mode, &rr);
+    if (rs != WTET_OK) throw Fail{rs, "host repair failed: " + rr.message};
+    c->stats.repair_regions = rr.regions;
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  c->stats.ms_repair = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  c->stats.sum_uninserted = K.sumU;
+  c->stats.sum_active = K.sumA;
+  c->stats.sum_flips = K.sumF;
+  c->stats.rounds = K.rounds;
+  c->stats.flip_iterations = K.iters;
+  c->stats.num_points = a.n;
+  if (resident) {
+    if (nviol > 0) {  // upload the repaired mesh
+      mout = (uint32_t)(tets.size() / 4);
+      ensure(c, c->prop, std::max<size_t>(tets.size(), 4) * 4, false);
+      ensure(c, c->fout, std::max<size_t>(adj.size(), 4) * 4, false);
+      HIP_CHECK(hipMemcpyAsync(c->prop.p, tets.data(), tets.size() * 4, hipMemcpyHostToDevice, c->stream));
+      HIP_CHECK(hipMemcpyAsync(c->fout.p, adj.data(), adj.size() * 4, hipMemcpyHostToDevice, c->stream));
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    a.dout->num_tets = mout;
+    a.dout->d_tets = P<uint32_t>(c->prop);
+    a.dout->d_adjacency = P<uint32_t>(c->fout);
+    a.dout->num_failed = (uint32_t)failed.size();
+    c->stats.num_tets_out = mout;
+    c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return;
+  }
+  wtet_result* out = a.out;
+  out->num_tets = (uint32_t)(tets.size() / 4);
+  out->tets = (uint32_t*)std::malloc(std::max<size_t>(tets.size(), 1) * 4);
+  out->adjacency = (uint32_t*)std::malloc(std::max<size_t>(adj.size(), 1) * 4);
+  out->num_failed = (uint32_t)failed.size();
+  out->failed_verts = (uint32_t*)std::malloc(std::max<size_t>(failed.size(), 1) * 4);
+  if (!out->tets || !out->adjacency || !out->failed_verts) throw std::bad_alloc();
+  std::memcpy(out->tets, tets.data(), tets.size() * 4);
+  std::memcpy(out->adjacency, adj.data(), adj.size() * 4);
+  if (!failed.empty()) std::memcpy(out->failed_verts, failed.data(), failed.size() * 4);
+  c->stats.num_tets_out = out->num_tets;
+  c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+wtet_status wtet_create(int device, wtet_ctx** out) {
+  if (!out) return WTET_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) {
+    (void)hipGetLastError();
+    return WTET_ERR_DEVICE;
+  }
+  wtet_ctx* c = new (std::nothrow) wtet_ctx();
+  if (!c) return WTET_ERR_OUT_OF_MEMORY;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_ctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return WTET_ERR_DEVICE;
+  }
+  *out = c;
+  return WTET_OK;
+}
+
+void wtet_destroy(wtet_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : c->all)
+    if (b->p) (void)hipFree(b->p);
+  if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+  for (auto& pr : c->evpool) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* wtet_last_error(const wtet_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void wtet_config_default(wtet_config* cfg) {
+  if (!cfg) return;
+  cfg->insertion_rule = 0;
+  cfg->enable_flipping = 1;
+  cfg->enable_sorting = 0;
+  cfg->enable_hilbert_sorting = 0;
+  cfg->enable_splaying = 1;
+  cfg->max_insert_iterations = 100;
+  cfg->max_flip_iterations = 10;
+  cfg->predicate_mode = 0;
+  cfg->collect_stats = 0;
+}
+
+static wtet_status delaunay_common(wtet_ctx* c, const float* xyz, bool dev, uint32_t n,
+                                   const wtet_config* cfg, wtet_result* out,
+                                   wtet_device_result* dout) {
+  if (!c || (!out && !dout)) return WTET_ERR_INVALID_ARGUMENT;
+  if (out) std::memset(out, 0, sizeof(*out));
+  if (dout) std::memset(dout, 0, sizeof(*dout));
+  if (!xyz) {
+    c->err = "points pointer is null";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (n < 4) {
+    c->err = "Need at least 4 points for 3D Delaunay, got " + std::to_string(n);
+    return WTET_ERR_TOO_FEW_POINTS;
+  }
+  if (n > (1u << 26)) {
+    c->err = "too many points for one call";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  DelaunayArgs a;
+  a.c = c;
+  a.xyz = xyz;
+  a.on_device = dev;
+  a.n = n;
+  if (cfg) a.cfg = *cfg;
+  else wtet_config_default(&a.cfg);
+  if (a.cfg.insertion_rule != 0 && a.cfg.insertion_rule != 1) {
+    c->err = "Invalid insertion_rule. Must be 'circumcenter' or 'centroid'.";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  a.out = out;
+  a.dout = dout;
+  wtet_status s = run_guarded(c, delaunay_impl, &a);
+  if (s != WTET_OK) {
+    if (out) wtet_result_free(out);
+    if (dout) std::memset(dout, 0, sizeof(*dout));
+  }
+  return s;
+}
+
+wtet_status wtet_delaunay(wtet_ctx* c, const float* xyz, uint32_t n, const wtet_config* cfg,
+                          wtet_result* out) {
+  return delaunay_common(c, xyz, false, n, cfg, out, nullptr);
+}
+
+wtet_status wtet_delaunay_device(wtet_ctx* c, const float* d_xyz, uint32_t n,
+                                 const wtet_config* cfg, wtet_result* out) {
+  return delaunay_common(c, d_xyz, true, n, cfg, out, nullptr);
+}
+
+wtet_status wtet_delaunay_resident(wtet_ctx* c, const float* d_xyz, uint32_t n,
+                                   const wtet_config* cfg, wtet_device_result* out) {
+  if (!out) return WTET_ERR_INVALID_ARGUMENT;
+  return delaunay_common(c, d_xyz, true, n, cfg, nullptr, out);
+}
+
+void wtet_result_free(wtet_result* r) {
+  if (!r) return;
+  std::free(r->tets);
+  std::free(r->adjacency);
+  std::free(r->failed_verts);
+  std::memset(r, 0, sizeof(*r));
+}
+
+wtet_status wtet_get_stats(const wtet_ctx* c, wtet_stats* out) {
+  if (!c || !out) return WTET_ERR_INVALID_ARGUMENT;
+  *out = c->stats;
+  return WTET_OK;
+}
+
+uint32_t wtet_encode_adjacency(uint32_t tet, uint32_t face) { return (tet << 5) | (face & 3u); }
+
+void wtet_decode_adjacency(uint32_t packed, uint32_t* tet, uint32_t* face) {
+  if (tet) *tet = packed >> 5;
+  if (face) *face = packed & 3u;
+}
+
+const char* wtet_version(void) { return "0.1.0"; }
+
+wtet_status wtet_set_profiling(wtet_ctx* c, int on) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  c->profiling = on != 0;
+  std::memset(c->ktime, 0, sizeof(c->ktime));
+  std::memset(c->kcount, 0, sizeof(c->kcount));
+  return WTET_OK;
+}
+
+wtet_status wtet_kernel_times(const wtet_ctx* c, uint32_t kind, const char** name,
+                              uint32_t* launches, double* total_ms) {
+  if (!c || kind >= (uint32_t)K_NKINDS) return WTET_ERR_INVALID_ARGUMENT;
+  if (name) *name = kKernelNames[kind];
+  if (launches) *launches = c->kcount[kind];
+  if (total_ms) *total_ms = c->ktime[kind];
+  return WTET_OK;
+}
+
+wtet_status wtet_device_info(wtet_ctx* c, char* buf, uint32_t len) {
+  if (!buf || len == 0) return WTET_ERR_INVALID_ARGUMENT;
+  hipDeviceProp_t p;
+  int dev = c ? c->device : 0;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    std::snprintf(buf, len, "no HIP device");
+    return WTET_ERR_DEVICE;
+  }
+  std::snprintf(buf, len, "GPU Device: %s (%s)\nCompute units: %d\nGlobal memory: %.1f GiB", p.name,
+                p.gcnArchName, p.multiProcessorCount, p.totalGlobalMem / 1073741824.0);
+  return WTET_OK;
+}
+
+wtet_status wtet_predicates_host(int kind, const float* pts, const uint32_t* ids, uint32_t n,
+                                 int mode, int32_t* out) {
+  if (!pts || !out || (kind < 2 && !ids) || kind < 0 || kind > 3) return WTET_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (kind == 0) {
+      const float* q = pts + 12 * (size_t)i;
+      const uint32_t* k = ids + 4 * (size_t)i;
+      out[i] = orient3d_sos(q, q + 3, q + 6, q + 9, k[0], k[1], k[2], k[3], mode, nullptr);
+    } else if (kind == 1) {
+      const float* q = pts + 15 * (size_t)i;
+      const uint32_t* k = ids + 5 * (size_t)i;
+      out[i] = orient4d(q, q + 3, q + 6, q + 9, q + 12, k[0], k[1], k[2], k[3], k[4], mode, nullptr);
+    } else if (kind == 2) {
+      const float* q = pts + 12 * (size_t)i;
+      out[i] = orient3d_raw(q, q + 3, q + 6, q + 9, mode, nullptr);
+    } else {
+      const float* q = pts + 15 * (size_t)i;
+      out[i] = insphere_raw(q, q + 3, q + 6, q + 9, q + 12, mode, nullptr);
+    }
+  }
+  return WTET_OK;
+}
+
+struct PredArgs {
+  wtet_ctx* c;
+  int kind;
+  const float* pts;
+  const uint32_t* ids;
+  uint32_t n;
+  int mode;
+  int32_t* out;
+  uint64_t* stats8;
+};
+
+static void pred_device_impl(void* a_) {
+  PredArgs& a = *static_cast<PredArgs*>(a_);
+  wtet_ctx* c = a.c;
+  HIP_CHECK(hipSetDevice(c->device));
+  size_t per = (a.kind == 0 || a.kind == 2) ? 4 : 5;
+  float* dp = nullptr;
+  uint32_t* di = nullptr;
+  int32_t* dout = nullptr;
+  uint64_t* dst = nullptr;
+  HIP_CHECK(hipMalloc(&dp, std::max<size_t>(1, a.n) * per * 12));
+  HIP_CHECK(hipMalloc(&di, std::max<size_t>(1, a.n) * per * 4));
+  HIP_CHECK(hipMalloc(&dout, std::max<size_t>(1, a.n) * 4));
+  HIP_CHECK(hipMalloc(&dst, 64));
+  HIP_CHECK(hipMemcpy(dp, a.pts, (size_t)a.n * per * 12, hipMemcpyHostToDevice));
+  if (a.ids) HIP_CHECK(hipMemcpy(di, a.ids, (size_t)a.n * per * 4, hipMemcpyHostToDevice));
+  launch_fill_u64(dst, 0ull, 8, c->stream);
+  launch_pred_batch(a.kind, dp, di, a.n, a.mode, dout, dst, c->stream);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipMemcpy(a.out, dout, (size_t)a.n * 4, hipMemcpyDeviceToHost));
+  if (a.stats8) HIP_CHECK(hipMemcpy(a.stats8, dst, 64, hipMemcpyDeviceToHost));
+  (void)hipFree(dp);
+  (void)hipFree(di);
+  (void)hipFree(dout);
+  (void)hipFree(dst);
+}
+
+wtet_status wtet_predicates_device(wtet_ctx* c, int kind, const float* pts, const uint32_t* ids,
+                                   uint32_t n, int mode, int32_t* out, uint64_t* stats8) {
+  if (!c || !pts || !out || (kind < 2 && !ids) || kind < 0 || kind > 3)
+    return WTET_ERR_INVALID_ARGUMENT;
+  PredArgs a{c, kind, pts, ids, n, mode, out, stats8};
+  return run_guarded(c, pred_device_impl, &a);
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+// Launchers for the triangulation kernels (defined in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ops.h"
+
+namespace wtet {
+
+constexpr int kBlock = 256;  // 4 wavefronts of 64
+
+inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+// point preparation: normalization (src/lib.rs:313-337), Morton keys, gather
+void launch_minmax(const float* xyz, uint32_t n, uint32_t* mm6, hipStream_t s);
+void launch_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
+                      uint32_t* keys, uint32_t* ids, hipStream_t s);
+void launch_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts,
+                          hipStream_t s);
+void sort_pairs(uint32_t* keys, uint32_t* ids, uint32_t* keys_tmp, uint32_t* ids_tmp,
+                uint32_t n, void* temp, size_t temp_bytes, hipStream_t s);
+size_t sort_temp_bytes(uint32_t n);
+
+// rounds
+void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
+                 hipStream_t s);
+void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
+                 uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
+                 uint32_t* blk, hipStream_t s);
+void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round,
+                  uint32_t* act_next, hipStream_t s);
+void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
+                         uint32_t round, uint64_t* st, hipStream_t s);
+void launch_flip_check(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t A,
+                       uint32_t it, uint64_t* st, hipStream_t s);
+void launch_flip_decide(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it, uint4* flips,
+                        uint32_t* act_next, hipStream_t s);
+void launch_flip_exec(const Mesh& M, const uint4* flips, uint32_t F, uint32_t it,
+                      uint32_t round, uint4* out, uint32_t* act_next, hipStream_t s);
+void launch_flip_update(const Mesh& M, const uint4* out, uint32_t F, uint32_t it,
+                        hipStream_t s);
+void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
+                     uint32_t round, uint64_t* st, hipStream_t s);
+void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st,
+                  hipStream_t s);
+
+// output: compact alive tets, original vertex ids, (tet << 5) | face adjacency
+void launch_output(const Mesh& M, uint32_t T, uint32_t n, uint32_t* slot2idx, uint32_t* blk,
+                   uint32_t* out_tets, uint32_t* out_adj, hipStream_t s);
+
+// counters
+void launch_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2, hipStream_t s);
+
+// fills
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+void launch_fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
+
+// predicate batches (for checking the device predicates)
+void launch_pred_batch(int kind, const float* pts, const uint32_t* idx, uint32_t n, int mode,
+                       int32_t* out, uint64_t* st, hipStream_t s);
+
+}  // namespace wtet

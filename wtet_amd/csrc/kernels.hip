@@ -1,0 +1,446 @@
+// HIP kernels of the triangulation loop (gfx950, wave64).
+//
+// Element-wise kernels wrap the bodies of ops.h.  List compaction uses
+// wave ballots: each wavefront ranks its lanes with popcount(ballot & lt),
+// wave totals are combined in LDS, block totals are scanned by one block,
+// and a scatter pass writes every element once (stable order).
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "kernels.h"
+
+namespace wtet {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// Adds this thread's local counters to the global u64 counters (one atomic
+// per counter per wavefront).  Must be reached by every lane of the wave.
+__device__ __forceinline__ void flush_stats(const PredStats& ls, uint64_t* st) {
+  if (!st) return;
+  unsigned long long* g = reinterpret_cast<unsigned long long*>(st);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    uint32_t v = ls.c[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(g + k, (unsigned long long)v);
+  }
+}
+
+// exclusive rank of `flag` within the block; *total = block count.
+// Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* total) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  uint64_t m = __ballot(flag);
+  int lane = __lane_id();
+  int wave = threadIdx.x >> 6;
+  uint32_t r = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[wave] = __popcll(m);
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    off += w < wave ? wsum[w] : 0u;
+    tot += wsum[w];
+  }
+  *total = tot;
+  return off + r;
+}
+
+// in-place exclusive scan of blk[0..nb) by one 1024-thread block
+// mode 0: insertion round -> ctr[C_WIN] = total, reserve 3*total slots
+// mode 1: output          -> ctr[C_ALIVE] = total
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* blk, uint32_t nb, uint32_t* ctr,
+                                                     int mode) {
+  __shared__ uint32_t part[1024];
+  uint32_t tid = threadIdx.x;
+  uint32_t chunk = (nb + 1023u) / 1024u;
+  uint32_t b0 = tid * chunk, b1 = min(nb, b0 + chunk);
+  uint32_t s = 0;
+  for (uint32_t i = b0; i < b1; ++i) s += blk[i];
+  part[tid] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024u; off <<= 1) {
+    uint32_t v = tid >= off ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[tid] - s;  // exclusive prefix of this chunk
+  for (uint32_t i = b0; i < b1; ++i) {
+    uint32_t v = blk[i];
+    blk[i] = run;
+    run += v;
+  }
+  if (tid == 1023u) {
+    uint32_t total = part[1023];
+    if (mode == 0) {
+      ctr[C_WIN] = total;
+      ctr[C_BASE] = ctr[C_TETS];
+      ctr[C_TETS] += 3u * total;
+    } else {
+      ctr[C_ALIVE] = total;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// point preparation
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ord_f32(float f) {
+  uint32_t u = f32_bits(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t o) {
+  uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+__global__ void k_minmax(const float* xyz, uint32_t n, uint32_t* mm6) {
+  uint32_t i = gid();
+  uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+  if (i < n)
+    for (int k = 0; k < 3; ++k) {
+      float v = xyz[3 * (size_t)i + k];
+      if (v == v) {
+        uint32_t o = ord_f32(v);
+        lo[k] = o;
+        hi[k] = o;
+      }
+    }
+  for (int k = 0; k < 3; ++k) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[k] = min(lo[k], (uint32_t)__shfl_xor(lo[k], off));
+      hi[k] = max(hi[k], (uint32_t)__shfl_xor(hi[k], off));
+    }
+    if (__lane_id() == 0) {
+      atomicMin(&mm6[k], lo[k]);
+      atomicMax(&mm6[3 + k], hi[k]);
+    }
+  }
+}
+
+WT_HD uint32_t expand10(uint32_t v) {
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+#pragma clang fp contract(off)
+__global__ void k_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
+                            uint32_t* keys, uint32_t* ids) {
+  uint32_t i = gid();
+  if (i >= n) return;
+  float mn[3], ext[3];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = unord_f32(mm6[k]);
+    ext[k] = unord_f32(mm6[3 + k]) - mn[k];
+  }
+  float s = ext[0];
+  if (ext[1] > s) s = ext[1];
+  if (ext[2] > s) s = ext[2];
+  if (!(s >= 1e-10f)) s = 1e-10f;
+  uint32_t q[3];
+  for (int k = 0; k < 3; ++k) {
+    float v = (xyz[3 * (size_t)i + k] - mn[k]) / s;  // IEEE f32 sub and div
+    xyzn[3 * (size_t)i + k] = v;
+    float c = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+    q[k] = (uint32_t)(c * 1023.0f);
+  }
+  keys[i] = (expand10(q[2]) << 2) | (expand10(q[1]) << 1) | expand10(q[0]);
+  ids[i] = i;
+}
+#pragma clang fp contract(on)
+
+__global__ void k_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts) {
+  uint32_t j = gid();
+  if (j >= n) return;
+  uint32_t o = ids[j];
+  float w;
+  __builtin_memcpy(&w, &o, 4);
+  pts[j] = make_float4(xyzn[3 * (size_t)o], xyzn[3 * (size_t)o + 1], xyzn[3 * (size_t)o + 2], w);
+}
+
+void launch_minmax(const float* xyz, uint32_t n, uint32_t* mm6, hipStream_t s) {
+  k_minmax<<<nblocks(n), kBlock, 0, s>>>(xyz, n, mm6);
+}
+void launch_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
+                      uint32_t* keys, uint32_t* ids, hipStream_t s) {
+  k_normalize<<<nblocks(n), kBlock, 0, s>>>(xyz, n, mm6, xyzn, keys, ids);
+}
+void launch_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts,
+                          hipStream_t s) {
+  k_gather_points<<<nblocks(n), kBlock, 0, s>>>(xyzn, ids, n, pts);
+}
+
+size_t sort_temp_bytes(uint32_t n) {
+  size_t bytes = 0;
+  (void)rocprim::radix_sort_pairs((void*)nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                            (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 30);
+  return bytes;
+}
+
+void sort_pairs(uint32_t* keys, uint32_t* ids, uint32_t* keys_tmp, uint32_t* ids_tmp, uint32_t n,
+                void* temp, size_t temp_bytes, hipStream_t s) {
+  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, ids, ids_tmp, n, 0, 30, s);
+}
+
+// ---------------------------------------------------------------------------
+// insertion rounds
+// ---------------------------------------------------------------------------
+__global__ void k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U) {
+  uint32_t i = gid();
+  if (i < U) vote_body(M, uvert, utet, i);
+}
+
+__global__ void k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t* blk) {
+  uint32_t i = gid();
+  bool w = i < U && is_winner(M, utet, i);
+  uint32_t tot;
+  block_rank(w, &tot);
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+__global__ void k_pick_scatter(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
+                               uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
+                               const uint32_t* blk) {
+  uint32_t i = gid();
+  bool w = i < U && is_winner(M, utet, i);
+  uint32_t tot;
+  uint32_t r = blk[blockIdx.x] + block_rank(w, &tot);
+  if (i >= U) return;
+  uint32_t t = utet[i], v = uvert[i];
+  if (w) {
+    wlist[r] = make_uint2(t, v);
+    M.tsplit[t] = make_uint2(round, M.ctr[C_BASE] + 3u * r);
+  } else {
+    uvert2[i - r] = v;
+    utet2[i - r] = t;
+  }
+}
+
+void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
+                 hipStream_t s) {
+  k_vote<<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U);
+}
+
+void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
+                 uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2, uint32_t* blk,
+                 hipStream_t s) {
+  uint32_t nb = nblocks(U);
+  k_pick_count<<<nb, kBlock, 0, s>>>(M, utet, U, blk);
+  k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, M.ctr, 0);
+  k_pick_scatter<<<nb, kBlock, 0, s>>>(M, uvert, utet, U, round, wlist, uvert2, utet2, blk);
+}
+
+__global__ void k_split(Mesh M, const uint2* wlist, uint32_t W, uint32_t round,
+                        uint32_t* act_next) {
+  uint32_t k = gid();
+  if (k < W) split_body(M, wlist, k, round, act_next);
+}
+
+void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round,
+                  uint32_t* act_next, hipStream_t s) {
+  if (W) k_split<<<nblocks(W), kBlock, 0, s>>>(M, wlist, W, round, act_next);
+}
+
+__global__ void k_split_points(Mesh M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
+                               uint32_t round, uint64_t* st) {
+  uint32_t i = gid();
+  PredStats ls = {};
+  if (i < U) split_points_body(M, uvert, utet, i, round, &ls);
+  flush_stats(ls, st);
+}
+
+void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
+                         uint32_t round, uint64_t* st, hipStream_t s) {
+  if (U) k_split_points<<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st);
+}
+
+// ---------------------------------------------------------------------------
+// flips
+// ---------------------------------------------------------------------------
+__global__ void k_flip_check(Mesh M, const uint32_t* act, uint4* prop, uint32_t A, uint32_t it,
+                             uint64_t* st) {
+  uint32_t j = gid();
+  PredStats ls = {};
+  if (j < A) flip_check_body(M, act, prop, j, it, &ls);
+  flush_stats(ls, st);
+}
+
+__global__ void k_flip_decide(Mesh M, const uint4* prop, uint32_t A, uint32_t it, uint4* flips,
+                              uint32_t* act_next) {
+  uint32_t j = gid();
+  if (j < A) flip_decide_body(M, prop, j, it, flips, act_next);
+}
+
+// F is an upper bound (the active-list length); the flip count itself is
+// read from the device counter, so the host does not wait for it.
+__global__ void k_flip_exec(Mesh M, const uint4* flips, uint32_t F, uint32_t it, uint32_t round,
+                            uint4* out, uint32_t* act_next) {
+  uint32_t k = gid();
+  if (k < F && k < M.ctr[C_FLIPS]) flip_exec_body(M, flips, k, it, round, out, act_next);
+}
+
+__global__ void k_flip_update(Mesh M, const uint4* out, uint32_t F, uint32_t it) {
+  uint32_t k = gid();
+  if (k < F && k < M.ctr[C_FLIPS]) flip_update_body(M, out, k, it);
+}
+
+// ctr[idx] = v, as a kernel so that a captured graph never records a memset
+__global__ void k_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2) {
+  if (threadIdx.x == 0) {
+    ctr[idx] = v;
+    if (idx2 >= 0) ctr[idx2] = v2;
+  }
+}
+
+void launch_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2, hipStream_t s) {
+  k_set_ctr<<<1, 64, 0, s>>>(ctr, idx, v, idx2, v2);
+}
+
+void launch_flip_check(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t A, uint32_t it,
+                       uint64_t* st, hipStream_t s) {
+  if (A) k_flip_check<<<nblocks(A), kBlock, 0, s>>>(M, act, prop, A, it, st);
+}
+void launch_flip_decide(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it, uint4* flips,
+                        uint32_t* act_next, hipStream_t s) {
+  if (A) k_flip_decide<<<nblocks(A), kBlock, 0, s>>>(M, prop, A, it, flips, act_next);
+}
+void launch_flip_exec(const Mesh& M, const uint4* flips, uint32_t F, uint32_t it, uint32_t round,
+                      uint4* out, uint32_t* act_next, hipStream_t s) {
+  if (F) k_flip_exec<<<nblocks(F), kBlock, 0, s>>>(M, flips, F, it, round, out, act_next);
+}
+void launch_flip_update(const Mesh& M, const uint4* out, uint32_t F, uint32_t it,
+                        hipStream_t s) {
+  if (F) k_flip_update<<<nblocks(F), kBlock, 0, s>>>(M, out, F, it);
+}
+
+__global__ void k_relocate(Mesh M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
+                           uint32_t round, uint64_t* st) {
+  uint32_t i = gid();
+  PredStats ls = {};
+  if (i < U) relocate_body(M, uvert, utet, i, round, &ls);
+  flush_stats(ls, st);
+}
+
+void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
+                     uint32_t round, uint64_t* st, hipStream_t s) {
+  if (U) k_relocate<<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st);
+}
+
+__global__ void k_audit(Mesh M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st) {
+  uint32_t t = gid();
+  PredStats ls = {};
+  if (t < T) audit_body(M, t, rec, cap, &ls);
+  flush_stats(ls, st);
+}
+
+void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st,
+                  hipStream_t s) {
+  if (T) k_audit<<<nblocks(T), kBlock, 0, s>>>(M, T, rec, cap, st);
+}
+
+// ---------------------------------------------------------------------------
+// output
+// ---------------------------------------------------------------------------
+__global__ void k_alive_count(Mesh M, uint32_t T, uint32_t* blk) {
+  uint32_t t = gid();
+  bool a = t < T && (M.tinfo[t] & kAlive);
+  uint32_t tot;
+  block_rank(a, &tot);
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+__global__ void k_alive_scatter(Mesh M, uint32_t T, const uint32_t* blk, uint32_t* slot2idx) {
+  uint32_t t = gid();
+  bool a = t < T && (M.tinfo[t] & kAlive);
+  uint32_t tot;
+  uint32_t r = blk[blockIdx.x] + block_rank(a, &tot);
+  if (t < T) slot2idx[t] = a ? r : kInvalid;
+}
+
+__global__ void k_output_write(Mesh M, uint32_t T, uint32_t n, const uint32_t* slot2idx,
+                               uint32_t* out_tets, uint32_t* out_adj) {
+  uint32_t t = gid();
+  if (t >= T) return;
+  uint32_t r = slot2idx[t];
+  if (r == kInvalid) return;
+  uint4 tv = M.tet[t], to = M.opp[t];
+  uint4 ov, oa;
+  for (int k = 0; k < 4; ++k) {
+    uint32_t v = get4(tv, k);
+    set4(ov, k, v < n ? pt_rank(M.pts[v]) : v);
+    uint32_t e = get4(to, k);
+    uint32_t nr = slot2idx[opp_tet(e)];
+    set4(oa, k, nr == kInvalid ? kInvalid : (nr << 5) | opp_face(e));
+  }
+  reinterpret_cast<uint4*>(out_tets)[r] = ov;
+  reinterpret_cast<uint4*>(out_adj)[r] = oa;
+}
+
+void launch_output(const Mesh& M, uint32_t T, uint32_t n, uint32_t* slot2idx, uint32_t* blk,
+                   uint32_t* out_tets, uint32_t* out_adj, hipStream_t s) {
+  uint32_t nb = nblocks(T);
+  k_alive_count<<<nb, kBlock, 0, s>>>(M, T, blk);
+  k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, M.ctr, 1);
+  k_alive_scatter<<<nb, kBlock, 0, s>>>(M, T, blk, slot2idx);
+  if (out_tets) k_output_write<<<nb, kBlock, 0, s>>>(M, T, n, slot2idx, out_tets, out_adj);
+}
+
+// ---------------------------------------------------------------------------
+// fills and predicate batches
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (n) k_fill_u32<<<nblocks(n), kBlock, 0, s>>>(p, v, n);
+}
+void launch_fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s) {
+  if (n) k_fill_u64<<<nblocks(n), kBlock, 0, s>>>(p, v, n);
+}
+
+__global__ void k_pred_batch(int kind, const float* pts, const uint32_t* idx, uint32_t n, int mode,
+                             int32_t* out, uint64_t* st) {
+  uint32_t i = gid();
+  PredStats ls = {};
+  if (i < n) {
+    if (kind == 0) {
+      const float* q = pts + 12 * (size_t)i;
+      const uint32_t* k = idx + 4 * (size_t)i;
+      out[i] = orient3d_sos(q, q + 3, q + 6, q + 9, k[0], k[1], k[2], k[3], mode, &ls);
+    } else if (kind == 1) {
+      const float* q = pts + 15 * (size_t)i;
+      const uint32_t* k = idx + 5 * (size_t)i;
+      out[i] = orient4d(q, q + 3, q + 6, q + 9, q + 12, k[0], k[1], k[2], k[3], k[4], mode, &ls);
+    } else if (kind == 2) {
+      const float* q = pts + 12 * (size_t)i;
+      out[i] = orient3d_raw(q, q + 3, q + 6, q + 9, mode, &ls);
+    } else {
+      const float* q = pts + 15 * (size_t)i;
+      out[i] = insphere_raw(q, q + 3, q + 6, q + 9, q + 12, mode, &ls);
+    }
+  }
+  flush_stats(ls, st);
+}
+
+void launch_pred_batch(int kind, const float* pts, const uint32_t* idx, uint32_t n, int mode,
+                       int32_t* out, uint64_t* st, hipStream_t s) {
+  if (n) k_pred_batch<<<nblocks(n), kBlock, 0, s>>>(kind, pts, idx, n, mode, out, st);
+}
+
+}  // namespace wtet

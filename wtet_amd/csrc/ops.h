@@ -1,0 +1,469 @@
+// Per-element bodies of the insertion/flip round kernels.
+//
+// One call handles one element (an uninserted point, a winning tet, an
+// active tet, an executed flip).  The __global__ wrappers live in
+// kernels.hip.  The algorithm follows the reference's phase-1 loop
+// (src/phase1.rs:16-1431): vote -> 1-to-4 split -> point split -> flip
+// iterations (check, vote, flip, adjacency update) -> point relocation,
+// re-designed around device-resident lists instead of full-array scans.
+#pragma once
+
+#include "common.h"
+
+namespace wtet {
+
+// device-side counters (Mesh::ctr)
+enum Ctr : int {
+  C_TETS = 0,      // next free tet slot
+  C_ACT_NEXT = 1,  // length of the next active list
+  C_FLIPS = 2,     // winning flips this iteration
+  C_WALK_FAIL = 3, // relocation walks that hit the step limit
+  C_VIOL = 4,      // non-Delaunay faces found by the audit
+  C_WIN = 5,       // insertions this round
+  C_BASE = 6,      // first slot reserved for this round's splits
+  C_ALIVE = 7,     // alive tets (output compaction)
+  C_FAILREC = 8,   // recorded violation entries
+  C_BADTET = 9,    // points found in a tet touching infinity (must stay 0)
+  C_COUNT = 16
+};
+
+struct Mesh {
+  const float4* pts;  // n + 5 points (n real, 4 super, 1 infinity)
+  uint4* tet;         // vertex ids
+  uint4* opp;         // neighbours: (tet << 2) | face
+  uint32_t* tinfo;    // kAlive | kQueued | kDead
+  uint64_t* tvote;    // insertion vote (min key)
+  uint64_t* fvote;    // flip vote (min key, iteration-tagged)
+  uint2* tsplit;      // (round, first new slot) for tets split this round
+  uint32_t* fit;      // flip iteration that destroyed the tet held in the slot
+  uint32_t* fround;   // round of the last flip that touched the slot
+  uint4* tmsg;        // where each face of a destroyed tet went
+  uint32_t* tfwd;     // live neighbour of a slot freed by a 3-2 flip
+  uint32_t* ctr;      // Ctr counters
+  uint32_t inf;       // id of the infinity vertex
+  int mode;           // PredMode
+};
+
+WT_HD float3 P3(const Mesh& M, uint32_t v) {
+  float4 p = M.pts[v];
+  return make_float3(p.x, p.y, p.z);
+}
+
+WT_HD void load_pt(const Mesh& M, uint32_t v, float* out, uint32_t* rank) {
+  float4 p = M.pts[v];
+  out[0] = p.x;
+  out[1] = p.y;
+  out[2] = p.z;
+  *rank = pt_rank(p);
+}
+
+// orient3d_sos on four vertex ids
+WT_HD int o3(const Mesh& M, uint32_t a, uint32_t b, uint32_t c, uint32_t d, PredStats* st) {
+  float A[3], B[3], C[3], D[3];
+  uint32_t ra, rb, rc, rd;
+  load_pt(M, a, A, &ra);
+  load_pt(M, b, B, &rb);
+  load_pt(M, c, C, &rc);
+  load_pt(M, d, D, &rd);
+  return orient3d_sos(A, B, C, D, ra, rb, rc, rd, M.mode, st);
+}
+
+WT_HD uint32_t* opp_word(const Mesh& M, uint32_t t, uint32_t f) {
+  return reinterpret_cast<uint32_t*>(&M.opp[t]) + f;
+}
+
+// ---------------------------------------------------------------------------
+// vote: each uninserted point bids for its tet; the key prefers the point
+// deepest inside the circumsphere (equivalently nearest the circumcentre:
+// -insphere = 6|vol|(r^2 - d^2) for a raw-negative tet), ties to the
+// lowest list position.  src/shaders/vote.wgsl:89-281 (two passes there).
+// ---------------------------------------------------------------------------
+WT_HD void vote_body(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t i) {
+  uint32_t t = utet[i];
+  uint4 tv = M.tet[t];
+  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) {
+    at_add_u32(&M.ctr[C_BADTET], 1u);
+    return;
+  }
+  float a[3], b[3], c[3], d[3], p[3];
+  uint32_t r;
+  load_pt(M, tv.x, a, &r);
+  load_pt(M, tv.y, b, &r);
+  load_pt(M, tv.z, c, &r);
+  load_pt(M, tv.w, d, &r);
+  load_pt(M, uvert[i], p, &r);
+  double det = insphere_eval<double>(a, b, c, d, p).det;
+  double val = -det;
+  if (!(val > 0.0)) val = 0.0;
+  float fv = (float)val;
+  uint64_t key = ((uint64_t)(0xFFFFFFFFu - f32_bits(fv)) << 32) | i;
+  at_min_u64(&M.tvote[t], key);
+}
+
+WT_HD bool is_winner(const Mesh& M, const uint32_t* utet, uint32_t i) {
+  return (uint32_t)(M.tvote[utet[i]] & 0xFFFFFFFFull) == i;
+}
+
+// ---------------------------------------------------------------------------
+// 1-to-4 split of tet t at point p (src/shaders/split.wgsl:104-327).
+// Child j replaces vertex j by p, so it keeps t's orientation and holds
+// t's face j.  Child 0 reuses slot t, children 1..3 use base..base+2.
+// ---------------------------------------------------------------------------
+WT_HD uint32_t child_slot(uint32_t t, uint32_t base, uint32_t j) {
+  return j == 0 ? t : base + j - 1;
+}
+
+WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t round,
+                      uint32_t* act_next) {
+  uint32_t t = wlist[k].x, p = wlist[k].y;
+  uint32_t base = M.tsplit[t].y;
+  uint4 tv = M.tet[t];
+  uint4 to = M.opp[t];
+  uint32_t c[4] = {t, base, base + 1, base + 2};
+  for (int j = 0; j < 4; ++j) {
+    uint4 nv = tv;
+    set4(nv, j, p);
+    uint4 no;
+    for (int i = 0; i < 4; ++i)
+      if (i != j) set4(no, i, enc_opp(c[i], (uint32_t)j));
+    uint32_t e = get4(to, j);
+    uint32_t nt = opp_tet(e), nf = opp_face(e);
+    uint2 ns = M.tsplit[nt];
+    if (ns.x == round) {
+      set4(no, j, enc_opp(child_slot(nt, ns.y, nf), nf));
+    } else {
+      set4(no, j, e);
+      *opp_word(M, nt, nf) = enc_opp(c[j], (uint32_t)j);
+    }
+    M.tet[c[j]] = nv;
+    M.opp[c[j]] = no;
+    M.tvote[c[j]] = kVoteNone;
+    uint32_t old = 0u;
+    if (j == 0) {
+      old = at_or_u32(&M.tinfo[t], kQueued);
+    } else {  // fresh slot: initialise every per-slot field
+      M.tinfo[c[j]] = kAlive | kQueued;
+      M.fvote[c[j]] = kVoteNone;
+      M.tsplit[c[j]] = make_uint2(0u, 0u);
+      M.fit[c[j]] = 0u;
+      M.fround[c[j]] = 0u;
+    }
+    if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = c[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// move an uninserted point of a split tet into its child
+// (src/shaders/split_points.wgsl:222-335): three orient3d_sos tests.
+// ---------------------------------------------------------------------------
+WT_HD void split_points_body(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
+                             uint32_t round, PredStats* st) {
+  uint32_t t = utet[i];
+  uint2 sp = M.tsplit[t];
+  if (sp.x != round) return;
+  uint32_t base = sp.y;
+  uint4 c0 = M.tet[t];      // (p, v1, v2, v3)
+  uint32_t v0 = M.tet[base].x;  // child 1 = (v0, p, v2, v3)
+  uint32_t p = c0.x, v1 = c0.y, v2 = c0.z, v3 = c0.w;
+  uint32_t q = uvert[i];
+  // children 0 | 1 across plane (p, v2, v3): +1 -> v0's side -> not child 0
+  int a = o3(M, p, v3, v2, q, st) > 0 ? 1 : 0;
+  // children 2 | 3 across plane (v0, p, v1): +1 -> v2's side -> not child 2
+  int b = o3(M, v0, p, v1, q, st) > 0 ? 3 : 2;
+  // decide between a and b on their shared face (child a's face b)
+  uint32_t vv[4] = {v0, v1, v2, v3};
+  vv[a] = p;
+  int s = o3(M, vv[face_vi(b, 0)], vv[face_vi(b, 1)], vv[face_vi(b, 2)], q, st);
+  int child = s > 0 ? a : b;
+  utet[i] = child_slot(t, base, (uint32_t)child);
+}
+
+// ---------------------------------------------------------------------------
+// flip proposals (src/shaders/check_delaunay_{fast,exact}.wgsl, flip votes
+// src/shaders/mark_rejected_flips.wgsl:61-178)
+// proposal: x = t, y = face | kind << 2 | edge << 3, z = neighbour, w = third tet
+// kind 0: 2-3 flip, 1: 3-2 flip around edge (edge = rotation of the face triple)
+// ---------------------------------------------------------------------------
+WT_HD uint64_t flip_key(uint32_t it, uint32_t t) {
+  return ((uint64_t)(0xFFFFFFFFu - it) << 32) | t;
+}
+
+WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t j,
+                           uint32_t it, PredStats* st) {
+  uint32_t t = act[j];
+  uint32_t info = at_and_u32(&M.tinfo[t], ~kQueued);
+  prop[j].x = kInvalid;
+  if (!(info & kAlive)) return;
+  uint4 tv = M.tet[t];
+  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return;
+  uint4 to = M.opp[t];
+  float P[4][3];
+  uint32_t R[4];
+  for (int k = 0; k < 4; ++k) load_pt(M, get4(tv, k), P[k], &R[k]);
+  for (int f = 0; f < 4; ++f) {
+    uint32_t e = get4(to, f);
+    uint32_t nt = opp_tet(e), nf = opp_face(e);
+    uint32_t o = get4(M.tet[nt], (int)nf);
+    if (o == M.inf) continue;
+    float O[3];
+    uint32_t ro;
+    load_pt(M, o, O, &ro);
+    if (orient4d(P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, M.mode, st) > 0)
+      continue;
+    // o lies inside the circumsphere: classify the flip
+    int px = face_vi(f, 0), py = face_vi(f, 1), pz = face_vi(f, 2);
+    int pos[3] = {px, py, pz};
+    int bad = -1, nbad = 0;
+    for (int k = 0; k < 3; ++k) {
+      int u = pos[k], w = pos[(k + 1) % 3];
+      // edge (u, w): the line a-o should pass on the face side -> raw > 0
+      int s = orient3d_sos(P[u], P[w], P[f], O, R[u], R[w], R[f], ro, M.mode, st);
+      if (s > 0) {
+        bad = k;
+        ++nbad;
+      }
+    }
+    uint32_t kind, third = kInvalid;
+    if (nbad == 0) {
+      kind = 0;
+    } else if (nbad == 1) {
+      // 3-2 needs the edge to have degree 3: the tet across t's face
+      // opposite the third vertex must have o as its apex
+      int pthird = pos[(bad + 2) % 3];
+      uint32_t me = get4(to, pthird);
+      uint32_t m = opp_tet(me), mf = opp_face(me);
+      if (get4(M.tet[m], (int)mf) != o) continue;
+      kind = 1;
+      third = m;
+    } else {
+      continue;
+    }
+    uint64_t key = flip_key(it, t);
+    at_min_u64(&M.fvote[t], key);
+    at_min_u64(&M.fvote[nt], key);
+    if (kind == 1) at_min_u64(&M.fvote[third], key);
+    prop[j] = make_uint4(t, (uint32_t)f | kind << 2 | (uint32_t)(kind ? bad : 0) << 3, nt, third);
+    return;
+  }
+}
+
+WT_HD void flip_decide_body(const Mesh& M, const uint4* prop, uint32_t j, uint32_t it,
+                            uint4* flips, uint32_t* act_next) {
+  uint4 pr = prop[j];
+  if (pr.x == kInvalid) return;
+  uint64_t key = flip_key(it, pr.x);
+  bool win = M.fvote[pr.x] == key && M.fvote[pr.z] == key;
+  if (((pr.y >> 2) & 1u) && M.fvote[pr.w] != key) win = false;
+  if (win) {
+    flips[append_slot(&M.ctr[C_FLIPS])] = pr;
+  } else {
+    uint32_t old = at_or_u32(&M.tinfo[pr.x], kQueued);
+    if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = pr.x;
+  }
+}
+
+WT_HD void queue_tet(const Mesh& M, uint32_t s, uint32_t* act_next) {
+  uint32_t old = at_or_u32(&M.tinfo[s], kAlive | kQueued);
+  if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = s;
+}
+
+// ---------------------------------------------------------------------------
+// execute a winning flip (src/shaders/flip.wgsl:531-845).  New tets reuse
+// the old slots; a 2-3 flip takes one fresh slot, a 3-2 flip frees one.
+// External faces keep the old neighbour pointer; flip_update_body fixes
+// them once every flip of the iteration is written.
+// out: x,y,z = slots of the new tets (z unused for 3-2), w = kind
+// ---------------------------------------------------------------------------
+WT_HD void flip_exec_body(const Mesh& M, const uint4* flips, uint32_t k, uint32_t it,
+                          uint32_t round, uint4* out, uint32_t* act_next) {
+  uint4 pr = flips[k];
+  uint32_t t = pr.x, f = pr.y & 3u, kind = (pr.y >> 2) & 1u, edge = (pr.y >> 3) & 3u;
+  uint32_t n = pr.z;
+  uint4 tv = M.tet[t], to = M.opp[t];
+  uint4 nv = M.tet[n], no = M.opp[n];
+  uint32_t nf = opp_face(get4(to, (int)f));
+  uint32_t a = get4(tv, (int)f), o = get4(nv, (int)nf);
+  int pos[3] = {face_vi((int)f, 0), face_vi((int)f, 1), face_vi((int)f, 2)};
+  if (kind == 0) {
+    uint32_t x = get4(tv, pos[0]), y = get4(tv, pos[1]), z = get4(tv, pos[2]);
+    int qx = find4(nv, x), qy = find4(nv, y), qz = find4(nv, z);
+    uint32_t s = at_add_u32(&M.ctr[C_TETS], 1u);
+    uint32_t Txy = t, Tyz = n, Tzx = s;
+    M.tet[Txy] = make_uint4(x, y, o, a);
+    M.tet[Tyz] = make_uint4(y, z, o, a);
+    M.tet[Tzx] = make_uint4(z, x, o, a);
+    M.opp[Txy] = make_uint4(enc_opp(Tyz, 1), enc_opp(Tzx, 0), get4(to, pos[2]), get4(no, qz));
+    M.opp[Tyz] = make_uint4(enc_opp(Tzx, 1), enc_opp(Txy, 0), get4(to, pos[0]), get4(no, qx));
+    M.opp[Tzx] = make_uint4(enc_opp(Txy, 1), enc_opp(Tyz, 0), get4(to, pos[1]), get4(no, qy));
+    uint4 mt = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+    set4(mt, pos[2], enc_opp(Txy, 2));
+    set4(mt, pos[0], enc_opp(Tyz, 2));
+    set4(mt, pos[1], enc_opp(Tzx, 2));
+    uint4 mn = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+    set4(mn, qz, enc_opp(Txy, 3));
+    set4(mn, qx, enc_opp(Tyz, 3));
+    set4(mn, qy, enc_opp(Tzx, 3));
+    M.tmsg[t] = mt;
+    M.tmsg[n] = mn;
+    M.fit[t] = it;
+    M.fit[n] = it;
+    M.fround[t] = round;
+    M.fround[n] = round;
+    M.tvote[s] = kVoteNone;
+    M.fvote[s] = kVoteNone;
+    M.tsplit[s] = make_uint2(0u, 0u);
+    M.fit[s] = 0u;
+    M.fround[s] = 0u;
+    M.tinfo[s] = kAlive | kQueued;  // fresh slot, cannot be queued yet
+    queue_tet(M, Txy, act_next);
+    queue_tet(M, Tyz, act_next);
+    act_next[append_slot(&M.ctr[C_ACT_NEXT])] = Tzx;
+    out[k] = make_uint4(Txy, Tyz, Tzx, 0u);
+  } else {
+    // rotate the face triple so the offending edge is (x, y), third z
+    uint32_t x = get4(tv, pos[edge]), y = get4(tv, pos[(edge + 1) % 3]),
+             z = get4(tv, pos[(edge + 2) % 3]);
+    int px = pos[edge], py = pos[(edge + 1) % 3];
+    uint32_t m = pr.w;
+    uint4 mv = M.tet[m], mo = M.opp[m];
+    int qx = find4(nv, x), qy = find4(nv, y);
+    int mx = find4(mv, x), my = find4(mv, y);
+    uint32_t A = t, B = n;
+    M.tet[A] = make_uint4(a, z, o, x);
+    M.tet[B] = make_uint4(z, a, o, y);
+    M.opp[A] = make_uint4(get4(no, qy), get4(mo, my), get4(to, py), enc_opp(B, 3));
+    M.opp[B] = make_uint4(get4(mo, mx), get4(no, qx), get4(to, px), enc_opp(A, 3));
+    uint4 mt = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+    set4(mt, py, enc_opp(A, 2));
+    set4(mt, px, enc_opp(B, 2));
+    uint4 mn = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+    set4(mn, qy, enc_opp(A, 0));
+    set4(mn, qx, enc_opp(B, 1));
+    uint4 mm = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+    set4(mm, my, enc_opp(A, 1));
+    set4(mm, mx, enc_opp(B, 0));
+    M.tmsg[t] = mt;
+    M.tmsg[n] = mn;
+    M.tmsg[m] = mm;
+    M.fit[t] = it;
+    M.fit[n] = it;
+    M.fit[m] = it;
+    M.fround[t] = round;
+    M.fround[n] = round;
+    M.fround[m] = round;
+    M.tinfo[m] = kDead;
+    M.tfwd[m] = A;
+    queue_tet(M, A, act_next);
+    queue_tet(M, B, act_next);
+    out[k] = make_uint4(A, B, kInvalid, 1u);
+  }
+}
+
+WT_HD void fix_face(const Mesh& M, uint32_t T, uint32_t e, uint32_t it) {
+  uint32_t ptr = *opp_word(M, T, e);
+  uint32_t q = opp_tet(ptr), g = opp_face(ptr);
+  if (M.fit[q] == it) {
+    *opp_word(M, T, e) = get4(M.tmsg[q], (int)g);
+  } else {
+    *opp_word(M, q, g) = enc_opp(T, e);
+  }
+}
+
+// src/shaders/update_opp.wgsl:73-220
+WT_HD void flip_update_body(const Mesh& M, const uint4* out, uint32_t k, uint32_t it) {
+  uint4 r = out[k];
+  if (r.w == 0u) {
+    uint32_t T[3] = {r.x, r.y, r.z};
+    for (int i = 0; i < 3; ++i) {
+      fix_face(M, T[i], 2, it);
+      fix_face(M, T[i], 3, it);
+    }
+  } else {
+    uint32_t T[2] = {r.x, r.y};
+    for (int i = 0; i < 2; ++i)
+      for (uint32_t e = 0; e < 3; ++e) fix_face(M, T[i], e, it);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// relocate points whose tet a flip replaced: a stochastic visibility walk
+// (Devillers, Pion, Teillaud 2002) from the old slot.  Replaces the
+// reference's flip-trace replay (src/shaders/relocate_points_fast.wgsl).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kWalkLimit = 1u << 16;
+
+WT_HD uint32_t walk_to(const Mesh& M, uint32_t t, uint32_t q, uint32_t seed, PredStats* st,
+                       bool* ok) {
+  float Q[3];
+  uint32_t rq;
+  load_pt(M, q, Q, &rq);
+  for (uint32_t step = 0; step < kWalkLimit; ++step) {
+    uint4 tv = M.tet[t];
+    uint4 to = M.opp[t];
+    uint32_t r0 = hash32(seed + step * 0x9E3779B9u) & 3u;
+    bool moved = false;
+    for (uint32_t kk = 0; kk < 4; ++kk) {
+      int f = (int)((kk + r0) & 3u);
+      uint32_t x = get4(tv, face_vi(f, 0)), y = get4(tv, face_vi(f, 1)), z = get4(tv, face_vi(f, 2));
+      float X[3], Y[3], Z[3];
+      uint32_t rx, ry, rz;
+      load_pt(M, x, X, &rx);
+      load_pt(M, y, Y, &ry);
+      load_pt(M, z, Z, &rz);
+      if (orient3d_sos(X, Y, Z, Q, rx, ry, rz, rq, M.mode, st) < 0) {
+        t = opp_tet(get4(to, f));
+        moved = true;
+        break;
+      }
+    }
+    if (!moved) {
+      *ok = true;
+      return t;
+    }
+  }
+  *ok = false;
+  return t;
+}
+
+WT_HD void relocate_body(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
+                         uint32_t round, PredStats* st) {
+  uint32_t t = utet[i];
+  if (M.fround[t] != round) return;
+  for (int guard = 0; guard < 64 && (M.tinfo[t] & kDead); ++guard) t = M.tfwd[t];
+  bool ok = false;
+  t = walk_to(M, t, uvert[i], i * 0x85EBCA6Bu, st, &ok);
+  if (!ok) at_add_u32(&M.ctr[C_WALK_FAIL], 1u);
+  utet[i] = t;
+}
+
+// ---------------------------------------------------------------------------
+// audit: every finite face, counted once from its lower slot
+// (src/lib.rs:25-78 checks real-only pairs; this covers all finite faces)
+// rec: (t << 2 | f) of violations, capacity cap
+// ---------------------------------------------------------------------------
+WT_HD void audit_body(const Mesh& M, uint32_t t, uint32_t* rec, uint32_t cap, PredStats* st) {
+  if (!(M.tinfo[t] & kAlive)) return;
+  uint4 tv = M.tet[t];
+  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return;
+  uint4 to = M.opp[t];
+  float P[4][3];
+  uint32_t R[4];
+  for (int k = 0; k < 4; ++k) load_pt(M, get4(tv, k), P[k], &R[k]);
+  for (int f = 0; f < 4; ++f) {
+    uint32_t e = get4(to, f);
+    uint32_t nt = opp_tet(e), nf = opp_face(e);
+    if (nt < t) continue;
+    uint32_t o = get4(M.tet[nt], (int)nf);
+    if (o == M.inf) continue;
+    float O[3];
+    uint32_t ro;
+    load_pt(M, o, O, &ro);
+    if (orient4d(P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, M.mode, st) < 0) {
+      at_add_u32(&M.ctr[C_VIOL], 1u);
+      uint32_t s = at_add_u32(&M.ctr[C_FAILREC], 1u);
+      if (s < cap) rec[s] = (t << 2) | (uint32_t)f;
+    }
+  }
+}
+
+}  // namespace wtet

@@ -1,0 +1,657 @@
+// Staged geometric predicates for the triangulation kernels (host + device).
+//
+// Inputs are f32 coordinates (the normalized points and the four finite
+// super vertices).  Each sign is decided by the first stage that can certify
+// it:
+//   1. FP32 semi-static filter: the determinant in f32 against Shewchuk's
+//      stage-A bound (J. R. Shewchuk, DCG 18, 1997) with eps = 2^-24, plus an
+//      absolute slack and a magnitude floor so f32 underflow cannot fool it;
+//   2. FP64 filter: the same formula in f64 with eps = 2^-53 (f32 inputs are
+//      exact in f64);
+//   3. exact stage: every coordinate is an integer multiple of 2^E, where E
+//      is the lowest set-bit exponent over the coordinates involved, so the
+//      determinant is evaluated exactly in fixed-width two's-complement
+//      integers (32-bit limbs; 64-bit limb pairs when the coordinate span is
+//      <= 60 bits, 192-bit limbs otherwise).
+// When the exact determinant is zero, Simulation of Simplicity decides, with
+// the cascades of the reference (src/predicates.rs:106-208 for orient3d,
+// :478-675 for insphere): Edelsbrunner-Muecke perturbations ranked by point
+// id, whose non-lifted 2D steps are the plain f64 expression of
+// src/predicates.rs:73-75 (reproduced bit for bit, no fused multiply-add).
+//
+// Conventions (as in the reference):
+//   orient3d_raw(a,b,c,d) = sign det[a-d; b-d; c-d]
+//   insphere_raw          = sign of Shewchuk's insphere
+//   orient3d_sos -> +1 when the perturbed raw determinant is < 0, else -1
+//   orient4d     -> +1 (outside) / -1 (inside) for a tet with raw orient < 0
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WT_HD __host__ __device__ __forceinline__
+#define WT_HDN static __host__ __device__ __noinline__
+
+namespace wtet {
+
+enum PredMode : int { PRED_FILTERED = 0, PRED_EXACT_ONLY = 1 };
+
+// Per-stage resolution counters: [orient3d f32, f64, exact!=0, exact==0,
+//                                 insphere f32, f64, exact!=0, exact==0]
+struct PredStats {
+  uint32_t c[8];
+};
+
+WT_HD void stat_inc(PredStats* s, int k) {
+  if (s) s->c[k] += 1;
+}
+
+// ---------------------------------------------------------------------------
+// fixed-width signed integers, little-endian 32-bit limbs, two's complement
+// ---------------------------------------------------------------------------
+template <int N>
+struct BI {
+  uint32_t w[N];
+};
+
+template <int N>
+WT_HD BI<N> bi_zero() {
+  BI<N> r;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.w[i] = 0u;
+  return r;
+}
+
+template <int N>
+WT_HD bool bi_isneg(const BI<N>& a) {
+  return (a.w[N - 1] >> 31) != 0u;
+}
+
+template <int N>
+WT_HD int bi_sign(const BI<N>& a) {
+  if (bi_isneg(a)) return -1;
+  uint32_t o = 0u;
+#pragma unroll
+  for (int i = 0; i < N; ++i) o |= a.w[i];
+  return o ? 1 : 0;
+}
+
+template <int N>
+WT_HD BI<N> bi_add(const BI<N>& a, const BI<N>& b) {
+  BI<N> r;
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    uint64_t s = (uint64_t)a.w[i] + b.w[i] + c;
+    r.w[i] = (uint32_t)s;
+    c = s >> 32;
+  }
+  return r;
+}
+
+template <int N>
+WT_HD BI<N> bi_sub(const BI<N>& a, const BI<N>& b) {
+  BI<N> r;
+  uint64_t c = 1;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    uint64_t s = (uint64_t)a.w[i] + (uint32_t)~b.w[i] + c;
+    r.w[i] = (uint32_t)s;
+    c = s >> 32;
+  }
+  return r;
+}
+
+template <int N>
+WT_HD BI<N> bi_neg(const BI<N>& a) {
+  return bi_sub(bi_zero<N>(), a);
+}
+
+// sign-extend (or truncate) to M limbs
+template <int M, int N>
+WT_HD BI<M> bi_ext(const BI<N>& a) {
+  BI<M> r;
+  uint32_t e = bi_isneg(a) ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+  for (int i = 0; i < M; ++i) r.w[i] = i < N ? a.w[i] : e;
+  return r;
+}
+
+template <int A, int B>
+WT_HD BI<A + B> bi_mul(const BI<A>& a, const BI<B>& b) {
+  bool na = bi_isneg(a), nb = bi_isneg(b);
+  BI<A> ua = na ? bi_neg(a) : a;
+  BI<B> ub = nb ? bi_neg(b) : b;
+  BI<A + B> r = bi_zero<A + B>();
+#pragma unroll
+  for (int i = 0; i < A; ++i) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      uint64_t t = (uint64_t)ua.w[i] * ub.w[j] + r.w[i + j] + c;
+      r.w[i + j] = (uint32_t)t;
+      c = t >> 32;
+    }
+    r.w[i + B] = (uint32_t)c;
+  }
+  return (na != nb) ? bi_neg(r) : r;
+}
+
+// ---------------------------------------------------------------------------
+// f32 -> integer multiple of 2^E
+// ---------------------------------------------------------------------------
+struct F32Parts {
+  uint32_t m;  // significand (0 for zero)
+  int e;       // value = +-m * 2^e
+  bool neg;
+};
+
+WT_HD uint32_t f32_bits(float v) {
+  uint32_t u;
+  __builtin_memcpy(&u, &v, 4);
+  return u;
+}
+
+WT_HD F32Parts f32_parts(float v) {
+  uint32_t u = f32_bits(v);
+  F32Parts p;
+  p.neg = (u >> 31) != 0u;
+  uint32_t be = (u >> 23) & 0xFFu;
+  uint32_t fr = u & 0x7FFFFFu;
+  if (be == 0u) {
+    p.m = fr;
+    p.e = -149;
+  } else {
+    p.m = fr | 0x800000u;
+    p.e = (int)be - 150;
+  }
+  return p;
+}
+
+WT_HD int ctz32(uint32_t x) { return __builtin_ctz(x); }
+WT_HD int clz32(uint32_t x) { return __builtin_clz(x); }
+
+// lowest / highest set-bit exponents over k coordinates; returns false when
+// every coordinate is zero
+WT_HD bool span_of(const float* v, int k, int* lo, int* hi) {
+  int L = 100000, H = -100000;
+  for (int i = 0; i < k; ++i) {
+    F32Parts p = f32_parts(v[i]);
+    if (p.m == 0u) continue;
+    int l = p.e + ctz32(p.m);
+    int h = p.e + 31 - clz32(p.m);
+    L = l < L ? l : L;
+    H = h > H ? h : H;
+  }
+  *lo = L;
+  *hi = H;
+  return L != 100000;
+}
+
+// coordinate as a D-limb integer in units of 2^E (requires shift + 24 bits
+// to fit below the sign bit)
+template <int D>
+WT_HD BI<D> to_fixed(float v, int E) {
+  F32Parts p = f32_parts(v);
+  BI<D> r = bi_zero<D>();
+  if (p.m == 0u) return r;
+  int s = p.e - E;  // >= 0
+  int limb = s >> 5, bit = s & 31;
+  uint64_t x = (uint64_t)p.m << bit;
+  if (limb < D) r.w[limb] = (uint32_t)x;
+  if (limb + 1 < D) r.w[limb + 1] = (uint32_t)(x >> 32);
+  return p.neg ? bi_neg(r) : r;
+}
+
+// ---------------------------------------------------------------------------
+// exact determinants (templated on the coordinate width D in limbs)
+// ---------------------------------------------------------------------------
+template <int D>
+WT_HD int exact_orient3d_T(const float* P, int E) {
+  // P: 4 points x 3 coords; det[a-d; b-d; c-d]
+  BI<D> d[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      d[r][k] = bi_sub(to_fixed<D>(P[r * 3 + k], E), to_fixed<D>(P[9 + k], E));
+  BI<2 * D> bc = bi_sub(bi_mul(d[1][0], d[2][1]), bi_mul(d[2][0], d[1][1]));
+  BI<2 * D> ca = bi_sub(bi_mul(d[2][0], d[0][1]), bi_mul(d[0][0], d[2][1]));
+  BI<2 * D> ab = bi_sub(bi_mul(d[0][0], d[1][1]), bi_mul(d[1][0], d[0][1]));
+  BI<3 * D> det = bi_add(bi_add(bi_mul(d[0][2], bc), bi_mul(d[1][2], ca)),
+                         bi_mul(d[2][2], ab));
+  return bi_sign(det);
+}
+
+template <int D>
+WT_HD int exact_insphere_T(const float* P, int E) {
+  // P: 5 points; Shewchuk's insphere with a..d relative to e
+  BI<D> x[4], y[4], z[4];
+  BI<2 * D> l[4];
+  for (int r = 0; r < 4; ++r) {
+    x[r] = bi_sub(to_fixed<D>(P[r * 3 + 0], E), to_fixed<D>(P[12], E));
+    y[r] = bi_sub(to_fixed<D>(P[r * 3 + 1], E), to_fixed<D>(P[13], E));
+    z[r] = bi_sub(to_fixed<D>(P[r * 3 + 2], E), to_fixed<D>(P[14], E));
+    l[r] = bi_add(bi_add(bi_mul(x[r], x[r]), bi_mul(y[r], y[r])), bi_mul(z[r], z[r]));
+  }
+  // 2x2 minors on (x, y)
+  BI<2 * D> ab = bi_sub(bi_mul(x[0], y[1]), bi_mul(x[1], y[0]));
+  BI<2 * D> bc = bi_sub(bi_mul(x[1], y[2]), bi_mul(x[2], y[1]));
+  BI<2 * D> cd = bi_sub(bi_mul(x[2], y[3]), bi_mul(x[3], y[2]));
+  BI<2 * D> da = bi_sub(bi_mul(x[3], y[0]), bi_mul(x[0], y[3]));
+  BI<2 * D> ac = bi_sub(bi_mul(x[0], y[2]), bi_mul(x[2], y[0]));
+  BI<2 * D> bd = bi_sub(bi_mul(x[1], y[3]), bi_mul(x[3], y[1]));
+  // 3x3 minors with z
+  BI<3 * D> abc = bi_add(bi_sub(bi_mul(z[0], bc), bi_mul(z[1], ac)), bi_mul(z[2], ab));
+  BI<3 * D> bcd = bi_add(bi_sub(bi_mul(z[1], cd), bi_mul(z[2], bd)), bi_mul(z[3], bc));
+  BI<3 * D> cda = bi_add(bi_add(bi_mul(z[2], da), bi_mul(z[3], ac)), bi_mul(z[0], cd));
+  BI<3 * D> dab = bi_add(bi_add(bi_mul(z[3], ab), bi_mul(z[0], bd)), bi_mul(z[1], da));
+  BI<5 * D> det = bi_add(bi_sub(bi_mul(l[3], abc), bi_mul(l[2], dab)),
+                         bi_sub(bi_mul(l[1], cda), bi_mul(l[0], bcd)));
+  return bi_sign(det);
+}
+
+// det [[p0 p1 |p|^2 1]] over 4 points (coordinates already permuted)
+template <int D>
+WT_HD int exact_orient3d_lifted_T(const float* P, int E) {
+  BI<D> X[4], Y[4], Z[4];
+  BI<2 * D> W[4];
+  for (int r = 0; r < 4; ++r) {
+    X[r] = to_fixed<D>(P[r * 3 + 0], E);
+    Y[r] = to_fixed<D>(P[r * 3 + 1], E);
+    Z[r] = to_fixed<D>(P[r * 3 + 2], E);
+    W[r] = bi_add(bi_add(bi_mul(X[r], X[r]), bi_mul(Y[r], Y[r])), bi_mul(Z[r], Z[r]));
+  }
+  BI<D> dx[3], dy[3];
+  BI<2 * D> dw[3];
+  for (int r = 0; r < 3; ++r) {
+    dx[r] = bi_sub(X[r], X[3]);
+    dy[r] = bi_sub(Y[r], Y[3]);
+    dw[r] = bi_sub(W[r], W[3]);
+  }
+  BI<2 * D> bc = bi_sub(bi_mul(dx[1], dy[2]), bi_mul(dx[2], dy[1]));
+  BI<2 * D> ca = bi_sub(bi_mul(dx[2], dy[0]), bi_mul(dx[0], dy[2]));
+  BI<2 * D> ab = bi_sub(bi_mul(dx[0], dy[1]), bi_mul(dx[1], dy[0]));
+  BI<4 * D> det = bi_add(bi_add(bi_mul(dw[0], bc), bi_mul(dw[1], ca)), bi_mul(dw[2], ab));
+  return bi_sign(det);
+}
+
+// det [[p0 |p|^2 1]] over 3 points
+template <int D>
+WT_HD int exact_orient2d_lifted_T(const float* P, int E) {
+  BI<D> X[3], Y[3], Z[3];
+  BI<2 * D> W[3];
+  for (int r = 0; r < 3; ++r) {
+    X[r] = to_fixed<D>(P[r * 3 + 0], E);
+    Y[r] = to_fixed<D>(P[r * 3 + 1], E);
+    Z[r] = to_fixed<D>(P[r * 3 + 2], E);
+    W[r] = bi_add(bi_add(bi_mul(X[r], X[r]), bi_mul(Y[r], Y[r])), bi_mul(Z[r], Z[r]));
+  }
+  BI<3 * D> det = bi_sub(bi_mul(bi_sub(X[0], X[2]), bi_sub(W[1], W[2])),
+                         bi_mul(bi_sub(W[0], W[2]), bi_sub(X[1], X[2])));
+  return bi_sign(det);
+}
+
+template <int D>
+WT_HD int exact_orient1d_lifted_T(const float* P, int E) {
+  BI<2 * D> W[2];
+  for (int r = 0; r < 2; ++r) {
+    BI<D> X = to_fixed<D>(P[r * 3 + 0], E), Y = to_fixed<D>(P[r * 3 + 1], E),
+          Z = to_fixed<D>(P[r * 3 + 2], E);
+    W[r] = bi_add(bi_add(bi_mul(X, X), bi_mul(Y, Y)), bi_mul(Z, Z));
+  }
+  return bi_sign(bi_sub(W[0], W[1]));
+}
+
+// Narrow (64-bit) coordinates when the span fits, else 192-bit.  Inputs are
+// normalized points and super vertices, so the span is below 190 bits.
+constexpr int kNarrowSpan = 60;
+
+#define WT_EXACT_DISPATCH(fn, P, k)                 \
+  int lo_, hi_;                                     \
+  if (!span_of(P, k, &lo_, &hi_)) return 0;         \
+  if (hi_ - lo_ + 1 <= kNarrowSpan) return fn<2>(P, lo_); \
+  return fn<6>(P, lo_);
+
+WT_HDN int exact_orient3d(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_T, P, 12) }
+WT_HDN int exact_insphere(const float* P) { WT_EXACT_DISPATCH(exact_insphere_T, P, 15) }
+WT_HDN int exact_orient3d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_lifted_T, P, 12) }
+WT_HDN int exact_orient2d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient2d_lifted_T, P, 9) }
+WT_HDN int exact_orient1d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient1d_lifted_T, P, 6) }
+
+#undef WT_EXACT_DISPATCH
+
+// ---------------------------------------------------------------------------
+// floating-point filters (Shewchuk stage A)
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+WT_HD int fsign(float v) { return (v > 0.0f) - (v < 0.0f); }
+WT_HD int dsign(double v) { return (v > 0.0) - (v < 0.0); }
+
+// returns 2 when undecided
+WT_HD int orient3d_f32(const float* a, const float* b, const float* c, const float* d) {
+  float adx = a[0] - d[0], bdx = b[0] - d[0], cdx = c[0] - d[0];
+  float ady = a[1] - d[1], bdy = b[1] - d[1], cdy = c[1] - d[1];
+  float adz = a[2] - d[2], bdz = b[2] - d[2], cdz = c[2] - d[2];
+  float bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+  float cdxady = cdx * ady, adxcdy = adx * cdy;
+  float adxbdy = adx * bdy, bdxady = bdx * ady;
+  float det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+  float perm = (fabsf(bdxcdy) + fabsf(cdxbdy)) * fabsf(adz) +
+               (fabsf(cdxady) + fabsf(adxcdy)) * fabsf(bdz) +
+               (fabsf(adxbdy) + fabsf(bdxady)) * fabsf(cdz);
+  const float kB = 4.1724e-07f;  // (7 + 56*2^-24) * 2^-24, rounded up
+  if (!(perm >= 8.0e-28f)) return 2;  // 2^-90: keep f32 underflow out
+  float eb = kB * perm + 1.0e-36f;
+  if (det > eb) return 1;
+  if (-det > eb) return -1;
+  return 2;
+}
+
+WT_HD int orient3d_f64(const float* af, const float* bf, const float* cf, const float* df) {
+  double adx = (double)af[0] - df[0], bdx = (double)bf[0] - df[0], cdx = (double)cf[0] - df[0];
+  double ady = (double)af[1] - df[1], bdy = (double)bf[1] - df[1], cdy = (double)cf[1] - df[1];
+  double adz = (double)af[2] - df[2], bdz = (double)bf[2] - df[2], cdz = (double)cf[2] - df[2];
+  double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+  double cdxady = cdx * ady, adxcdy = adx * cdy;
+  double adxbdy = adx * bdy, bdxady = bdx * ady;
+  double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+  double perm = (fabs(bdxcdy) + fabs(cdxbdy)) * fabs(adz) +
+                (fabs(cdxady) + fabs(adxcdy)) * fabs(bdz) +
+                (fabs(adxbdy) + fabs(bdxady)) * fabs(cdz);
+  const double kB = 7.7715611723762e-16;  // (7 + 56*2^-53) * 2^-53, rounded up
+  double eb = kB * perm;
+  if (det > eb) return 1;
+  if (-det > eb) return -1;
+  return 2;
+}
+
+template <typename T>
+struct InsphereParts {
+  T det, perm;
+};
+
+template <typename T>
+WT_HD InsphereParts<T> insphere_eval(const float* a, const float* b, const float* c,
+                                     const float* d, const float* e) {
+  T aex = (T)a[0] - (T)e[0], bex = (T)b[0] - (T)e[0], cex = (T)c[0] - (T)e[0], dex = (T)d[0] - (T)e[0];
+  T aey = (T)a[1] - (T)e[1], bey = (T)b[1] - (T)e[1], cey = (T)c[1] - (T)e[1], dey = (T)d[1] - (T)e[1];
+  T aez = (T)a[2] - (T)e[2], bez = (T)b[2] - (T)e[2], cez = (T)c[2] - (T)e[2], dez = (T)d[2] - (T)e[2];
+  T aexbey = aex * bey, bexaey = bex * aey, ab = aexbey - bexaey;
+  T bexcey = bex * cey, cexbey = cex * bey, bc = bexcey - cexbey;
+  T cexdey = cex * dey, dexcey = dex * cey, cd = cexdey - dexcey;
+  T dexaey = dex * aey, aexdey = aex * dey, da = dexaey - aexdey;
+  T aexcey = aex * cey, cexaey = cex * aey, ac = aexcey - cexaey;
+  T bexdey = bex * dey, dexbey = dex * bey, bd = bexdey - dexbey;
+  T abc = aez * bc - bez * ac + cez * ab;
+  T bcd = bez * cd - cez * bd + dez * bc;
+  T cda = cez * da + dez * ac + aez * cd;
+  T dab = dez * ab + aez * bd + bez * da;
+  T alift = aex * aex + aey * aey + aez * aez;
+  T blift = bex * bex + bey * bey + bez * bez;
+  T clift = cex * cex + cey * cey + cez * cez;
+  T dlift = dex * dex + dey * dey + dez * dez;
+  InsphereParts<T> r;
+  r.det = (dlift * abc - clift * dab) + (blift * cda - alift * bcd);
+  T ez = aez < 0 ? -aez : aez, fz = bez < 0 ? -bez : bez, gz = cez < 0 ? -cez : cez,
+    hz = dez < 0 ? -dez : dez;
+#define WT_ABS(v) ((v) < 0 ? -(v) : (v))
+  T aexbeyp = WT_ABS(aexbey), bexaeyp = WT_ABS(bexaey);
+  T bexceyp = WT_ABS(bexcey), cexbeyp = WT_ABS(cexbey);
+  T cexdeyp = WT_ABS(cexdey), dexceyp = WT_ABS(dexcey);
+  T dexaeyp = WT_ABS(dexaey), aexdeyp = WT_ABS(aexdey);
+  T aexceyp = WT_ABS(aexcey), cexaeyp = WT_ABS(cexaey);
+  T bexdeyp = WT_ABS(bexdey), dexbeyp = WT_ABS(dexbey);
+#undef WT_ABS
+  r.perm = ((cexdeyp + dexceyp) * fz + (dexbeyp + bexdeyp) * gz + (bexceyp + cexbeyp) * hz) * alift +
+           ((dexaeyp + aexdeyp) * gz + (aexceyp + cexaeyp) * hz + (cexdeyp + dexceyp) * ez) * blift +
+           ((aexbeyp + bexaeyp) * hz + (bexdeyp + dexbeyp) * ez + (dexaeyp + aexdeyp) * fz) * clift +
+           ((bexceyp + cexbeyp) * ez + (cexaeyp + aexceyp) * fz + (aexbeyp + bexaeyp) * gz) * dlift;
+  return r;
+}
+
+WT_HD int insphere_f32(const float* a, const float* b, const float* c, const float* d,
+                       const float* e) {
+  InsphereParts<float> r = insphere_eval<float>(a, b, c, d, e);
+  const float kB = 9.5368e-07f;  // (16 + 224*2^-24) * 2^-24, rounded up
+  if (!(r.perm >= 8.271806e-25f)) return 2;  // 2^-80
+  float eb = kB * r.perm + 7.7e-34f;           // + 2^-110
+  if (r.det > eb) return 1;
+  if (-r.det > eb) return -1;
+  return 2;
+}
+
+WT_HD int insphere_f64(const float* a, const float* b, const float* c, const float* d,
+                       const float* e) {
+  InsphereParts<double> r = insphere_eval<double>(a, b, c, d, e);
+  const double kB = 1.7763568394003e-15;  // (16 + 224*2^-53) * 2^-53, rounded up
+  double eb = kB * r.perm;
+  if (r.det > eb) return 1;
+  if (-r.det > eb) return -1;
+  return 2;
+}
+
+// src/predicates.rs:73-75, bit for bit: four f64 subtractions, two products,
+// one subtraction, each rounded to nearest (contraction is off in this file)
+WT_HD double orient2d_plain(double a0, double a1, double b0, double b1, double c0, double c1) {
+  double t1 = a0 - c0;
+  double t2 = b1 - c1;
+  double t3 = a1 - c1;
+  double t4 = b0 - c0;
+  double p1 = t1 * t2;
+  double p2 = t3 * t4;
+  return p1 - p2;
+}
+
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------
+// staged raw signs
+// ---------------------------------------------------------------------------
+WT_HD int orient3d_raw(const float* a, const float* b, const float* c, const float* d, int mode,
+                       PredStats* st) {
+  if (mode == PRED_FILTERED) {
+    int s = orient3d_f32(a, b, c, d);
+    if (s != 2) {
+      stat_inc(st, 0);
+      return s;
+    }
+    s = orient3d_f64(a, b, c, d);
+    if (s != 2) {
+      stat_inc(st, 1);
+      return s;
+    }
+  }
+  float P[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
+  int s = exact_orient3d(P);
+  stat_inc(st, s != 0 ? 2 : 3);
+  return s;
+}
+
+WT_HD int insphere_raw(const float* a, const float* b, const float* c, const float* d,
+                       const float* e, int mode, PredStats* st) {
+  if (mode == PRED_FILTERED) {
+    int s = insphere_f32(a, b, c, d, e);
+    if (s != 2) {
+      stat_inc(st, 4);
+      return s;
+    }
+    s = insphere_f64(a, b, c, d, e);
+    if (s != 2) {
+      stat_inc(st, 5);
+      return s;
+    }
+  }
+  float P[15] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2],
+                 d[0], d[1], d[2], e[0], e[1], e[2]};
+  int s = exact_insphere(P);
+  stat_inc(st, s != 0 ? 6 : 7);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// SoS cascades (reference semantics)
+// ---------------------------------------------------------------------------
+WT_HD int sgn_diff(float a, float b) { return (a > b) - (a < b); }
+
+// src/predicates.rs:106-208 after the exact determinant vanished
+WT_HDN int orient3d_sos_only(const float* a, const float* b, const float* c, const float* d,
+                             uint32_t ia, uint32_t ib, uint32_t ic, uint32_t id) {
+  uint32_t v[4] = {ia, ib, ic, id};
+  const float* p[4] = {a, b, c, d};
+  int parity = 1;
+  for (int i = 0; i < 3; ++i) {
+    int mi = i;
+    for (int j = i + 1; j < 4; ++j)
+      if (v[j] < v[mi]) mi = j;
+    if (mi != i) {
+      uint32_t tv = v[i];
+      v[i] = v[mi];
+      v[mi] = tv;
+      const float* tp = p[i];
+      p[i] = p[mi];
+      p[mi] = tp;
+      parity = -parity;
+    }
+  }
+  // (point, point, point, axis u, axis w) for the orient2d depths
+  int depth = 0, r = 0;
+  for (; depth < 13; ++depth) {
+    int P0 = -1, P1 = 0, P2 = 0, U = 0, W = 0;
+    switch (depth) {
+      case 0: P0 = 1; P1 = 2; P2 = 3; U = 0; W = 1; break;
+      case 1: P0 = 1; P1 = 2; P2 = 3; U = 0; W = 2; break;
+      case 2: P0 = 1; P1 = 2; P2 = 3; U = 1; W = 2; break;
+      case 3: P0 = 0; P1 = 2; P2 = 3; U = 0; W = 1; break;
+      case 4: r = sgn_diff(p[2][0], p[3][0]); break;
+      case 5: r = sgn_diff(p[2][1], p[3][1]); break;
+      case 6: P0 = 0; P1 = 2; P2 = 3; U = 0; W = 2; break;
+      case 7: r = sgn_diff(p[2][2], p[3][2]); break;
+      case 8: P0 = 0; P1 = 2; P2 = 3; U = 1; W = 2; break;
+      case 9: P0 = 0; P1 = 1; P2 = 3; U = 0; W = 1; break;
+      case 10: r = sgn_diff(p[1][0], p[3][0]); break;
+      case 11: r = sgn_diff(p[1][1], p[3][1]); break;
+      default: r = sgn_diff(p[0][0], p[3][0]); break;
+    }
+    if (P0 >= 0)
+      r = dsign(orient2d_plain(p[P0][U], p[P0][W], p[P1][U], p[P1][W], p[P2][U], p[P2][W]));
+    if (r != 0) break;
+  }
+  if (depth == 13) r = 1;
+  if (depth == 1 || depth == 3 || depth == 5 || depth == 8 || depth == 10) r = -r;
+  r *= parity;
+  return r < 0 ? 1 : -1;
+}
+
+WT_HD int orient3d_sos(const float* a, const float* b, const float* c, const float* d,
+                       uint32_t ia, uint32_t ib, uint32_t ic, uint32_t id, int mode,
+                       PredStats* st) {
+  int s = orient3d_raw(a, b, c, d, mode, st);
+  if (s != 0) return s < 0 ? 1 : -1;
+  return orient3d_sos_only(a, b, c, d, ia, ib, ic, id);
+}
+
+// The insphere cascade as data: kind (3: 4-point, 2: 3-point, 1: 2-point,
+// 0: constant +1), lifted, coordinate order (0 xyz, 1 xzy, 2 yzx, 3 yxz,
+// 4 zxy), ranks of the points used.  Depths 1..49 of src/predicates.rs:513-652.
+struct SosStep {
+  uint8_t kind, lifted, perm, r0, r1, r2, r3;
+};
+
+WT_HD SosStep sos4_step(int depth) {
+  // packed: kind<<24 | lifted<<20 | perm<<16 | r0<<12 | r1<<8 | r2<<4 | r3
+  const uint32_t T[50] = {
+      0u,
+      0x3001234u, 0x3101234u, 0x3111234u, 0x3121234u, 0x3000234u, 0x2002340u,
+      0x2012340u, 0x2022340u, 0x3100234u, 0x2102340u, 0x2132340u, 0x3110234u,
+      0x2142340u, 0x3120234u, 0x3000134u, 0x2001340u, 0x2011340u, 0x2021340u,
+      0x2000340u, 0x1003400u, 0x1033400u, 0x2010340u, 0x1043400u, 0x2020340u,
+      0x3100134u, 0x2101340u, 0x2131340u, 0x2100340u, 0x1103400u, 0x2130340u,
+      0x3110134u, 0x2141340u, 0x2140340u, 0x3120134u, 0x3000124u, 0x2001240u,
+      0x2011240u, 0x2021240u, 0x2000240u, 0x1002400u, 0x1032400u, 0x2010240u,
+      0x1042400u, 0x2020240u, 0x2000140u, 0x1001400u, 0x1031400u, 0x1000400u,
+      0u};
+  uint32_t x = T[depth];
+  SosStep s;
+  s.kind = (uint8_t)(x >> 24);
+  s.lifted = (uint8_t)((x >> 20) & 0xF);
+  s.perm = (uint8_t)((x >> 16) & 0xF);
+  s.r0 = (uint8_t)((x >> 12) & 0xF);
+  s.r1 = (uint8_t)((x >> 8) & 0xF);
+  s.r2 = (uint8_t)((x >> 4) & 0xF);
+  s.r3 = (uint8_t)(x & 0xF);
+  return s;
+}
+
+WT_HD bool sos4_flip(int depth) {
+  // depths 1,3,7,9,11,14,15,16,18,20,22,23,26,30,31,32,37,39,41,44,46
+  const uint64_t mask = (1ull << 1) | (1ull << 3) | (1ull << 7) | (1ull << 9) | (1ull << 11) |
+                        (1ull << 14) | (1ull << 15) | (1ull << 16) | (1ull << 18) |
+                        (1ull << 20) | (1ull << 22) | (1ull << 23) | (1ull << 26) |
+                        (1ull << 30) | (1ull << 31) | (1ull << 32) | (1ull << 37) |
+                        (1ull << 39) | (1ull << 41) | (1ull << 44) | (1ull << 46);
+  return (mask >> depth) & 1ull;
+}
+
+WT_HD void perm_coords(const float* p, int perm, float* q) {
+  int i0 = 0, i1 = 1, i2 = 2;
+  switch (perm) {
+    case 1: i1 = 2; i2 = 1; break;
+    case 2: i0 = 1; i1 = 2; i2 = 0; break;
+    case 3: i0 = 1; i1 = 0; i2 = 2; break;
+    case 4: i0 = 2; i1 = 0; i2 = 1; break;
+    default: break;
+  }
+  q[0] = p[i0];
+  q[1] = p[i1];
+  q[2] = p[i2];
+}
+
+// src/predicates.rs:478-675 after the exact insphere vanished
+WT_HDN int orient4d_sos_only(const float* a, const float* b, const float* c, const float* d,
+                             const float* e, uint32_t ia, uint32_t ib, uint32_t ic,
+                             uint32_t id, uint32_t ie) {
+  uint32_t idx[5] = {ia, ib, ic, id, ie};
+  const float* pt[5] = {a, b, c, d, e};
+  int swaps = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 4; j > i; --j)
+      if (idx[j] < idx[j - 1]) {
+        uint32_t t = idx[j];
+        idx[j] = idx[j - 1];
+        idx[j - 1] = t;
+        const float* tp = pt[j];
+        pt[j] = pt[j - 1];
+        pt[j - 1] = tp;
+        ++swaps;
+      }
+  int depth = 1, r = 1;
+  for (; depth < 49; ++depth) {
+    SosStep s = sos4_step(depth);
+    float Q[12];
+    int rk[4] = {s.r0, s.r1, s.r2, s.r3};
+    for (int k = 0; k <= (int)s.kind; ++k) perm_coords(pt[rk[k]], s.perm, Q + 3 * k);
+    if (s.kind == 3) {
+      r = s.lifted ? exact_orient3d_lifted(Q) : exact_orient3d(Q);
+    } else if (s.kind == 2) {
+      r = s.lifted ? exact_orient2d_lifted(Q)
+                   : dsign(orient2d_plain(Q[0], Q[1], Q[3], Q[4], Q[6], Q[7]));
+    } else {
+      r = s.lifted ? exact_orient1d_lifted(Q) : sgn_diff(Q[0], Q[3]);
+    }
+    if (r != 0) break;
+  }
+  if (depth == 49) r = 1;
+  if (sos4_flip(depth)) r = -r;
+  if (swaps & 1) r = -r;
+  return r > 0 ? 1 : -1;
+}
+
+WT_HD int orient4d(const float* a, const float* b, const float* c, const float* d,
+                   const float* e, uint32_t ia, uint32_t ib, uint32_t ic, uint32_t id,
+                   uint32_t ie, int mode, PredStats* st) {
+  int s = insphere_raw(a, b, c, d, e, mode, st);
+  if (s != 0) return s;
+  return orient4d_sos_only(a, b, c, d, e, ia, ib, ic, id, ie);
+}
+
+}  // namespace wtet

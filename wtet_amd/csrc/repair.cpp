@@ -1,0 +1,574 @@
+// Host phase 2 (see repair.h) and the host quality check.
+//
+// Repair by region re-triangulation: around a non-Delaunay face take the
+// region R of tets near it, compute the Delaunay triangulation D of R's
+// vertices together with the four super vertices (Bowyer-Watson with the
+// same SoS predicates as the kernels), and when every boundary face of R is
+// a face of D, replace R by the tets of D that lie inside R.  Every Delaunay
+// tet of the whole point set with vertices in R is a tet of D, so once the
+// new boundary faces pass the insphere test the result is the triangulation
+// the Delaunay lemma certifies; otherwise R grows and the step repeats.  In
+// the limit R is the whole mesh and D the full triangulation, which is what
+// the reference's full_bw_reconstruction computes (src/phase2.rs:383-606).
+#include "repair.h"
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <deque>
+#include <unordered_map>
+#include <unordered_set>
+
+#include "common.h"
+
+namespace {
+
+using namespace wtet;
+
+struct PointSet {
+  const float* xyzn;
+  uint32_t n;
+  float sup[4][3];
+  const float* p(uint32_t v) const { return v < n ? xyzn + 3 * (size_t)v : sup[v - n]; }
+};
+
+int o3sos(const PointSet& S, uint32_t a, uint32_t b, uint32_t c, uint32_t d, int mode) {
+  return orient3d_sos(S.p(a), S.p(b), S.p(c), S.p(d), a, b, c, d, mode, nullptr);
+}
+
+int o4(const PointSet& S, const uint32_t* t, uint32_t q, int mode) {
+  return orient4d(S.p(t[0]), S.p(t[1]), S.p(t[2]), S.p(t[3]), S.p(q), t[0], t[1], t[2], t[3], q,
+                  mode, nullptr);
+}
+
+struct FaceKey {
+  uint32_t a, b, c;
+  bool operator==(const FaceKey& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct FaceHash {
+  size_t operator()(const FaceKey& k) const {
+    uint64_t h = k.a * 0x9E3779B97F4A7C15ull;
+    h ^= (k.b + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2));
+    h ^= (k.c * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2));
+    return (size_t)h;
+  }
+};
+FaceKey face_key(uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t a[3] = {x, y, z};
+  std::sort(a, a + 3);
+  return FaceKey{a[0], a[1], a[2]};
+}
+
+// ---------------------------------------------------------------------------
+// local Bowyer-Watson over a vertex subset plus the super vertices
+// ---------------------------------------------------------------------------
+struct Local {
+  std::vector<uint32_t> v;   // 4 per tet
+  std::vector<int64_t> a;    // 4 per tet: neighbour * 4 + face, -1 none
+  std::vector<uint8_t> alive;
+  std::vector<int64_t> mark;
+  std::vector<int64_t> freel;
+
+  int64_t make() {
+    if (!freel.empty()) {
+      int64_t t = freel.back();
+      freel.pop_back();
+      return t;
+    }
+    int64_t t = (int64_t)alive.size();
+    v.resize(v.size() + 4);
+    a.resize(a.size() + 4, -1);
+    alive.push_back(0);
+    mark.push_back(-1);
+    return t;
+  }
+};
+
+uint32_t morton10(const float* p) {
+  auto q = [](float x) {
+    float c = x < 0.f ? 0.f : (x > 1.f ? 1.f : x);
+    uint32_t v = (uint32_t)(c * 1023.0f);
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+  };
+  return (q(p[2]) << 2) | (q(p[1]) << 1) | q(p[0]);
+}
+
+bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Local& L,
+                    std::string& err) {
+  uint32_t n = S.n;
+  L = Local();
+  int64_t t0 = L.make();
+  uint32_t s[4] = {n, n + 1, n + 2, n + 3};
+  if (o3sos(S, s[0], s[1], s[2], s[3], mode) <= 0) std::swap(s[0], s[1]);
+  std::memcpy(&L.v[0], s, 16);
+  L.alive[t0] = 1;
+  std::sort(ids.begin(), ids.end(), [&](uint32_t x, uint32_t y) {
+    uint32_t mx = morton10(S.p(x)), my = morton10(S.p(y));
+    return mx != my ? mx < my : x < y;
+  });
+  int64_t last = t0;
+  uint32_t rng = 0x2545F491u;
+  std::vector<int64_t> cav, stack;
+  struct BF {
+    uint32_t f[3];
+    int64_t out;
+  };
+  std::vector<BF> bf;
+  std::unordered_map<uint64_t, std::pair<int64_t, int>> edges;
+  auto conflict = [&](int64_t t, uint32_t q) { return o4(S, &L.v[4 * t], q, mode) < 0; };
+  for (size_t it = 0; it < ids.size(); ++it) {
+    uint32_t q = ids[it];
+    // visibility walk
+    int64_t t = last;
+    bool found = false;
+    for (size_t step = 0; step < 4 * L.alive.size() + 64; ++step) {
+      rng = rng * 1664525u + 1013904223u;
+      int off = (int)(rng >> 30);
+      int moved = 0;
+      for (int k = 0; k < 4; ++k) {
+        int f = (k + off) & 3;
+        const uint32_t* tv = &L.v[4 * t];
+        if (o3sos(S, tv[face_vi(f, 0)], tv[face_vi(f, 1)], tv[face_vi(f, 2)], q, mode) < 0) {
+          int64_t nb = L.a[4 * t + f];
+          if (nb < 0) {
+            err = "walk left the super tetrahedron";
+            return false;
+          }
+          t = nb >> 2;
+          moved = 1;
+          break;
+        }
+      }
+      if (!moved) {
+        found = true;
+        break;
+      }
+    }
+    if (!found || !conflict(t, q)) {
+      t = -1;
+      for (int64_t u = 0; u < (int64_t)L.alive.size(); ++u)
+        if (L.alive[u] && conflict(u, q)) {
+          t = u;
+          break;
+        }
+      if (t < 0) {
+        err = "no conflicting tet for a point";
+        return false;
+      }
+    }
+    // cavity
+    int64_t stamp = (int64_t)it;
+    cav.clear();
+    stack.clear();
+    L.mark[t] = stamp;
+    cav.push_back(t);
+    stack.push_back(t);
+    while (!stack.empty()) {
+      int64_t u = stack.back();
+      stack.pop_back();
+      for (int f = 0; f < 4; ++f) {
+        int64_t nb = L.a[4 * u + f];
+        if (nb < 0) continue;
+        int64_t w = nb >> 2;
+        if (L.mark[w] == stamp || L.mark[w] == -2 - stamp) continue;
+        if (conflict(w, q)) {
+          L.mark[w] = stamp;
+          cav.push_back(w);
+          stack.push_back(w);
+        } else {
+          L.mark[w] = -2 - stamp;
+        }
+      }
+    }
+    bf.clear();
+    for (int64_t u : cav)
+      for (int f = 0; f < 4; ++f) {
+        int64_t nb = L.a[4 * u + f];
+        if (nb >= 0 && L.mark[nb >> 2] == stamp) continue;
+        BF b;
+        for (int k = 0; k < 3; ++k) b.f[k] = L.v[4 * u + face_vi(f, k)];
+        b.out = nb;
+        bf.push_back(b);
+      }
+    for (int64_t u : cav) {
+      L.alive[u] = 0;
+      L.freel.push_back(u);
+    }
+    edges.clear();
+    for (const BF& b : bf) {
+      uint32_t x = b.f[0], y = b.f[1], z = b.f[2];
+      if (o3sos(S, x, y, z, q, mode) <= 0) std::swap(x, y);
+      int64_t nt = L.make();
+      uint32_t* tv = &L.v[4 * nt];
+      tv[0] = x;
+      tv[1] = y;
+      tv[2] = z;
+      tv[3] = q;
+      L.alive[nt] = 1;
+      L.mark[nt] = -1;
+      L.a[4 * nt + 3] = b.out;
+      if (b.out >= 0) L.a[b.out] = 4 * nt + 3;
+      for (int f = 0; f < 3; ++f) {
+        uint32_t e0 = tv[(f + 1) % 3], e1 = tv[(f + 2) % 3];
+        uint64_t key = e0 < e1 ? ((uint64_t)e0 << 32 | e1) : ((uint64_t)e1 << 32 | e0);
+        auto itx = edges.find(key);
+        if (itx == edges.end()) {
+          edges.emplace(key, std::make_pair(nt, f));
+        } else {
+          L.a[4 * nt + f] = 4 * itx->second.first + itx->second.second;
+          L.a[4 * itx->second.first + itx->second.second] = 4 * nt + f;
+          edges.erase(itx);
+        }
+      }
+      last = nt;
+    }
+    if (!edges.empty()) {
+      err = "cavity boundary is not closed";
+      return false;
+    }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// the working mesh
+// ---------------------------------------------------------------------------
+struct Work {
+  std::vector<uint32_t> tv;  // 4 per tet
+  std::vector<int64_t> ta;   // 4 per tet: t * 4 + f, -1 none
+  std::vector<uint8_t> alive;
+  std::vector<uint32_t> inR;  // region stamp
+  uint32_t inf;
+  bool has_inf(int64_t t) const {
+    const uint32_t* v = &tv[4 * t];
+    return v[0] == inf || v[1] == inf || v[2] == inf || v[3] == inf;
+  }
+};
+
+bool face_bad(const PointSet& S, const Work& W, int64_t t, int f, int mode) {
+  int64_t nb = W.ta[4 * t + f];
+  if (nb < 0) return false;
+  int64_t u = nb >> 2;
+  if (W.has_inf(t) || W.has_inf(u)) return false;
+  uint32_t o = W.tv[4 * u + (nb & 3)];
+  return o4(S, &W.tv[4 * t], o, mode) < 0;
+}
+
+}  // namespace
+
+wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>& tets,
+                            std::vector<uint32_t>& adj, const std::vector<uint32_t>& viol,
+                            int mode, RepairReport* rep) {
+  PointSet S;
+  S.xyzn = xyzn;
+  S.n = n;
+  const float sv[4][3] = {{-100.f, -100.f, -100.f}, {400.f, -100.f, -100.f},
+                          {-100.f, 400.f, -100.f}, {-100.f, -100.f, 400.f}};
+  std::memcpy(S.sup, sv, sizeof(sv));
+  mode = mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
+
+  Work W;
+  W.inf = n + 4;
+  size_t M = tets.size() / 4;
+  W.tv = tets;
+  W.ta.resize(4 * M);
+  for (size_t i = 0; i < 4 * M; ++i)
+    W.ta[i] = adj[i] == kInvalid ? -1 : (int64_t)(adj[i] >> 5) * 4 + (adj[i] & 3u);
+  W.alive.assign(M, 1);
+  W.inR.assign(M, 0);
+
+  std::deque<std::pair<int64_t, int>> queue;
+  for (uint32_t r : viol) queue.emplace_back((int64_t)(r >> 2), (int)(r & 3u));
+
+  uint32_t stamp = 0;
+  std::vector<int64_t> R, ring;
+  while (!queue.empty()) {
+    auto [t, f] = queue.front();
+    queue.pop_front();
+    if (t >= (int64_t)W.alive.size() || !W.alive[t] || !face_bad(S, W, t, f, mode)) continue;
+    // seed region: the two tets of the face and their neighbours
+    R.clear();
+    ++stamp;
+    auto add = [&](int64_t x) {
+      if (x < 0 || !W.alive[x] || W.has_inf(x) || W.inR[x] == stamp) return;
+      W.inR[x] = stamp;
+      R.push_back(x);
+    };
+    add(t);
+    add(W.ta[4 * t + f] >> 2);
+    bool done = false;
+    while (!done) {
+      // one ring around R
+      ring.clear();
+      for (int64_t x : R)
+        for (int h = 0; h < 4; ++h)
+          if (W.ta[4 * x + h] >= 0) ring.push_back(W.ta[4 * x + h] >> 2);
+      for (int64_t x : ring) add(x);
+      if (R.size() * 2 > W.alive.size()) {
+        for (int64_t x = 0; x < (int64_t)W.alive.size(); ++x) add(x);
+      }
+      // vertices
+      std::vector<uint32_t> ids;
+      for (int64_t x : R)
+        for (int k = 0; k < 4; ++k) ids.push_back(W.tv[4 * x + k]);
+      std::sort(ids.begin(), ids.end());
+      ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+      std::vector<uint32_t> real;
+      for (uint32_t v : ids)
+        if (v < n) real.push_back(v);
+      Local L;
+      std::string err;
+      if (!local_delaunay(S, real, mode, L, err)) {
+        if (rep) rep->message = err;
+        return WTET_ERR_INTERNAL;
+      }
+      // face map of D
+      std::unordered_map<FaceKey, std::array<int64_t, 2>, FaceHash> fmap;
+      for (int64_t d = 0; d < (int64_t)L.alive.size(); ++d) {
+        if (!L.alive[d]) continue;
+        for (int h = 0; h < 4; ++h) {
+          const uint32_t* dv = &L.v[4 * d];
+          FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
+          auto it = fmap.find(k);
+          if (it == fmap.end()) fmap.emplace(k, std::array<int64_t, 2>{4 * d + h, -1});
+          else it->second[1] = 4 * d + h;
+        }
+      }
+      // boundary faces of R and their inner D tets
+      std::unordered_map<FaceKey, int64_t, FaceHash> bface;  // key -> outside (t*4+f) or -1
+      std::vector<int64_t> seeds;
+      bool missing = false;
+      std::vector<int64_t> grow_with;
+      for (int64_t x : R)
+        for (int h = 0; h < 4; ++h) {
+          int64_t nb = W.ta[4 * x + h];
+          if (nb >= 0 && W.inR[nb >> 2] == stamp) continue;
+          const uint32_t* xv = &W.tv[4 * x];
+          uint32_t fa = xv[face_vi(h, 0)], fb = xv[face_vi(h, 1)], fc = xv[face_vi(h, 2)];
+          FaceKey k = face_key(fa, fb, fc);
+          auto it = fmap.find(k);
+          if (it == fmap.end()) {
+            missing = true;
+            if (nb >= 0) grow_with.push_back(nb >> 2);
+            continue;
+          }
+          bface.emplace(k, nb);
+          // the D tet on R's side: its apex lies on the inner side of (fa, fb, fc)
+          for (int s2 = 0; s2 < 2; ++s2) {
+            int64_t df = it->second[s2];
+            if (df < 0) continue;
+            uint32_t apex = L.v[4 * (df >> 2) + (df & 3)];
+            if (o3sos(S, fa, fb, fc, apex, mode) > 0) seeds.push_back(df >> 2);
+          }
+        }
+      if (missing) {
+        for (int64_t x : grow_with) add(x);
+        continue;
+      }
+      // flood the inside of R in D
+      std::unordered_map<int64_t, int64_t> inner;  // D tet -> new mesh tet
+      std::vector<int64_t> st = seeds;
+      for (int64_t d : seeds) inner.emplace(d, -1);
+      while (!st.empty()) {
+        int64_t d = st.back();
+        st.pop_back();
+        for (int h = 0; h < 4; ++h) {
+          const uint32_t* dv = &L.v[4 * d];
+          FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
+          if (bface.count(k)) continue;
+          int64_t nb = L.a[4 * d + h];
+          if (nb < 0) {
+            if (rep) rep->message = "region flood reached the hull";
+            return WTET_ERR_INTERNAL;
+          }
+          if (inner.emplace(nb >> 2, -1).second) st.push_back(nb >> 2);
+        }
+      }
+      // replace R by the inner tets of D
+      for (int64_t x : R) W.alive[x] = 0;
+      for (auto& kv : inner) {
+        int64_t nt = (int64_t)W.alive.size();
+        kv.second = nt;
+        W.tv.insert(W.tv.end(), L.v.begin() + 4 * kv.first, L.v.begin() + 4 * kv.first + 4);
+        W.ta.insert(W.ta.end(), 4, -1);
+        W.alive.push_back(1);
+        W.inR.push_back(0);
+      }
+      for (auto& kv : inner) {
+        int64_t d = kv.first, nt = kv.second;
+        for (int h = 0; h < 4; ++h) {
+          const uint32_t* dv = &L.v[4 * d];
+          FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
+          auto bf = bface.find(k);
+          if (bf != bface.end()) {
+            int64_t out = bf->second;
+            W.ta[4 * nt + h] = out;
+            if (out >= 0) W.ta[out] = 4 * nt + h;
+            queue.emplace_back(nt, h);
+          } else {
+            int64_t nb = L.a[4 * d + h];
+            int64_t u = inner.at(nb >> 2);
+            W.ta[4 * nt + h] = 4 * u + (nb & 3);
+          }
+        }
+      }
+      if (rep) {
+        rep->regions++;
+        rep->max_region = std::max<uint32_t>(rep->max_region, (uint32_t)R.size());
+      }
+      done = true;
+    }
+  }
+  // compact
+  std::vector<int64_t> idx(W.alive.size(), -1);
+  int64_t m = 0;
+  for (size_t i = 0; i < W.alive.size(); ++i)
+    if (W.alive[i]) idx[i] = m++;
+  tets.assign(4 * (size_t)m, 0);
+  adj.assign(4 * (size_t)m, kInvalid);
+  for (size_t i = 0; i < W.alive.size(); ++i) {
+    if (!W.alive[i]) continue;
+    int64_t j = idx[i];
+    for (int h = 0; h < 4; ++h) {
+      tets[4 * j + h] = W.tv[4 * i + h];
+      int64_t nb = W.ta[4 * i + h];
+      if (nb >= 0 && idx[nb >> 2] >= 0) adj[4 * j + h] = ((uint32_t)idx[nb >> 2] << 5) | (uint32_t)(nb & 3);
+    }
+  }
+  return WTET_OK;
+}
+
+// ---------------------------------------------------------------------------
+// check_quality (src/python.rs:282-339 -> src/lib.rs:25-78)
+// ---------------------------------------------------------------------------
+extern "C" wtet_status wtet_check_quality(const float* xyz, uint32_t n, const uint32_t* tets,
+                                          const uint32_t* adjacency, uint32_t num_tets,
+                                          uint64_t* violations, uint64_t* interior_tets) {
+  if (!xyz || !tets || !adjacency || !violations || !interior_tets) return WTET_ERR_INVALID_ARGUMENT;
+  if (n < 1) return WTET_ERR_INVALID_ARGUMENT;
+  // normalize exactly as the triangulation does (src/lib.rs:313-337)
+  std::vector<float> pn((size_t)n * 3);
+  float mn[3] = {3.40282347e+38f, 3.40282347e+38f, 3.40282347e+38f};
+  float mx[3] = {-3.40282347e+38f, -3.40282347e+38f, -3.40282347e+38f};
+  for (uint32_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      float v = xyz[3 * (size_t)i + k];
+      if (v < mn[k]) mn[k] = v;
+      if (v > mx[k]) mx[k] = v;
+    }
+  float s = mx[0] - mn[0];
+  if (mx[1] - mn[1] > s) s = mx[1] - mn[1];
+  if (mx[2] - mn[2] > s) s = mx[2] - mn[2];
+  if (!(s >= 1e-10f)) s = 1e-10f;
+  for (uint32_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) pn[3 * (size_t)i + k] = (xyz[3 * (size_t)i + k] - mn[k]) / s;
+  PointSet S;
+  S.xyzn = pn.data();
+  S.n = n;
+  const float sv[4][3] = {{-100.f, -100.f, -100.f}, {400.f, -100.f, -100.f},
+                          {-100.f, 400.f, -100.f}, {-100.f, -100.f, 400.f}};
+  std::memcpy(S.sup, sv, sizeof(sv));
+  uint64_t viol = 0, interior = 0;
+  for (uint32_t t = 0; t < num_tets; ++t) {
+    const uint32_t* tv = tets + 4 * (size_t)t;
+    if (tv[0] >= n || tv[1] >= n || tv[2] >= n || tv[3] >= n) continue;
+    ++interior;
+    for (int f = 0; f < 4; ++f) {
+      uint32_t e = adjacency[4 * (size_t)t + f];
+      if (e == kInvalid) continue;
+      uint32_t u = e >> 5, g = e & 3u;
+      if (t >= u) continue;
+      if (u >= num_tets) return WTET_ERR_INVALID_ARGUMENT;
+      const uint32_t* uv = tets + 4 * (size_t)u;
+      if (uv[0] >= n || uv[1] >= n || uv[2] >= n || uv[3] >= n) continue;
+      if (o4(S, tv, uv[g], PRED_FILTERED) < 0) ++viol;
+    }
+  }
+  *violations = viol;
+  *interior_tets = interior;
+  return WTET_OK;
+}
+
+// ---------------------------------------------------------------------------
+// phase 2 alone
+// ---------------------------------------------------------------------------
+extern "C" wtet_status wtet_repair(const float* xyzn, uint32_t n, uint32_t** tets,
+                                   uint32_t** adjacency, uint32_t* num_tets, int mode,
+                                   uint32_t* regions) {
+  if (!xyzn || !tets || !*tets || !adjacency || !num_tets || n < 1) return WTET_ERR_INVALID_ARGUMENT;
+  try {
+    size_t M = *num_tets;
+    std::vector<uint32_t> tv(*tets, *tets + 4 * M), ta(4 * M, kInvalid);
+    for (size_t i = 0; i < 4 * M; ++i)
+      if (tv[i] > n + 4) return WTET_ERR_INVALID_ARGUMENT;
+    if (*adjacency) {
+      ta.assign(*adjacency, *adjacency + 4 * M);
+    } else {
+      std::unordered_map<FaceKey, int64_t, FaceHash> seen;
+      for (size_t t = 0; t < M; ++t)
+        for (int f = 0; f < 4; ++f) {
+          uint32_t a[3];
+          int j = 0;
+          for (int v = 0; v < 4; ++v)
+            if (v != f) a[j++] = tv[4 * t + v];
+          FaceKey k = face_key(a[0], a[1], a[2]);
+          auto it = seen.find(k);
+          if (it == seen.end()) {
+            seen.emplace(k, (int64_t)(4 * t + f));
+          } else {
+            int64_t o = it->second;
+            ta[4 * t + f] = ((uint32_t)(o >> 2) << 5) | (uint32_t)(o & 3);
+            ta[o] = ((uint32_t)t << 5) | (uint32_t)f;
+            seen.erase(it);
+          }
+        }
+    }
+    // every face that fails the insphere test seeds the repair
+    PointSet S;
+    S.xyzn = xyzn;
+    S.n = n;
+    const float sv[4][3] = {{-100.f, -100.f, -100.f}, {400.f, -100.f, -100.f},
+                            {-100.f, 400.f, -100.f}, {-100.f, -100.f, 400.f}};
+    std::memcpy(S.sup, sv, sizeof(sv));
+    int pm = mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
+    std::vector<uint32_t> viol;
+    for (size_t t = 0; t < M; ++t) {
+      const uint32_t* v = &tv[4 * t];
+      bool inf = v[0] == n + 4 || v[1] == n + 4 || v[2] == n + 4 || v[3] == n + 4;
+      if (inf) continue;
+      for (int f = 0; f < 4; ++f) {
+        uint32_t e = ta[4 * t + f];
+        if (e == kInvalid) continue;
+        uint32_t u = e >> 5, g = e & 3u;
+        if (u <= t || u >= M) continue;
+        const uint32_t* w = &tv[4 * (size_t)u];
+        if (w[0] == n + 4 || w[1] == n + 4 || w[2] == n + 4 || w[3] == n + 4) continue;
+        if (o4(S, v, w[g], pm) < 0) viol.push_back(((uint32_t)t << 2) | (uint32_t)f);
+      }
+    }
+    RepairReport rr;
+    wtet_status s = WTET_OK;
+    if (!viol.empty()) s = repair_delaunay(xyzn, n, tv, ta, viol, mode, &rr);
+    if (s != WTET_OK) return s;
+    uint32_t* ot = (uint32_t*)std::malloc(std::max<size_t>(tv.size(), 1) * 4);
+    uint32_t* oa = (uint32_t*)std::malloc(std::max<size_t>(ta.size(), 1) * 4);
+    if (!ot || !oa) {
+      std::free(ot);
+      std::free(oa);
+      return WTET_ERR_OUT_OF_MEMORY;
+    }
+    std::memcpy(ot, tv.data(), tv.size() * 4);
+    std::memcpy(oa, ta.data(), ta.size() * 4);
+    *tets = ot;
+    *adjacency = oa;
+    *num_tets = (uint32_t)(tv.size() / 4);
+    if (regions) *regions = rr.regions;
+    return WTET_OK;
+  } catch (...) {
+    return WTET_ERR_OUT_OF_MEMORY;
+  }
+}

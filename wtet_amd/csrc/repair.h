@@ -1,0 +1,26 @@
+// Host phase 2: make the triangulation exactly Delaunay when the concurrent
+// GPU flipping left faces it could not flip (the reference's CPU fix-up
+// chain, src/lib.rs:227-307: cpu_flip_violations -> full_bw_reconstruction
+// -> star splaying).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/wtet.h"
+
+struct RepairReport {
+  uint32_t regions = 0;      // regions re-triangulated
+  uint32_t max_region = 0;   // largest region (tets)
+  std::string message;
+};
+
+// xyzn: n normalized points (input id order).  tets/adj: the whole mesh in
+// the result format (input ids, super n..n+3, infinity n+4; adjacency
+// (t << 5) | f).  viol: (t << 2) | f of non-Delaunay faces.  On success the
+// arrays hold the exact Delaunay triangulation.
+wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>& tets,
+                            std::vector<uint32_t>& adj, const std::vector<uint32_t>& viol,
+                            int mode, RepairReport* rep);
