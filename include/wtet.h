@@ -69,6 +69,8 @@ typedef struct wtet_stats {
   uint32_t repair_regions; /* regions re-triangulated on the host             */
   uint32_t walk_failures;
   uint32_t grow_events;    /* device buffer growths                           */
+  uint32_t duplicates;     /* exact duplicate points not inserted (all but the
+                              highest id of each group)                       */
   double ms_device;        /* normalized points in HBM -> tets in HBM          */
   double ms_repair;        /* host repair                                      */
   double ms_total;         /* whole call                                       */
@@ -127,6 +129,13 @@ void wtet_decode_adjacency(uint32_t packed, uint32_t *tet, uint32_t *face);
 /* __version__ and gpu_info(), src/python.rs:355-371, 397-399 */
 const char *wtet_version(void);
 wtet_status wtet_device_info(wtet_ctx *ctx, char *buf, uint32_t len);
+
+/* Device memory for callers without their own HIP allocator (one process
+ * should hold one HIP runtime): copy host bytes to a new device allocation,
+ * copy device bytes back, release.  Synchronous. */
+wtet_status wtet_device_upload(wtet_ctx *ctx, const void *host, size_t bytes, void **d_out);
+wtet_status wtet_device_download(wtet_ctx *ctx, const void *d_src, void *host_dst, size_t bytes);
+wtet_status wtet_device_free(wtet_ctx *ctx, void *d_ptr);
 
 /* Per-kernel-group timing with HIP events on the context's stream (the
  * reference's GpuProfiler/CpuProfiler scopes, src/profiler.rs:5-224).

@@ -198,10 +198,14 @@ typedef struct {
   uint32_t id;
 } KeyId;
 
+/* Morton order; equal codes by descending id, so that of exact duplicates
+ * the highest id goes in first and the others -- redundant under the lifted
+ * SoS, which perturbs |p|^2 independently -- find no conflicting tet and are
+ * skipped, as in the reference's Bowyer-Watson (src/phase2.rs:440-443). */
 static int cmp_keyid(const void *x, const void *y) {
   const KeyId *p = (const KeyId *)x, *q = (const KeyId *)y;
   if (p->code != q->code) return p->code < q->code ? -1 : 1;
-  return p->id < q->id ? -1 : (p->id > q->id);
+  return p->id > q->id ? -1 : (p->id < q->id);
 }
 
 int64_t o_delaunay_bw(const float *pts, int64_t n, uint32_t **tets_out,

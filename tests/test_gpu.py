@@ -179,18 +179,18 @@ def test_module_level_api(oracle):
 
 
 def test_resident_matches_host_result(oracle, ctx):
-    """Device-resident entry point: same mesh as the host-result call."""
-    torch = pytest.importorskip("torch")
+    """Device-resident points and result: same mesh as the host-result call."""
     p = W.uniform(30000, 8)
-    t = torch.from_numpy(p).to("cuda:0")
-    torch.cuda.synchronize()
-    m, dt, da = ctx.delaunay_resident(t.data_ptr(), len(p))
-
-    class _Dev:
-        def __init__(self, ptr, n):
-            self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<i4", "data": (ptr, False),
-                                             "version": 3}
-
-    tets = torch.as_tensor(_Dev(dt, m * 4), device="cuda:0").cpu().numpy().view(np.uint32).reshape(m, 4)
-    ref = oracle.canonical_real(ctx.delaunay(p).tets, len(p))
-    assert np.array_equal(oracle.canonical_real(tets, len(p)), ref)
+    d = ctx.upload(p)
+    try:
+        m, dt, da = ctx.delaunay_resident(d, len(p))
+        tets = ctx.download(dt, (m, 4), np.uint32)
+        adj = ctx.download(da, (m, 4), np.uint32)
+    finally:
+        ctx.free(d)
+    host = ctx.delaunay(p)
+    assert m == host.num_tets
+    assert np.array_equal(oracle.canonical_real(tets, len(p)), oracle.canonical_real(host.tets, len(p)))
+    back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
+    idx = (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m)
+    assert np.array_equal(back, idx)

@@ -79,6 +79,7 @@ class _Stats(ctypes.Structure):
         ("repair_regions", ctypes.c_uint32),
         ("walk_failures", ctypes.c_uint32),
         ("grow_events", ctypes.c_uint32),
+        ("duplicates", ctypes.c_uint32),
         ("ms_device", ctypes.c_double),
         ("ms_repair", ctypes.c_double),
         ("ms_total", ctypes.c_double),
@@ -120,6 +121,9 @@ def lib():
             L.wtet_version.restype = ctypes.c_char_p
             L.wtet_device_info.argtypes = [P, ctypes.c_char_p, u32]
             L.wtet_set_profiling.argtypes = [P, ctypes.c_int]
+            L.wtet_device_upload.argtypes = [P, P, ctypes.c_size_t, ctypes.POINTER(P)]
+            L.wtet_device_download.argtypes = [P, P, P, ctypes.c_size_t]
+            L.wtet_device_free.argtypes = [P, P]
             L.wtet_kernel_times.argtypes = [P, u32, ctypes.POINTER(ctypes.c_char_p),
                                             ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)]
             L.wtet_predicates_host.argtypes = [ctypes.c_int, P, P, u32, ctypes.c_int, P]
@@ -167,6 +171,23 @@ class Context:
         d = {k: getattr(s, k) for k, _ in _Stats._fields_ if k != "pred"}
         d["pred"] = list(s.pred)
         return d
+
+    def upload(self, arr: np.ndarray) -> int:
+        """Copy a host array to a new device allocation; returns its address."""
+        a = np.ascontiguousarray(arr)
+        d = ctypes.c_void_p()
+        if lib().wtet_device_upload(self._h, _p(a), a.nbytes, ctypes.byref(d)) != 0:
+            raise RuntimeError(f"device upload failed: {self.error()}")
+        return d.value
+
+    def download(self, ptr: int, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        if lib().wtet_device_download(self._h, ctypes.c_void_p(ptr), _p(out), out.nbytes) != 0:
+            raise RuntimeError(f"device download failed: {self.error()}")
+        return out
+
+    def free(self, ptr: int):
+        lib().wtet_device_free(self._h, ctypes.c_void_p(ptr))
 
     def set_profiling(self, on: bool = True):
         lib().wtet_set_profiling(self._h, int(on))

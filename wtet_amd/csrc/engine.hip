@@ -46,8 +46,14 @@ struct wtet_ctx {
   // lists
   DevBuf uvert[2], utet[2], wlist, act[2], prop, flips, fout, blk, slot2idx, rec;
   // counters
-  DevBuf ctr, gstats;
+  DevBuf ctr, gstats, patch;
   uint32_t* h_ctr = nullptr;  // pinned mirror of the counters
+  // pinned staging for the mesh when the host needs it (repair / host result)
+  uint32_t* h_tets = nullptr;
+  uint32_t* h_adj = nullptr;
+  float* h_xyzn = nullptr;
+  uint32_t* h_patch = nullptr;
+  size_t h_cap = 0, h_npts = 0, h_patch_cap = 0;
   uint64_t cap_tets = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -65,7 +71,7 @@ struct wtet_ctx {
                    &pts,    &tet,    &opp,    &tinfo, &tvote,  &fvote,    &tsplit,   &fit,
                    &fround, &tmsg,   &tfwd,   &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &flips,  &fout,     &blk,      &slot2idx,
-                   &rec,    &ctr,    &gstats};
+                   &rec,    &ctr,    &gstats, &patch};
     all.assign(std::begin(l), std::end(l));
   }
 };
@@ -108,6 +114,21 @@ void ensure(wtet_ctx* c, DevBuf& b, size_t bytes, bool keep = false) {
 template <typename T>
 T* P(DevBuf& b) {
   return reinterpret_cast<T*>(b.p);
+}
+
+template <typename T>
+void ensure_pinned(T*& p, size_t& cap, size_t need) {
+  if (p && cap >= need) return;
+  if (p) HIP_CHECK(hipHostFree(p));
+  p = nullptr;
+  size_t nc = std::max(need, cap + cap / 2);
+  if (hipHostMalloc((void**)&p, nc * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    cap = 0;
+    throw Fail{WTET_ERR_OUT_OF_MEMORY, "pinned host allocation failed"};
+  }
+  cap = nc;
 }
 
 // per-tet-slot arrays sized for `cap` slots; keep=true preserves contents
@@ -282,8 +303,8 @@ void collect_times(wtet_ctx* c);
 uint32_t flip_iteration(wtet_ctx* c, uint32_t n, int mode, uint32_t& ac, uint32_t& A,
                         uint32_t& it, uint32_t round, Counters& K, uint64_t* st) {
   if (c->evused > 4096) collect_times(c);
-  // worst case every proposal wins and takes one fresh slot
-  read_ctr(c);
+  // worst case every proposal wins and takes one fresh slot (h_ctr holds the
+  // counters read at the end of the previous step)
   uint64_t need = (uint64_t)c->h_ctr[C_TETS] + A + 64;
   if (need > c->cap_tets) ensure_tets(c, need + need / 2, true);
   Mesh M = mesh_of(c, n, mode);
@@ -310,17 +331,14 @@ uint32_t flip_iteration(wtet_ctx* c, uint32_t n, int mode, uint32_t& ac, uint32_
 }
 
 void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& cfg, Counters& K,
-                std::vector<uint32_t>& out_tets, std::vector<uint32_t>& out_adj,
-                std::vector<uint32_t>& viol_rec, uint32_t& nviol, bool resident,
-                uint32_t& mout) {
+                std::vector<uint32_t>& viol_rec, uint32_t& nviol, bool resident, uint32_t& mout) {
   hipStream_t s = c->stream;
   int mode = cfg.predicate_mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
-  uint64_t* st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
 
   ensure(c, c->xyzn, (size_t)n * 12);
-  ensure(c, c->keys, (size_t)n * 4);
+  ensure(c, c->keys, (size_t)n * 8);
   ensure(c, c->ids, (size_t)n * 4);
-  ensure(c, c->keys2, (size_t)n * 4);
+  ensure(c, c->keys2, (size_t)n * 8);
   ensure(c, c->ids2, (size_t)n * 4);
   ensure(c, c->sort_tmp, std::max<size_t>(sort_temp_bytes(n), 16));
   ensure(c, c->mm6, 64);
@@ -334,6 +352,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
   ensure(c, c->gstats, 64);
   ensure(c, c->rec, (1u << 20) * 4);
   ensure_tets(c, 7ull * n + 4096, false);
+  uint64_t* st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
 
   HIP_CHECK(hipEventRecord(c->ev0, s));
   // points: normalize (f32, as the reference), Morton keys, sort, gather
@@ -344,25 +363,23 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
   launch_fill_u64(P<uint64_t>(c->gstats), 0ull, 8, s);
   timed(c, K_PREP, [&] {
     launch_minmax(d_xyz, n, mm6, s);
-    launch_normalize(d_xyz, n, mm6, P<float>(c->xyzn), P<uint32_t>(c->keys), P<uint32_t>(c->ids), s);
-    sort_pairs(P<uint32_t>(c->keys), P<uint32_t>(c->ids), P<uint32_t>(c->keys2),
+    launch_normalize(d_xyz, n, mm6, P<float>(c->xyzn), P<uint64_t>(c->keys), P<uint32_t>(c->ids), s);
+    sort_pairs(P<uint64_t>(c->keys), P<uint32_t>(c->ids), P<uint64_t>(c->keys2),
                P<uint32_t>(c->ids2), n, c->sort_tmp.p, c->sort_tmp.bytes, s);
     launch_gather_points(P<float>(c->xyzn), P<uint32_t>(c->ids2), n, P<float4>(c->pts), s);
+    // uninserted list: every point (minus redundant duplicates) in the super tet
+    launch_init_uninserted(P<float4>(c->pts), P<uint64_t>(c->keys2), n, P<uint32_t>(c->blk),
+                           P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), P<uint32_t>(c->ctr), s);
   });
   HIP_CHECK(hipGetLastError());
   upload_points_tail(c, n);
   upload_initial_tets(c, n);
   launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u, -1, 0u, s);
+  read_ctr(c);
+  uint32_t U = c->h_ctr[C_NPTS];
+  c->stats.duplicates = n - U;
 
-  // uninserted list: every point in the super tet (slot 0)
-  {
-    std::vector<uint32_t> iota(n);
-    for (uint32_t i = 0; i < n; ++i) iota[i] = i;
-    HIP_CHECK(hipMemcpyAsync(c->uvert[0].p, iota.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
-    launch_fill_u32(P<uint32_t>(c->utet[0]), 0u, n, s);
-  }
-
-  uint32_t U = n, cur = 0, ac = 0, A = 0, it = 0, round = 0;
+  uint32_t cur = 0, ac = 0, A = 0, it = 0, round = 0;
   uint32_t max_rounds = std::max<uint32_t>(cfg.max_insert_iterations, 1u);
   uint32_t max_fi = std::max<uint32_t>(cfg.max_flip_iterations, 1u);
   bool flipping = cfg.enable_flipping != 0;
@@ -447,20 +464,21 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
   mout = Mout;
   nviol = c->h_ctr[C_VIOL];
   if (!resident || nviol > 0) {
-    out_tets.resize((size_t)Mout * 4);
-    out_adj.resize((size_t)Mout * 4);
-    HIP_CHECK(hipMemcpyAsync(out_tets.data(), c->prop.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(out_adj.data(), c->fout.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
+    // room for the repair to append (it rarely adds more than a few tets)
+    size_t cap = (size_t)Mout + Mout / 32 + 65536;
+    size_t c1 = c->h_cap, c2 = c->h_cap;
+    ensure_pinned(c->h_tets, c1, cap * 4);
+    ensure_pinned(c->h_adj, c2, cap * 4);
+    c->h_cap = std::min(c1, c2);
+    HIP_CHECK(hipMemcpyAsync(c->h_tets, c->prop.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(c->h_adj, c->fout.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
   }
   uint32_t nrec = std::min(c->h_ctr[C_FAILREC], rec_cap);
   viol_rec.resize(nrec);
   if (nrec) {
-    // records are slot-based; map to output indices on the host
-    std::vector<uint32_t> s2i(T);
+    // records are slot-based; map them to output indices on the device
+    launch_map_records(P<uint32_t>(c->rec), nrec, P<uint32_t>(c->slot2idx), s);
     HIP_CHECK(hipMemcpyAsync(viol_rec.data(), c->rec.p, (size_t)nrec * 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipMemcpyAsync(s2i.data(), c->slot2idx.p, (size_t)T * 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    for (auto& r : viol_rec) r = (s2i[r >> 2] << 2) | (r & 3u);
   }
   if (cfg.collect_stats)
     HIP_CHECK(hipMemcpyAsync(c->stats.pred, c->gstats.p, 64, hipMemcpyDeviceToHost, s));
@@ -518,35 +536,40 @@ void delaunay_impl(void* a_) {
     d_xyz = P<float>(c->xyz_in);
   }
   Counters K;
-  std::vector<uint32_t> tets, adj, rec;
+  std::vector<uint32_t> rec;
   uint32_t nviol = 0, mout = 0;
   bool resident = a.dout != nullptr;
-  run_device(c, d_xyz, a.n, a.cfg, K, tets, adj, rec, nviol, resident, mout);
+  run_device(c, d_xyz, a.n, a.cfg, K, rec, nviol, resident, mout);
   c->stats.violations_gpu = nviol;
+  MeshBuf mb{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
+  bool on_host = !resident || nviol > 0;
   // failed_verts: the vertex opposite each non-Delaunay real-only face
   std::vector<uint32_t> failed;
-  for (uint32_t r : rec) {
-    if (tets.empty()) break;
-    uint32_t t = r >> 2, f = r & 3u;
-    uint32_t e = adj[4 * (size_t)t + f];
-    if (e == kInvalid) continue;
-    uint32_t u = e >> 5, g = e & 3u;
-    bool real = true;
-    for (int k = 0; k < 4; ++k)
-      real = real && tets[4 * (size_t)t + k] < a.n && tets[4 * (size_t)u + k] < a.n;
-    if (real) failed.push_back(tets[4 * (size_t)u + g]);
-  }
+  if (on_host)
+    for (uint32_t r : rec) {
+      uint32_t t = r >> 2, f = r & 3u;
+      uint32_t e = mb.adj[4 * (size_t)t + f];
+      if (e == kInvalid) continue;
+      uint32_t u = e >> 5, g = e & 3u;
+      bool real = true;
+      for (int k = 0; k < 4; ++k)
+        real = real && mb.tets[4 * (size_t)t + k] < a.n && mb.tets[4 * (size_t)u + k] < a.n;
+      if (real) failed.push_back(mb.tets[4 * (size_t)u + g]);
+    }
   std::sort(failed.begin(), failed.end());
   failed.erase(std::unique(failed.begin(), failed.end()), failed.end());
   auto t1 = std::chrono::steady_clock::now();
+  bool repaired = false;
   if (nviol > 0 && a.cfg.enable_splaying) {
-    // host repair needs the normalized coordinates of the input ids
-    std::vector<float> xyzn((size_t)a.n * 3);
-    HIP_CHECK(hipMemcpy(xyzn.data(), c->xyzn.p, (size_t)a.n * 12, hipMemcpyDeviceToHost));
+    // the repair needs the normalized coordinates of the input ids
+    ensure_pinned(c->h_xyzn, c->h_npts, (size_t)a.n * 3);
+    HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)a.n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     RepairReport rr;
-    wtet_status rs = repair_delaunay(xyzn.data(), a.n, tets, adj, rec, a.cfg.predicate_mode, &rr);
+    wtet_status rs = repair_delaunay(c->h_xyzn, a.n, mb, rec, a.cfg.predicate_mode, &rr);
     if (rs != WTET_OK) throw Fail{rs, "host repair failed: " + rr.message};
     c->stats.repair_regions = rr.regions;
+    repaired = true;
   }
   auto t2 = std::chrono::steady_clock::now();
   c->stats.ms_repair = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -557,13 +580,28 @@ void delaunay_impl(void* a_) {
   c->stats.flip_iterations = K.iters;
   c->stats.num_points = a.n;
   if (resident) {
-    if (nviol > 0) {  // upload the repaired mesh
-      mout = (uint32_t)(tets.size() / 4);
-      ensure(c, c->prop, std::max<size_t>(tets.size(), 4) * 4, false);
-      ensure(c, c->fout, std::max<size_t>(adj.size(), 4) * 4, false);
-      HIP_CHECK(hipMemcpyAsync(c->prop.p, tets.data(), tets.size() * 4, hipMemcpyHostToDevice, c->stream));
-      HIP_CHECK(hipMemcpyAsync(c->fout.p, adj.data(), adj.size() * 4, hipMemcpyHostToDevice, c->stream));
-      HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (repaired) {
+      // upload only the slots the repair wrote: (slot, tet[4], adj[4])
+      size_t np = mb.changed.size();
+      if (np) {
+        ensure_pinned(c->h_patch, c->h_patch_cap, np * 9);
+        for (size_t i = 0; i < np; ++i) {
+          uint32_t sl = mb.changed[i];
+          uint32_t* q = c->h_patch + 9 * i;
+          q[0] = sl;
+          std::memcpy(q + 1, mb.tets + 4 * (size_t)sl, 16);
+          std::memcpy(q + 5, mb.adj + 4 * (size_t)sl, 16);
+        }
+        ensure(c, c->patch, np * 36);
+        if ((uint64_t)mb.count > c->prop.bytes / 16 || (uint64_t)mb.count > c->fout.bytes / 16)
+          throw Fail{WTET_ERR_INTERNAL, "repaired mesh larger than the device output buffers"};
+        HIP_CHECK(hipMemcpyAsync(c->patch.p, c->h_patch, np * 36, hipMemcpyHostToDevice, c->stream));
+        launch_apply_patch(P<uint32_t>(c->patch), (uint32_t)np, P<uint32_t>(c->prop),
+                           P<uint32_t>(c->fout), c->stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+      }
+      mout = (uint32_t)mb.count;
     }
     a.dout->num_tets = mout;
     a.dout->d_tets = P<uint32_t>(c->prop);
@@ -573,6 +611,7 @@ void delaunay_impl(void* a_) {
     c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return;
   }
+  std::vector<uint32_t> tets(mb.tets, mb.tets + 4 * mb.count), adj(mb.adj, mb.adj + 4 * mb.count);
   wtet_result* out = a.out;
   out->num_tets = (uint32_t)(tets.size() / 4);
   out->tets = (uint32_t*)std::malloc(std::max<size_t>(tets.size(), 1) * 4);
@@ -620,6 +659,10 @@ void wtet_destroy(wtet_ctx* c) {
   for (DevBuf* b : c->all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+  if (c->h_tets) (void)hipHostFree(c->h_tets);
+  if (c->h_adj) (void)hipHostFree(c->h_adj);
+  if (c->h_xyzn) (void)hipHostFree(c->h_xyzn);
+  if (c->h_patch) (void)hipHostFree(c->h_patch);
   for (auto& pr : c->evpool) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
@@ -722,6 +765,44 @@ void wtet_decode_adjacency(uint32_t packed, uint32_t* tet, uint32_t* face) {
 }
 
 const char* wtet_version(void) { return "0.1.0"; }
+
+wtet_status wtet_device_upload(wtet_ctx* c, const void* host, size_t bytes, void** d_out) {
+  if (!c || !host || !d_out || bytes == 0) return WTET_ERR_INVALID_ARGUMENT;
+  *d_out = nullptr;
+  if (hipSetDevice(c->device) != hipSuccess || hipMalloc(d_out, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    c->err = "device allocation failed";
+    return WTET_ERR_OUT_OF_MEMORY;
+  }
+  if (hipMemcpy(*d_out, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(*d_out);
+    *d_out = nullptr;
+    c->err = "host to device copy failed";
+    return WTET_ERR_DEVICE;
+  }
+  return WTET_OK;
+}
+
+wtet_status wtet_device_download(wtet_ctx* c, const void* d_src, void* host_dst, size_t bytes) {
+  if (!c || !d_src || !host_dst) return WTET_ERR_INVALID_ARGUMENT;
+  if (hipSetDevice(c->device) != hipSuccess ||
+      hipMemcpy(host_dst, d_src, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    c->err = "device to host copy failed";
+    return WTET_ERR_DEVICE;
+  }
+  return WTET_OK;
+}
+
+wtet_status wtet_device_free(wtet_ctx* c, void* d_ptr) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  if (d_ptr && hipFree(d_ptr) != hipSuccess) {
+    (void)hipGetLastError();
+    return WTET_ERR_DEVICE;
+  }
+  return WTET_OK;
+}
 
 wtet_status wtet_set_profiling(wtet_ctx* c, int on) {
   if (!c) return WTET_ERR_INVALID_ARGUMENT;

@@ -15,12 +15,16 @@ inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBloc
 // point preparation: normalization (src/lib.rs:313-337), Morton keys, gather
 void launch_minmax(const float* xyz, uint32_t n, uint32_t* mm6, hipStream_t s);
 void launch_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
-                      uint32_t* keys, uint32_t* ids, hipStream_t s);
+                      uint64_t* keys, uint32_t* ids, hipStream_t s);
 void launch_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts,
                           hipStream_t s);
-void sort_pairs(uint32_t* keys, uint32_t* ids, uint32_t* keys_tmp, uint32_t* ids_tmp,
+void sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp,
                 uint32_t n, void* temp, size_t temp_bytes, hipStream_t s);
 size_t sort_temp_bytes(uint32_t n);
+// uninserted list = every point except redundant exact duplicates;
+// ctr[C_NPTS] = its length
+void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n, uint32_t* blk,
+                            uint32_t* uvert, uint32_t* utet, uint32_t* ctr, hipStream_t s);
 
 // rounds
 void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
@@ -48,6 +52,12 @@ void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64
 // output: compact alive tets, original vertex ids, (tet << 5) | face adjacency
 void launch_output(const Mesh& M, uint32_t T, uint32_t n, uint32_t* slot2idx, uint32_t* blk,
                    uint32_t* out_tets, uint32_t* out_adj, hipStream_t s);
+
+// violation records (slot << 2 | face) -> output indices
+void launch_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx, hipStream_t s);
+// write (slot, tet[4], adj[4]) records into the output arrays
+void launch_apply_patch(const uint32_t* patch, uint32_t n, uint32_t* tets, uint32_t* adj,
+                        hipStream_t s);
 
 // counters
 void launch_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2, hipStream_t s);

@@ -54,6 +54,7 @@ __device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* total) {
 // in-place exclusive scan of blk[0..nb) by one 1024-thread block
 // mode 0: insertion round -> ctr[C_WIN] = total, reserve 3*total slots
 // mode 1: output          -> ctr[C_ALIVE] = total
+// mode 2: point list      -> ctr[C_NPTS] = total
 __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* blk, uint32_t nb, uint32_t* ctr,
                                                      int mode) {
   __shared__ uint32_t part[1024];
@@ -82,8 +83,10 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* blk, uint32_t nb
       ctr[C_WIN] = total;
       ctr[C_BASE] = ctr[C_TETS];
       ctr[C_TETS] += 3u * total;
-    } else {
+    } else if (mode == 1) {
       ctr[C_ALIVE] = total;
+    } else {
+      ctr[C_NPTS] = total;
     }
   }
 }
@@ -126,17 +129,20 @@ __global__ void k_minmax(const float* xyz, uint32_t n, uint32_t* mm6) {
   }
 }
 
-WT_HD uint32_t expand10(uint32_t v) {
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
+// spread the low 21 bits of v to every third bit of a 64-bit word
+__device__ __forceinline__ uint64_t expand21(uint64_t v) {
+  v &= 0x1FFFFFull;
+  v = (v | (v << 32)) & 0x001F00000000FFFFull;
+  v = (v | (v << 16)) & 0x001F0000FF0000FFull;
+  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
   return v;
 }
 
 #pragma clang fp contract(off)
 __global__ void k_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
-                            uint32_t* keys, uint32_t* ids) {
+                            uint64_t* keys, uint32_t* ids) {
   uint32_t i = gid();
   if (i >= n) return;
   float mn[3], ext[3];
@@ -153,9 +159,11 @@ __global__ void k_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, f
     float v = (xyz[3 * (size_t)i + k] - mn[k]) / s;  // IEEE f32 sub and div
     xyzn[3 * (size_t)i + k] = v;
     float c = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
-    q[k] = (uint32_t)(c * 1023.0f);
+    q[k] = (uint32_t)(c * 2097151.0f);
   }
-  keys[i] = (expand10(q[2]) << 2) | (expand10(q[1]) << 1) | expand10(q[0]);
+  // 63-bit Morton key: spatial order for memory locality (the insertion
+  // order cannot change the triangulation)
+  keys[i] = (expand21(q[2]) << 2) | (expand21(q[1]) << 1) | expand21(q[0]);
   ids[i] = i;
 }
 #pragma clang fp contract(on)
@@ -173,7 +181,7 @@ void launch_minmax(const float* xyz, uint32_t n, uint32_t* mm6, hipStream_t s) {
   k_minmax<<<nblocks(n), kBlock, 0, s>>>(xyz, n, mm6);
 }
 void launch_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
-                      uint32_t* keys, uint32_t* ids, hipStream_t s) {
+                      uint64_t* keys, uint32_t* ids, hipStream_t s) {
   k_normalize<<<nblocks(n), kBlock, 0, s>>>(xyz, n, mm6, xyzn, keys, ids);
 }
 void launch_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts,
@@ -183,22 +191,113 @@ void launch_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, fl
 
 size_t sort_temp_bytes(uint32_t n) {
   size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs((void*)nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                            (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 30);
+  (void)rocprim::radix_sort_pairs((void*)nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 63);
   return bytes;
 }
 
-void sort_pairs(uint32_t* keys, uint32_t* ids, uint32_t* keys_tmp, uint32_t* ids_tmp, uint32_t n,
+void sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp, uint32_t n,
                 void* temp, size_t temp_bytes, hipStream_t s) {
-  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, ids, ids_tmp, n, 0, 30, s);
+  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, ids, ids_tmp, n, 0, 63, s);
+}
+
+// Exact duplicates (equal normalized coordinates) share a key, so they sit
+// within a run of equal keys after the sort.  All but the highest input id
+// are redundant under the lifted SoS and are not inserted; the uninserted
+// list is built from the rest (stable, one scan).
+__device__ __forceinline__ bool redundant_dup(const float4* pts, const uint64_t* keys, uint32_t n,
+                                              uint32_t j, uint32_t* capped) {
+  float4 p = pts[j];
+  uint32_t id = pt_rank(p);
+  uint64_t kj = keys[j];
+  const int kCap = 64;
+  for (int d = -1; d <= 1; d += 2) {
+    int steps = 0;
+    for (int64_t k = (int64_t)j + d; k >= 0 && k < (int64_t)n && keys[k] == kj; k += d) {
+      if (++steps > kCap) {
+        *capped = 1u;
+        break;
+      }
+      float4 q = pts[k];
+      if (q.x == p.x && q.y == p.y && q.z == p.z && pt_rank(q) > id) return true;
+    }
+  }
+  return false;
+}
+
+__global__ void k_uninit_count(const float4* pts, const uint64_t* keys, uint32_t n,
+                               uint32_t* blk, uint32_t* ctr) {
+  uint32_t j = gid();
+  uint32_t capped = 0;
+  bool keep = j < n && !redundant_dup(pts, keys, n, j, &capped);
+  if (capped) atomicAdd(&ctr[C_DUPSCAN], 1u);
+  uint32_t tot;
+  block_rank(keep, &tot);
+  if (threadIdx.x == 0) blk[blockIdx.x] = tot;
+}
+
+__global__ void k_uninit_scatter(const float4* pts, const uint64_t* keys, uint32_t n,
+                                 const uint32_t* blk, uint32_t* uvert, uint32_t* utet) {
+  uint32_t j = gid();
+  uint32_t capped = 0;
+  bool keep = j < n && !redundant_dup(pts, keys, n, j, &capped);
+  uint32_t tot;
+  uint32_t r = blk[blockIdx.x] + block_rank(keep, &tot);
+  if (keep) {
+    uvert[r] = j;
+    utet[r] = 0u;  // every point starts in the super tet (slot 0)
+  }
+}
+
+void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n, uint32_t* blk,
+                            uint32_t* uvert, uint32_t* utet, uint32_t* ctr, hipStream_t s) {
+  uint32_t nb = nblocks(n);
+  k_uninit_count<<<nb, kBlock, 0, s>>>(pts, keys, n, blk, ctr);
+  k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, ctr, 2);
+  k_uninit_scatter<<<nb, kBlock, 0, s>>>(pts, keys, n, blk, uvert, utet);
 }
 
 // ---------------------------------------------------------------------------
 // insertion rounds
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  uint32_t lo = __shfl((uint32_t)v, src), hi = __shfl((uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Lanes voting for the same tet as the wave's first pending lane combine
+// their keys with a wave min first (a few passes), so early rounds -- when
+// most points still share a few tets -- issue one atomic per wave instead of
+// one per point; the remaining lanes vote directly.
 __global__ void k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U) {
   uint32_t i = gid();
-  if (i < U) vote_body(M, uvert, utet, i);
+  uint64_t key = kVoteNone;
+  uint32_t t = 0;
+  if (i < U) {
+    t = utet[i];
+    key = vote_key(M, uvert, utet, i);
+  }
+  bool pending = key != kVoteNone;
+  for (int pass = 0; pass < 4; ++pass) {
+    uint64_t act = __ballot(pending);
+    if (!act) break;
+    int leader = __ffsll((unsigned long long)act) - 1;
+    uint32_t lt = __shfl(t, leader);
+    bool mine = pending && t == lt;
+    uint64_t k = mine ? key : kVoteNone;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      uint64_t o = shfl_xor_u64(k, off);
+      k = o < k ? o : k;
+    }
+    if (__lane_id() == leader) at_min_u64(&M.tvote[lt], k);
+    if (mine) pending = false;
+  }
+  if (pending) at_min_u64(&M.tvote[t], key);
 }
 
 __global__ void k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t* blk) {
@@ -394,6 +493,29 @@ void launch_output(const Mesh& M, uint32_t T, uint32_t n, uint32_t* slot2idx, ui
   k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, M.ctr, 1);
   k_alive_scatter<<<nb, kBlock, 0, s>>>(M, T, blk, slot2idx);
   if (out_tets) k_output_write<<<nb, kBlock, 0, s>>>(M, T, n, slot2idx, out_tets, out_adj);
+}
+
+__global__ void k_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx) {
+  uint32_t i = gid();
+  if (i < n) rec[i] = (slot2idx[rec[i] >> 2] << 2) | (rec[i] & 3u);
+}
+
+void launch_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx, hipStream_t s) {
+  if (n) k_map_records<<<nblocks(n), kBlock, 0, s>>>(rec, n, slot2idx);
+}
+
+__global__ void k_apply_patch(const uint32_t* patch, uint32_t n, uint32_t* tets, uint32_t* adj) {
+  uint32_t i = gid();
+  if (i >= n) return;
+  const uint32_t* q = patch + 9 * (size_t)i;
+  uint32_t sl = q[0];
+  reinterpret_cast<uint4*>(tets)[sl] = make_uint4(q[1], q[2], q[3], q[4]);
+  reinterpret_cast<uint4*>(adj)[sl] = make_uint4(q[5], q[6], q[7], q[8]);
+}
+
+void launch_apply_patch(const uint32_t* patch, uint32_t n, uint32_t* tets, uint32_t* adj,
+                        hipStream_t s) {
+  if (n) k_apply_patch<<<nblocks(n), kBlock, 0, s>>>(patch, n, tets, adj);
 }
 
 // ---------------------------------------------------------------------------

@@ -24,6 +24,8 @@ enum Ctr : int {
   C_ALIVE = 7,     // alive tets (output compaction)
   C_FAILREC = 8,   // recorded violation entries
   C_BADTET = 9,    // points found in a tet touching infinity (must stay 0)
+  C_NPTS = 10,     // points to insert (input minus exact duplicates)
+  C_DUPSCAN = 11,  // duplicate scans that hit their length cap
   C_COUNT = 16
 };
 
@@ -78,12 +80,13 @@ WT_HD uint32_t* opp_word(const Mesh& M, uint32_t t, uint32_t f) {
 // -insphere = 6|vol|(r^2 - d^2) for a raw-negative tet), ties to the
 // lowest list position.  src/shaders/vote.wgsl:89-281 (two passes there).
 // ---------------------------------------------------------------------------
-WT_HD void vote_body(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t i) {
+// returns kVoteNone when the point must not vote
+WT_HD uint64_t vote_key(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t i) {
   uint32_t t = utet[i];
   uint4 tv = M.tet[t];
   if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) {
     at_add_u32(&M.ctr[C_BADTET], 1u);
-    return;
+    return kVoteNone;
   }
   float a[3], b[3], c[3], d[3], p[3];
   uint32_t r;
@@ -96,8 +99,12 @@ WT_HD void vote_body(const Mesh& M, const uint32_t* uvert, const uint32_t* utet,
   double val = -det;
   if (!(val > 0.0)) val = 0.0;
   float fv = (float)val;
-  uint64_t key = ((uint64_t)(0xFFFFFFFFu - f32_bits(fv)) << 32) | i;
-  at_min_u64(&M.tvote[t], key);
+  return ((uint64_t)(0xFFFFFFFFu - f32_bits(fv)) << 32) | i;
+}
+
+WT_HD void vote_body(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t i) {
+  uint64_t key = vote_key(M, uvert, utet, i);
+  if (key != kVoteNone) at_min_u64(&M.tvote[utet[i]], key);
 }
 
 WT_HD bool is_winner(const Mesh& M, const uint32_t* utet, uint32_t i) {

@@ -106,10 +106,14 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
   if (o3sos(S, s[0], s[1], s[2], s[3], mode) <= 0) std::swap(s[0], s[1]);
   std::memcpy(&L.v[0], s, 16);
   L.alive[t0] = 1;
-  std::sort(ids.begin(), ids.end(), [&](uint32_t x, uint32_t y) {
-    uint32_t mx = morton10(S.p(x)), my = morton10(S.p(y));
-    return mx != my ? mx < my : x < y;
-  });
+  // random insertion order (deterministic): a new point then mostly lies
+  // inside the hull of those already inserted, so its cavity stays small
+  std::sort(ids.begin(), ids.end());
+  uint64_t sh = 0x9E3779B97F4A7C15ull ^ ids.size();
+  for (size_t i = ids.size(); i > 1; --i) {
+    sh = sh * 6364136223846793005ull + 1442695040888963407ull;
+    std::swap(ids[i - 1], ids[(size_t)((sh >> 33) % i)]);
+  }
   int64_t last = t0;
   uint32_t rng = 0x2545F491u;
   std::vector<int64_t> cav, stack;
@@ -155,10 +159,9 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
           t = u;
           break;
         }
-      if (t < 0) {
-        err = "no conflicting tet for a point";
-        return false;
-      }
+      if (t < 0) continue;  // redundant under the lifted SoS (an exact duplicate):
+                            // the reference's Bowyer-Watson skips it too
+                            // (src/phase2.rs:440-443)
     }
     // cavity
     int64_t stamp = (int64_t)it;
@@ -235,34 +238,36 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
 }
 
 // ---------------------------------------------------------------------------
-// the working mesh
+// in-place repair on the result arrays (ids, adjacency (t << 5) | f)
 // ---------------------------------------------------------------------------
-struct Work {
-  std::vector<uint32_t> tv;  // 4 per tet
-  std::vector<int64_t> ta;   // 4 per tet: t * 4 + f, -1 none
-  std::vector<uint8_t> alive;
-  std::vector<uint32_t> inR;  // region stamp
+struct Mesh32 {
+  MeshBuf& m;
   uint32_t inf;
-  bool has_inf(int64_t t) const {
-    const uint32_t* v = &tv[4 * t];
+  size_t count() const { return m.count; }
+  bool dead(uint32_t t) const { return m.tets[4 * (size_t)t] == kInvalid; }
+  bool has_inf(uint32_t t) const {
+    const uint32_t* v = &m.tets[4 * (size_t)t];
     return v[0] == inf || v[1] == inf || v[2] == inf || v[3] == inf;
   }
+  uint32_t* tv(uint32_t t) { return &m.tets[4 * (size_t)t]; }
+  const uint32_t* tv(uint32_t t) const { return &m.tets[4 * (size_t)t]; }
+  uint32_t& ta(uint32_t t, int f) { return m.adj[4 * (size_t)t + f]; }
+  uint32_t ta(uint32_t t, int f) const { return m.adj[4 * (size_t)t + f]; }
 };
 
-bool face_bad(const PointSet& S, const Work& W, int64_t t, int f, int mode) {
-  int64_t nb = W.ta[4 * t + f];
-  if (nb < 0) return false;
-  int64_t u = nb >> 2;
+bool face_bad(const PointSet& S, const Mesh32& W, uint32_t t, int f, int mode) {
+  uint32_t e = W.ta(t, f);
+  if (e == kInvalid) return false;
+  uint32_t u = e >> 5;
   if (W.has_inf(t) || W.has_inf(u)) return false;
-  uint32_t o = W.tv[4 * u + (nb & 3)];
-  return o4(S, &W.tv[4 * t], o, mode) < 0;
+  uint32_t o = W.tv(u)[e & 3u];
+  return o4(S, W.tv(t), o, mode) < 0;
 }
 
 }  // namespace
 
-wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>& tets,
-                            std::vector<uint32_t>& adj, const std::vector<uint32_t>& viol,
-                            int mode, RepairReport* rep) {
+wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
+                            const std::vector<uint32_t>& viol, int mode, RepairReport* rep) {
   PointSet S;
   S.xyzn = xyzn;
   S.n = n;
@@ -270,94 +275,80 @@ wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>
                           {-100.f, 400.f, -100.f}, {-100.f, -100.f, 400.f}};
   std::memcpy(S.sup, sv, sizeof(sv));
   mode = mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
+  Mesh32 W{mesh, n + 4};
+  std::vector<uint32_t>& changed = mesh.changed;
 
-  Work W;
-  W.inf = n + 4;
-  size_t M = tets.size() / 4;
-  W.tv = tets;
-  W.ta.resize(4 * M);
-  for (size_t i = 0; i < 4 * M; ++i)
-    W.ta[i] = adj[i] == kInvalid ? -1 : (int64_t)(adj[i] >> 5) * 4 + (adj[i] & 3u);
-  W.alive.assign(M, 1);
-  W.inR.assign(M, 0);
+  std::deque<std::pair<uint32_t, int>> queue;
+  for (uint32_t r : viol) queue.emplace_back(r >> 2, (int)(r & 3u));
+  std::vector<uint32_t> holes;
+  std::unordered_set<uint32_t> inR;
+  std::vector<uint32_t> R, ring;
 
-  std::deque<std::pair<int64_t, int>> queue;
-  for (uint32_t r : viol) queue.emplace_back((int64_t)(r >> 2), (int)(r & 3u));
-
-  uint32_t stamp = 0;
-  std::vector<int64_t> R, ring;
   while (!queue.empty()) {
     auto [t, f] = queue.front();
     queue.pop_front();
-    if (t >= (int64_t)W.alive.size() || !W.alive[t] || !face_bad(S, W, t, f, mode)) continue;
-    // seed region: the two tets of the face and their neighbours
+    if (t >= W.count() || W.dead(t) || !face_bad(S, W, t, f, mode)) continue;
     R.clear();
-    ++stamp;
-    auto add = [&](int64_t x) {
-      if (x < 0 || !W.alive[x] || W.has_inf(x) || W.inR[x] == stamp) return;
-      W.inR[x] = stamp;
+    inR.clear();
+    auto add = [&](uint32_t x) {
+      if (x >= W.count() || W.dead(x) || W.has_inf(x) || inR.count(x)) return;
+      inR.insert(x);
       R.push_back(x);
     };
     add(t);
-    add(W.ta[4 * t + f] >> 2);
+    add(W.ta(t, f) >> 5);
     bool done = false;
     while (!done) {
-      // one ring around R
       ring.clear();
-      for (int64_t x : R)
+      for (uint32_t x : R)
         for (int h = 0; h < 4; ++h)
-          if (W.ta[4 * x + h] >= 0) ring.push_back(W.ta[4 * x + h] >> 2);
-      for (int64_t x : ring) add(x);
-      if (R.size() * 2 > W.alive.size()) {
-        for (int64_t x = 0; x < (int64_t)W.alive.size(); ++x) add(x);
-      }
-      // vertices
-      std::vector<uint32_t> ids;
-      for (int64_t x : R)
-        for (int k = 0; k < 4; ++k) ids.push_back(W.tv[4 * x + k]);
-      std::sort(ids.begin(), ids.end());
-      ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+          if (W.ta(x, h) != kInvalid) ring.push_back(W.ta(x, h) >> 5);
+      for (uint32_t x : ring) add(x);
+      if (R.size() * 2 > W.count())
+        for (uint32_t x = 0; x < W.count(); ++x) add(x);
       std::vector<uint32_t> real;
-      for (uint32_t v : ids)
-        if (v < n) real.push_back(v);
+      for (uint32_t x : R)
+        for (int k = 0; k < 4; ++k)
+          if (W.tv(x)[k] < n) real.push_back(W.tv(x)[k]);
+      std::sort(real.begin(), real.end());
+      real.erase(std::unique(real.begin(), real.end()), real.end());
       Local L;
       std::string err;
       if (!local_delaunay(S, real, mode, L, err)) {
         if (rep) rep->message = err;
         return WTET_ERR_INTERNAL;
       }
-      // face map of D
       std::unordered_map<FaceKey, std::array<int64_t, 2>, FaceHash> fmap;
+      fmap.reserve(4 * L.alive.size());
       for (int64_t d = 0; d < (int64_t)L.alive.size(); ++d) {
         if (!L.alive[d]) continue;
+        const uint32_t* dv = &L.v[4 * d];
         for (int h = 0; h < 4; ++h) {
-          const uint32_t* dv = &L.v[4 * d];
           FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
           auto it = fmap.find(k);
           if (it == fmap.end()) fmap.emplace(k, std::array<int64_t, 2>{4 * d + h, -1});
           else it->second[1] = 4 * d + h;
         }
       }
-      // boundary faces of R and their inner D tets
-      std::unordered_map<FaceKey, int64_t, FaceHash> bface;  // key -> outside (t*4+f) or -1
+      // boundary faces of R -> outside neighbour (t << 5 | f, or kInvalid)
+      std::unordered_map<FaceKey, uint32_t, FaceHash> bface;
       std::vector<int64_t> seeds;
+      std::vector<uint32_t> grow_with;
       bool missing = false;
-      std::vector<int64_t> grow_with;
-      for (int64_t x : R)
+      for (uint32_t x : R)
         for (int h = 0; h < 4; ++h) {
-          int64_t nb = W.ta[4 * x + h];
-          if (nb >= 0 && W.inR[nb >> 2] == stamp) continue;
-          const uint32_t* xv = &W.tv[4 * x];
+          uint32_t nb = W.ta(x, h);
+          if (nb != kInvalid && inR.count(nb >> 5)) continue;
+          const uint32_t* xv = W.tv(x);
           uint32_t fa = xv[face_vi(h, 0)], fb = xv[face_vi(h, 1)], fc = xv[face_vi(h, 2)];
           FaceKey k = face_key(fa, fb, fc);
           auto it = fmap.find(k);
           if (it == fmap.end()) {
             missing = true;
-            if (nb >= 0) grow_with.push_back(nb >> 2);
+            if (nb != kInvalid) grow_with.push_back(nb >> 5);
             continue;
           }
           bface.emplace(k, nb);
-          // the D tet on R's side: its apex lies on the inner side of (fa, fb, fc)
           for (int s2 = 0; s2 < 2; ++s2) {
             int64_t df = it->second[s2];
             if (df < 0) continue;
@@ -366,18 +357,20 @@ wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>
           }
         }
       if (missing) {
-        for (int64_t x : grow_with) add(x);
+        for (uint32_t x : grow_with) add(x);
         continue;
       }
-      // flood the inside of R in D
-      std::unordered_map<int64_t, int64_t> inner;  // D tet -> new mesh tet
+      // inner tets of D
+      std::unordered_map<int64_t, uint32_t> inner;
+      std::vector<int64_t> order;
       std::vector<int64_t> st = seeds;
-      for (int64_t d : seeds) inner.emplace(d, -1);
+      for (int64_t d : seeds)
+        if (inner.emplace(d, kInvalid).second) order.push_back(d);
       while (!st.empty()) {
         int64_t d = st.back();
         st.pop_back();
+        const uint32_t* dv = &L.v[4 * d];
         for (int h = 0; h < 4; ++h) {
-          const uint32_t* dv = &L.v[4 * d];
           FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
           if (bface.count(k)) continue;
           int64_t nb = L.a[4 * d + h];
@@ -385,34 +378,54 @@ wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>
             if (rep) rep->message = "region flood reached the hull";
             return WTET_ERR_INTERNAL;
           }
-          if (inner.emplace(nb >> 2, -1).second) st.push_back(nb >> 2);
+          if (inner.emplace(nb >> 2, kInvalid).second) {
+            st.push_back(nb >> 2);
+            order.push_back(nb >> 2);
+          }
         }
       }
-      // replace R by the inner tets of D
-      for (int64_t x : R) W.alive[x] = 0;
-      for (auto& kv : inner) {
-        int64_t nt = (int64_t)W.alive.size();
-        kv.second = nt;
-        W.tv.insert(W.tv.end(), L.v.begin() + 4 * kv.first, L.v.begin() + 4 * kv.first + 4);
-        W.ta.insert(W.ta.end(), 4, -1);
-        W.alive.push_back(1);
-        W.inR.push_back(0);
+      // slots: R's slots first, then appended ones; leftovers become holes
+      size_t k = 0;
+      for (int64_t d : order) {
+        uint32_t slot;
+        if (k < R.size()) {
+          slot = R[k++];
+        } else {
+          if (mesh.count >= mesh.cap) {
+            if (rep) rep->message = "repair needs more room than the staging buffer has";
+            return WTET_ERR_OUT_OF_MEMORY;
+          }
+          slot = (uint32_t)mesh.count++;
+          for (int h = 0; h < 4; ++h) W.ta(slot, h) = kInvalid;
+        }
+        inner[d] = slot;
+        changed.push_back(slot);
       }
-      for (auto& kv : inner) {
-        int64_t d = kv.first, nt = kv.second;
+      for (; k < R.size(); ++k) {
+        W.tv(R[k])[0] = kInvalid;
+        holes.push_back(R[k]);
+      }
+      for (int64_t d : order) {
+        uint32_t slot = inner[d];
+        std::memcpy(W.tv(slot), &L.v[4 * d], 16);
+      }
+      for (int64_t d : order) {
+        uint32_t slot = inner[d];
+        const uint32_t* dv = &L.v[4 * d];
         for (int h = 0; h < 4; ++h) {
-          const uint32_t* dv = &L.v[4 * d];
-          FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
-          auto bf = bface.find(k);
+          FaceKey fk = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
+          auto bf = bface.find(fk);
           if (bf != bface.end()) {
-            int64_t out = bf->second;
-            W.ta[4 * nt + h] = out;
-            if (out >= 0) W.ta[out] = 4 * nt + h;
-            queue.emplace_back(nt, h);
+            uint32_t out = bf->second;
+            W.ta(slot, h) = out;
+            if (out != kInvalid) {
+              W.ta(out >> 5, (int)(out & 3u)) = (slot << 5) | (uint32_t)h;
+              changed.push_back(out >> 5);
+            }
+            queue.emplace_back(slot, h);
           } else {
             int64_t nb = L.a[4 * d + h];
-            int64_t u = inner.at(nb >> 2);
-            W.ta[4 * nt + h] = 4 * u + (nb & 3);
+            W.ta(slot, h) = (inner.at(nb >> 2) << 5) | (uint32_t)(nb & 3);
           }
         }
       }
@@ -423,24 +436,37 @@ wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>
       done = true;
     }
   }
-  // compact
-  std::vector<int64_t> idx(W.alive.size(), -1);
-  int64_t m = 0;
-  for (size_t i = 0; i < W.alive.size(); ++i)
-    if (W.alive[i]) idx[i] = m++;
-  tets.assign(4 * (size_t)m, 0);
-  adj.assign(4 * (size_t)m, kInvalid);
-  for (size_t i = 0; i < W.alive.size(); ++i) {
-    if (!W.alive[i]) continue;
-    int64_t j = idx[i];
-    for (int h = 0; h < 4; ++h) {
-      tets[4 * j + h] = W.tv[4 * i + h];
-      int64_t nb = W.ta[4 * i + h];
-      if (nb >= 0 && idx[nb >> 2] >= 0) adj[4 * j + h] = ((uint32_t)idx[nb >> 2] << 5) | (uint32_t)(nb & 3);
+  // fill holes with tets moved from the end
+  std::sort(holes.begin(), holes.end());
+  auto trim = [&]() {
+    while (mesh.count > 0 && W.dead((uint32_t)(mesh.count - 1))) --mesh.count;
+  };
+  for (uint32_t h : holes) {
+    trim();
+    if (h >= mesh.count) continue;
+    uint32_t last = (uint32_t)(mesh.count - 1);
+    std::memcpy(W.tv(h), W.tv(last), 16);
+    for (int f = 0; f < 4; ++f) {
+      uint32_t e = W.ta(last, f);
+      W.ta(h, f) = e;
+      if (e != kInvalid) {
+        W.ta(e >> 5, (int)(e & 3u)) = (h << 5) | (uint32_t)f;
+        changed.push_back(e >> 5);
+      }
     }
+    changed.push_back(h);
+    --mesh.count;
   }
+  trim();
+  std::sort(changed.begin(), changed.end());
+  changed.erase(std::unique(changed.begin(), changed.end()), changed.end());
+  while (!changed.empty() && changed.back() >= mesh.count) changed.pop_back();
   return WTET_OK;
 }
+
+namespace {
+using namespace wtet;
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // check_quality (src/python.rs:282-339 -> src/lib.rs:25-78)
@@ -552,8 +578,13 @@ extern "C" wtet_status wtet_repair(const float* xyzn, uint32_t n, uint32_t** tet
     }
     RepairReport rr;
     wtet_status s = WTET_OK;
-    if (!viol.empty()) s = repair_delaunay(xyzn, n, tv, ta, viol, mode, &rr);
+    tv.resize(tv.size() + 4 * 65536);
+    ta.resize(ta.size() + 4 * 65536);
+    MeshBuf mb{tv.data(), ta.data(), M, M + 65536, {}};
+    if (!viol.empty()) s = repair_delaunay(xyzn, n, mb, viol, mode, &rr);
     if (s != WTET_OK) return s;
+    tv.resize(4 * mb.count);
+    ta.resize(4 * mb.count);
     uint32_t* ot = (uint32_t*)std::malloc(std::max<size_t>(tv.size(), 1) * 4);
     uint32_t* oa = (uint32_t*)std::malloc(std::max<size_t>(ta.size(), 1) * 4);
     if (!ot || !oa) {

@@ -17,10 +17,21 @@ struct RepairReport {
   std::string message;
 };
 
-// xyzn: n normalized points (input id order).  tets/adj: the whole mesh in
-// the result format (input ids, super n..n+3, infinity n+4; adjacency
-// (t << 5) | f).  viol: (t << 2) | f of non-Delaunay faces.  On success the
-// arrays hold the exact Delaunay triangulation.
-wtet_status repair_delaunay(const float* xyzn, uint32_t n, std::vector<uint32_t>& tets,
-                            std::vector<uint32_t>& adj, const std::vector<uint32_t>& viol,
-                            int mode, RepairReport* rep);
+// The mesh in the result format (input ids, super n..n+3, infinity n+4;
+// adjacency (t << 5) | f) in caller-owned buffers with room for `cap` tets.
+// The repair edits it in place: new tets reuse freed slots or are appended,
+// holes are filled from the end; `changed` lists every slot whose tet or
+// adjacency row was written (sorted, < count on return).
+struct MeshBuf {
+  uint32_t* tets;
+  uint32_t* adj;
+  size_t count;
+  size_t cap;
+  std::vector<uint32_t> changed;
+};
+
+// xyzn: n normalized points (input id order).  viol: (t << 2) | f of
+// non-Delaunay faces.  On success the mesh is the exact Delaunay
+// triangulation.
+wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
+                            const std::vector<uint32_t>& viol, int mode, RepairReport* rep);
