@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the exact-only comparison")
+    ap.add_argument("--ring", type=int, default=1, help="insertion independence radius (0, 1, 2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -60,6 +61,7 @@ def main():
     pts = W.uniform(args.n, args.seed)
     ctx = wtet_amd.Context(0)
     d_pts = ctx.upload(pts)  # resident in HBM before any timed region
+    ctx.set_insertion_ring(args.ring)
     filt = wtet_amd.Config(predicate_mode="filtered")
     exact = wtet_amd.Config(predicate_mode="exact")
 
@@ -141,7 +143,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"uniform_{args.n}_seed{args.seed}", "points": args.n,
                    "distribution": "i.i.d. U[0,1)^3 f32, numpy PCG64", "predicates": "f32->f64->exact",
-                   "batched": False},
+                   "batched": False, "insertion_ring": args.ring},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_exact": plain,

@@ -137,10 +137,18 @@ wtet_status wtet_device_upload(wtet_ctx *ctx, const void *host, size_t bytes, vo
 wtet_status wtet_device_download(wtet_ctx *ctx, const void *d_src, void *host_dst, size_t bytes);
 wtet_status wtet_device_free(wtet_ctx *ctx, void *d_ptr);
 
+/* Points inserted in one round must lie in tets at least `ring`+1
+ * face-steps apart (ring 0: one point per tet, as the reference's vote,
+ * src/shaders/vote.wgsl:235-344; default 1).  A wider ring leaves fewer
+ * faces that concurrent flipping cannot fix, at the cost of more rounds.
+ * The triangulation does not depend on it.  No counterpart in the
+ * reference. */
+wtet_status wtet_set_insertion_ring(wtet_ctx *ctx, int ring);
+
 /* Per-kernel-group timing with HIP events on the context's stream (the
  * reference's GpuProfiler/CpuProfiler scopes, src/profiler.rs:5-224).
  * wtet_set_profiling resets the totals; kinds 0..11 are prepare_points,
- * vote, pick, split, split_points, flip_check, flip_decide, flip_exec,
+ * vote, pick, split, split_points, flip_check, (unused), flip_decide_exec,
  * flip_update, relocate, audit, output.  Totals accumulate over calls. */
 wtet_status wtet_set_profiling(wtet_ctx *ctx, int on);
 wtet_status wtet_kernel_times(const wtet_ctx *ctx, uint32_t kind, const char **name,

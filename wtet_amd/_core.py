@@ -121,6 +121,7 @@ def lib():
             L.wtet_version.restype = ctypes.c_char_p
             L.wtet_device_info.argtypes = [P, ctypes.c_char_p, u32]
             L.wtet_set_profiling.argtypes = [P, ctypes.c_int]
+            L.wtet_set_insertion_ring.argtypes = [P, ctypes.c_int]
             L.wtet_device_upload.argtypes = [P, P, ctypes.c_size_t, ctypes.POINTER(P)]
             L.wtet_device_download.argtypes = [P, P, P, ctypes.c_size_t]
             L.wtet_device_free.argtypes = [P, P]
@@ -188,6 +189,11 @@ class Context:
 
     def free(self, ptr: int):
         lib().wtet_device_free(self._h, ctypes.c_void_p(ptr))
+
+    def set_insertion_ring(self, ring: int):
+        """Insertion independence radius in face-steps (0, 1 or 2)."""
+        if lib().wtet_set_insertion_ring(self._h, int(ring)) != 0:
+            raise ValueError("ring must be 0, 1 or 2")
 
     def set_profiling(self, on: bool = True):
         lib().wtet_set_profiling(self._h, int(on))

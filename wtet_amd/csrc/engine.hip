@@ -60,6 +60,7 @@ struct wtet_ctx {
   // per-kernel timing with HIP events on the context's stream
   // (src/profiler.rs GpuProfiler scopes, here actually recorded)
   bool profiling = false;
+  int ring = 1;  // insertion independence radius (face-steps), see is_winner
   std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
   std::vector<int> evkind;
   size_t evused = 0;
@@ -148,7 +149,6 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
   ensure(c, c->act[0], cap * 4, keep);
   ensure(c, c->act[1], cap * 4, keep);
   ensure(c, c->prop, cap * 16, false);
-  ensure(c, c->flips, cap * 16, false);
   ensure(c, c->fout, cap * 16, false);
   ensure(c, c->slot2idx, cap * 4, false);
   ensure(c, c->blk, (cap / kBlock + 2) * 4, false);
@@ -161,7 +161,7 @@ enum KernelKind : int {
   K_FLIP_EXEC, K_FLIP_UPDATE, K_RELOCATE, K_AUDIT, K_OUTPUT, K_NKINDS
 };
 const char* kKernelNames[K_NKINDS] = {"prepare_points", "vote", "pick", "split", "split_points",
-                                      "flip_check", "flip_decide", "flip_exec", "flip_update",
+                                      "flip_check", "unused", "flip_decide_exec", "flip_update",
                                       "relocate", "audit", "output"};
 
 // brackets the launches issued by fn with an event pair when profiling
@@ -313,10 +313,8 @@ uint32_t flip_iteration(wtet_ctx* c, uint32_t n, int mode, uint32_t& ac, uint32_
   uint32_t* cur = P<uint32_t>(c->act[ac]);
   uint32_t* nxt = P<uint32_t>(c->act[ac ^ 1]);
   timed(c, K_FLIP_CHECK, [&] { launch_flip_check(M, cur, P<uint4>(c->prop), A, it, st, c->stream); });
-  timed(c, K_FLIP_DECIDE,
-        [&] { launch_flip_decide(M, P<uint4>(c->prop), A, it, P<uint4>(c->flips), nxt, c->stream); });
   timed(c, K_FLIP_EXEC, [&] {
-    launch_flip_exec(M, P<uint4>(c->flips), A, it, round, P<uint4>(c->fout), nxt, c->stream);
+    launch_flip_decide_exec(M, P<uint4>(c->prop), A, it, round, P<uint4>(c->fout), nxt, c->stream);
   });
   timed(c, K_FLIP_UPDATE, [&] { launch_flip_update(M, P<uint4>(c->fout), A, it, c->stream); });
   HIP_CHECK(hipGetLastError());
@@ -388,11 +386,12 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
     if (round > std::max<uint32_t>(max_rounds, 4096u))
       throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
     Mesh M = mesh_of(c, n, mode);
+    launch_set_ctr(M.ctr, C_ULEN, U, -1, 0u, s);
     timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
     timed(c, K_PICK, [&] {
       launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round,
                   P<uint2>(c->wlist), P<uint32_t>(c->uvert[cur ^ 1]),
-                  P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), s);
+                  P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), c->ring, s);
     });
     HIP_CHECK(hipGetLastError());
     read_ctr(c);
@@ -801,6 +800,12 @@ wtet_status wtet_device_free(wtet_ctx* c, void* d_ptr) {
     (void)hipGetLastError();
     return WTET_ERR_DEVICE;
   }
+  return WTET_OK;
+}
+
+wtet_status wtet_set_insertion_ring(wtet_ctx* c, int ring) {
+  if (!c || ring < 0 || ring > 2) return WTET_ERR_INVALID_ARGUMENT;
+  c->ring = ring;
   return WTET_OK;
 }
 

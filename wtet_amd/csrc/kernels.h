@@ -29,19 +29,18 @@ void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n,
 // rounds
 void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                  hipStream_t s);
+// ring: insert only where no tet within `ring` face-steps has a smaller vote
 void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                  uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
-                 uint32_t* blk, hipStream_t s);
+                 uint32_t* blk, int ring, hipStream_t s);
 void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round,
                   uint32_t* act_next, hipStream_t s);
 void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
                          uint32_t round, uint64_t* st, hipStream_t s);
 void launch_flip_check(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t A,
                        uint32_t it, uint64_t* st, hipStream_t s);
-void launch_flip_decide(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it, uint4* flips,
-                        uint32_t* act_next, hipStream_t s);
-void launch_flip_exec(const Mesh& M, const uint4* flips, uint32_t F, uint32_t it,
-                      uint32_t round, uint4* out, uint32_t* act_next, hipStream_t s);
+void launch_flip_decide_exec(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it,
+                             uint32_t round, uint4* out, uint32_t* act_next, hipStream_t s);
 void launch_flip_update(const Mesh& M, const uint4* out, uint32_t F, uint32_t it,
                         hipStream_t s);
 void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,

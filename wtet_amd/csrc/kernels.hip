@@ -81,6 +81,7 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* blk, uint32_t nb
     uint32_t total = part[1023];
     if (mode == 0) {
       ctr[C_WIN] = total;
+      ctr[C_SURV] = ctr[C_ULEN] - total;
       ctr[C_BASE] = ctr[C_TETS];
       ctr[C_TETS] += 3u * total;
     } else if (mode == 1) {
@@ -300,9 +301,9 @@ __global__ void k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint
   if (pending) at_min_u64(&M.tvote[t], key);
 }
 
-__global__ void k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t* blk) {
+__global__ void k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t* blk, int ring) {
   uint32_t i = gid();
-  bool w = i < U && is_winner(M, utet, i);
+  bool w = i < U && is_winner(M, utet, i, ring);
   uint32_t tot;
   block_rank(w, &tot);
   if (threadIdx.x == 0) blk[blockIdx.x] = tot;
@@ -310,9 +311,9 @@ __global__ void k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t*
 
 __global__ void k_pick_scatter(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                                uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
-                               const uint32_t* blk) {
+                               const uint32_t* blk, int ring) {
   uint32_t i = gid();
-  bool w = i < U && is_winner(M, utet, i);
+  bool w = i < U && is_winner(M, utet, i, ring);
   uint32_t tot;
   uint32_t r = blk[blockIdx.x] + block_rank(w, &tot);
   if (i >= U) return;
@@ -331,13 +332,21 @@ void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uin
   k_vote<<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U);
 }
 
+// votes of tets whose winner was deferred must not survive into the next
+// round (list positions change); split resets the votes of winning tets
+__global__ void k_clear_votes(Mesh M, const uint32_t* utet2, const uint32_t* ctr) {
+  uint32_t i = gid();
+  if (i < ctr[C_SURV]) M.tvote[utet2[i]] = kVoteNone;
+}
+
 void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                  uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2, uint32_t* blk,
-                 hipStream_t s) {
+                 int ring, hipStream_t s) {
   uint32_t nb = nblocks(U);
-  k_pick_count<<<nb, kBlock, 0, s>>>(M, utet, U, blk);
+  k_pick_count<<<nb, kBlock, 0, s>>>(M, utet, U, blk, ring);
   k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, M.ctr, 0);
-  k_pick_scatter<<<nb, kBlock, 0, s>>>(M, uvert, utet, U, round, wlist, uvert2, utet2, blk);
+  k_pick_scatter<<<nb, kBlock, 0, s>>>(M, uvert, utet, U, round, wlist, uvert2, utet2, blk, ring);
+  if (ring > 0) k_clear_votes<<<nb, kBlock, 0, s>>>(M, utet2, M.ctr);
 }
 
 __global__ void k_split(Mesh M, const uint2* wlist, uint32_t W, uint32_t round,
@@ -375,20 +384,14 @@ __global__ void k_flip_check(Mesh M, const uint32_t* act, uint4* prop, uint32_t 
   flush_stats(ls, st);
 }
 
-__global__ void k_flip_decide(Mesh M, const uint4* prop, uint32_t A, uint32_t it, uint4* flips,
-                              uint32_t* act_next) {
+__global__ void k_flip_decide_exec(Mesh M, const uint4* prop, uint32_t A, uint32_t it,
+                                   uint32_t round, uint4* out, uint32_t* act_next) {
   uint32_t j = gid();
-  if (j < A) flip_decide_body(M, prop, j, it, flips, act_next);
+  if (j < A) flip_decide_exec_body(M, prop, j, it, round, out, act_next);
 }
 
 // F is an upper bound (the active-list length); the flip count itself is
 // read from the device counter, so the host does not wait for it.
-__global__ void k_flip_exec(Mesh M, const uint4* flips, uint32_t F, uint32_t it, uint32_t round,
-                            uint4* out, uint32_t* act_next) {
-  uint32_t k = gid();
-  if (k < F && k < M.ctr[C_FLIPS]) flip_exec_body(M, flips, k, it, round, out, act_next);
-}
-
 __global__ void k_flip_update(Mesh M, const uint4* out, uint32_t F, uint32_t it) {
   uint32_t k = gid();
   if (k < F && k < M.ctr[C_FLIPS]) flip_update_body(M, out, k, it);
@@ -410,13 +413,9 @@ void launch_flip_check(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t
                        uint64_t* st, hipStream_t s) {
   if (A) k_flip_check<<<nblocks(A), kBlock, 0, s>>>(M, act, prop, A, it, st);
 }
-void launch_flip_decide(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it, uint4* flips,
-                        uint32_t* act_next, hipStream_t s) {
-  if (A) k_flip_decide<<<nblocks(A), kBlock, 0, s>>>(M, prop, A, it, flips, act_next);
-}
-void launch_flip_exec(const Mesh& M, const uint4* flips, uint32_t F, uint32_t it, uint32_t round,
-                      uint4* out, uint32_t* act_next, hipStream_t s) {
-  if (F) k_flip_exec<<<nblocks(F), kBlock, 0, s>>>(M, flips, F, it, round, out, act_next);
+void launch_flip_decide_exec(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it,
+                             uint32_t round, uint4* out, uint32_t* act_next, hipStream_t s) {
+  if (A) k_flip_decide_exec<<<nblocks(A), kBlock, 0, s>>>(M, prop, A, it, round, out, act_next);
 }
 void launch_flip_update(const Mesh& M, const uint4* out, uint32_t F, uint32_t it,
                         hipStream_t s) {

@@ -26,6 +26,8 @@ enum Ctr : int {
   C_BADTET = 9,    // points found in a tet touching infinity (must stay 0)
   C_NPTS = 10,     // points to insert (input minus exact duplicates)
   C_DUPSCAN = 11,  // duplicate scans that hit their length cap
+  C_ULEN = 12,     // uninserted list length at the start of the round
+  C_SURV = 13,     // points left after this round's insertions
   C_COUNT = 16
 };
 
@@ -107,8 +109,30 @@ WT_HD void vote_body(const Mesh& M, const uint32_t* uvert, const uint32_t* utet,
   if (key != kVoteNone) at_min_u64(&M.tvote[utet[i]], key);
 }
 
-WT_HD bool is_winner(const Mesh& M, const uint32_t* utet, uint32_t i) {
-  return (uint32_t)(M.tvote[utet[i]] & 0xFFFFFFFFull) == i;
+// Point i is inserted this round when it holds its tet's vote and no tet
+// within `ring` face-steps holds a smaller key.  Inserting only into
+// non-adjacent tets keeps the stars of simultaneous insertions apart, which
+// is what leaves concurrent flipping unable to finish (faces that are
+// neither 2-3 nor 3-2 flippable); the deferred points win a later round.
+// Reads votes only, so every thread sees the same decision.
+WT_HD bool is_winner(const Mesh& M, const uint32_t* utet, uint32_t i, int ring) {
+  uint32_t t = utet[i];
+  uint64_t kt = M.tvote[t];
+  if ((uint32_t)(kt & 0xFFFFFFFFull) != i) return false;
+  if (ring <= 0) return true;
+  uint4 to = M.opp[t];
+  for (int f = 0; f < 4; ++f) {
+    uint32_t nt = opp_tet(get4(to, f));
+    if (M.tvote[nt] < kt) return false;
+    if (ring > 1) {
+      uint4 no = M.opp[nt];
+      for (int g = 0; g < 4; ++g) {
+        uint32_t n2 = opp_tet(get4(no, g));
+        if (n2 != t && M.tvote[n2] < kt) return false;
+      }
+    }
+  }
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -254,21 +278,6 @@ WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint
   }
 }
 
-WT_HD void flip_decide_body(const Mesh& M, const uint4* prop, uint32_t j, uint32_t it,
-                            uint4* flips, uint32_t* act_next) {
-  uint4 pr = prop[j];
-  if (pr.x == kInvalid) return;
-  uint64_t key = flip_key(it, pr.x);
-  bool win = M.fvote[pr.x] == key && M.fvote[pr.z] == key;
-  if (((pr.y >> 2) & 1u) && M.fvote[pr.w] != key) win = false;
-  if (win) {
-    flips[append_slot(&M.ctr[C_FLIPS])] = pr;
-  } else {
-    uint32_t old = at_or_u32(&M.tinfo[pr.x], kQueued);
-    if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = pr.x;
-  }
-}
-
 WT_HD void queue_tet(const Mesh& M, uint32_t s, uint32_t* act_next) {
   uint32_t old = at_or_u32(&M.tinfo[s], kAlive | kQueued);
   if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = s;
@@ -281,9 +290,8 @@ WT_HD void queue_tet(const Mesh& M, uint32_t s, uint32_t* act_next) {
 // them once every flip of the iteration is written.
 // out: x,y,z = slots of the new tets (z unused for 3-2), w = kind
 // ---------------------------------------------------------------------------
-WT_HD void flip_exec_body(const Mesh& M, const uint4* flips, uint32_t k, uint32_t it,
-                          uint32_t round, uint4* out, uint32_t* act_next) {
-  uint4 pr = flips[k];
+WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t round,
+                          uint4* outk, uint32_t* act_next) {
   uint32_t t = pr.x, f = pr.y & 3u, kind = (pr.y >> 2) & 1u, edge = (pr.y >> 3) & 3u;
   uint32_t n = pr.z;
   uint4 tv = M.tet[t], to = M.opp[t];
@@ -325,7 +333,7 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4* flips, uint32_t k, uint32_
     queue_tet(M, Txy, act_next);
     queue_tet(M, Tyz, act_next);
     act_next[append_slot(&M.ctr[C_ACT_NEXT])] = Tzx;
-    out[k] = make_uint4(Txy, Tyz, Tzx, 0u);
+    *outk = make_uint4(Txy, Tyz, Tzx, 0u);
   } else {
     // rotate the face triple so the offending edge is (x, y), third z
     uint32_t x = get4(tv, pos[edge]), y = get4(tv, pos[(edge + 1) % 3]),
@@ -362,7 +370,27 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4* flips, uint32_t k, uint32_
     M.tfwd[m] = A;
     queue_tet(M, A, act_next);
     queue_tet(M, B, act_next);
-    out[k] = make_uint4(A, B, kInvalid, 1u);
+    *outk = make_uint4(A, B, kInvalid, 1u);
+  }
+}
+
+// A proposal wins when it holds the vote of every tet it involves; the
+// winner executes at once (its tets belong to no other winner), losers go
+// back to the active list (src/shaders/mark_rejected_flips.wgsl:61-178 and
+// src/shaders/flip.wgsl:531-845, one pass here).
+WT_HD void flip_decide_exec_body(const Mesh& M, const uint4* prop, uint32_t j, uint32_t it,
+                                 uint32_t round, uint4* out, uint32_t* act_next) {
+  uint4 pr = prop[j];
+  if (pr.x == kInvalid) return;
+  uint64_t key = flip_key(it, pr.x);
+  bool win = M.fvote[pr.x] == key && M.fvote[pr.z] == key;
+  if (((pr.y >> 2) & 1u) && M.fvote[pr.w] != key) win = false;
+  if (win) {
+    uint32_t k = append_slot(&M.ctr[C_FLIPS]);
+    flip_exec_body(M, pr, it, round, &out[k], act_next);
+  } else {
+    uint32_t old = at_or_u32(&M.tinfo[pr.x], kQueued);
+    if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = pr.x;
   }
 }
 
