@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the exact-only comparison")
     ap.add_argument("--ring", type=int, default=1, help="insertion independence radius (0, 1, 2)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-cloud batched leg")
+    ap.add_argument("--batch-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -112,6 +114,46 @@ def main():
         plain = {"value": args.n / (ms_plain / 1e3), "ms_per_step": round(ms_plain, 3)}
         print(f"exact-only: {ms_plain:.2f} ms/step", file=sys.stderr, flush=True)
 
+    batch = None
+    if not args.no_batch:
+        # 4,096 independent clouds (log-normal sizes, median 2,000, Gaussian
+        # mixtures): one batched pass vs single-instance calls per cloud
+        bp, boff = W.mixed_batch(4096, 4)
+        d_b = ctx.upload(bp)
+        nb = len(bp)
+        ctx.delaunay_batch_resident(d_b, boff, filt)
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            ctx.delaunay_batch_resident(d_b, boff, filt)
+        ms_b = (time.perf_counter() - t0) * 1e3 / args.batch_steps
+        bst = ctx.stats()
+        print(f"batched: {ms_b:.2f} ms/step, {nb / (ms_b / 1e3):.4g} points/s", file=sys.stderr, flush=True)
+
+        def singles(cfg, clouds):
+            t0 = time.perf_counter()
+            pts_done = 0
+            for c in clouds:
+                a, b = int(boff[c]), int(boff[c + 1])
+                ctx.delaunay_resident(d_b + 12 * a, b - a, cfg)
+                pts_done += b - a
+            return pts_done / (time.perf_counter() - t0)
+
+        sample = list(range(256))
+        singles(filt, sample[:8])
+        single_filt = singles(filt, sample)
+        single_plain = singles(exact, sample)
+        ctx.free(d_b)
+        batch = {"workload": "mixed_batch_4096_seed4", "clouds": 4096, "points": nb,
+                 "value": round(nb / (ms_b / 1e3), 1), "unit": "points/s", "ms_per_step": round(ms_b, 3),
+                 "steps": args.batch_steps,
+                 "single_instance_filtered": round(single_filt, 1),
+                 "single_instance_plain_exact": round(single_plain, 1),
+                 "single_instance_sample": "clouds 0..255, one resident call each",
+                 "speedup_vs_single_plain": round(nb / (ms_b / 1e3) / single_plain, 2),
+                 "ms_device": round(bst["ms_device"], 3), "ms_repair": round(bst["ms_repair"], 3),
+                 "violations_gpu": bst["violations_gpu"], "rounds": bst["rounds"],
+                 "flip_iterations": bst["flip_iterations"]}
+
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -148,6 +190,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_exact": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
+        "batched": batch,
         "parity_with_oracle": parity,
         "stage_fractions": stages,
         "ms_device": round(st_t["ms_device"], 3),

@@ -112,6 +112,53 @@ typedef struct wtet_device_result {
 wtet_status wtet_delaunay_resident(wtet_ctx *ctx, const float *d_xyz, uint32_t n,
                                    const wtet_config *cfg, wtet_device_result *out);
 
+/* Many independent clouds in one pass: cloud c is the points
+ * [offsets[c], offsets[c+1]) of xyz (num_clouds+1 offsets, offsets[0] = 0,
+ * at least 4 points per cloud), normalized on its own, and its result is the
+ * triangulation wtet_delaunay gives for those points alone (cloud-local
+ * vertex ids: input id within the cloud, super n_c..n_c+3, infinity n_c+4).
+ * Every kernel launch covers all clouds.  No counterpart in the reference,
+ * which triangulates one cloud per delaunay_3d call (src/lib.rs:160-310);
+ * the per-cloud semantics are those of that call. */
+typedef struct wtet_batch_result {
+  uint32_t num_clouds;
+  uint64_t num_tets;        /* rows over all clouds */
+  uint32_t *tets;           /* num_tets*4 */
+  uint32_t *adjacency;      /* num_tets*4: (row within the cloud << 5) | face */
+  uint64_t *tet_offsets;    /* num_clouds+1: cloud c owns rows [tet_offsets[c], tet_offsets[c+1]) */
+  uint64_t num_failed;
+  uint32_t *failed_verts;   /* per cloud, as wtet_result.failed_verts */
+  uint64_t *failed_offsets; /* num_clouds+1 */
+} wtet_batch_result;
+
+wtet_status wtet_delaunay_batch(wtet_ctx *ctx, const float *xyz, const uint64_t *offsets,
+                                uint32_t num_clouds, const wtet_config *cfg,
+                                wtet_batch_result *out);
+/* Same with the points already in device memory (offsets in host memory). */
+wtet_status wtet_delaunay_batch_device(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
+                                       uint32_t num_clouds, const wtet_config *cfg,
+                                       wtet_batch_result *out);
+void wtet_batch_result_free(wtet_batch_result *res);
+
+/* Points and result both in device memory.  Cloud c owns the device rows
+ * [tet_start[c], tet_start[c] + tet_count[c]); rows of different clouds do
+ * not overlap but need not be in cloud order (a cloud the host repair
+ * rewrote is placed after the others).  Device arrays and the two host
+ * arrays are owned by the context and valid until its next call. */
+typedef struct wtet_batch_device_result {
+  uint32_t num_clouds;
+  uint64_t rows;              /* rows used in the device arrays */
+  uint32_t *d_tets;
+  uint32_t *d_adjacency;
+  const uint64_t *tet_start;  /* host, num_clouds */
+  const uint32_t *tet_count;  /* host, num_clouds */
+  uint64_t num_failed;
+} wtet_batch_device_result;
+
+wtet_status wtet_delaunay_batch_resident(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
+                                         uint32_t num_clouds, const wtet_config *cfg,
+                                         wtet_batch_device_result *out);
+
 /* Counters of the last wtet_delaunay* call on this context. */
 wtet_status wtet_get_stats(const wtet_ctx *ctx, wtet_stats *out);
 

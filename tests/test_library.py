@@ -161,3 +161,28 @@ def test_python_surface_errors_and_reprs():
                                 np.zeros(0, np.uint32))
     assert repr(r) == "DelaunayResult(num_tets=7, failed_verts=0)"
     assert wtet_amd.__version__ == "0.1.0"
+
+
+def test_batch_input_validation():
+    from wtet_amd import _core
+    with pytest.raises(ValueError, match="at least 4 points"):
+        _core._batch_input([np.zeros((3, 3), np.float32)], None)
+    with pytest.raises(ValueError, match=r"offsets\[-1\]"):
+        _core._batch_input(np.zeros((10, 3), np.float32), np.array([0, 5, 9]))
+    with pytest.raises(ValueError, match="nondecreasing"):
+        _core._check_offsets([0, 8, 4])
+    with pytest.raises(ValueError, match="starting at 0"):
+        _core._check_offsets([1, 8])
+    with pytest.raises(ValueError, match=r"shape \(N, 3\)"):
+        _core._batch_input([np.zeros((5, 2), np.float32)], None)
+    pts, off = _core._batch_input([np.zeros((4, 3)), np.ones((5, 3))], None)
+    assert off.tolist() == [0, 4, 9] and pts.dtype == np.float32 and pts.shape == (9, 3)
+
+
+def test_mixed_batch_workload_shape():
+    from wtet_amd import workloads as W
+    pts, off = W.mixed_batch(64, 9)
+    sizes = np.diff(off.astype(np.int64))
+    assert len(off) == 65 and off[0] == 0 and int(off[-1]) == len(pts)
+    assert sizes.min() >= 64 and sizes.max() <= 50000
+    assert pts.dtype == np.float32 and pts.min() >= 0 and pts.max() <= 1

@@ -2,6 +2,7 @@
 reference's ``wtet`` Python package (python/wtet/__init__.py:47-69)."""
 from ._core import (  # noqa: F401
     INVALID,
+    BatchResult,
     Config,
     Context,
     DelaunayResult,
@@ -9,6 +10,7 @@ from ._core import (  # noqa: F401
     check_quality,
     decode_adjacency,
     delaunay,
+    delaunay_batch,
     encode_adjacency,
     gpu_info,
     initialize_gpu,
@@ -26,4 +28,6 @@ __all__ = [
     "INVALID",
     "__version__",
     "Context",
+    "delaunay_batch",
+    "BatchResult",
 ]

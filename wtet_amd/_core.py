@@ -64,6 +64,31 @@ class _DevResult(ctypes.Structure):
     ]
 
 
+class _BatchResult(ctypes.Structure):
+    _fields_ = [
+        ("num_clouds", ctypes.c_uint32),
+        ("num_tets", ctypes.c_uint64),
+        ("tets", ctypes.POINTER(ctypes.c_uint32)),
+        ("adjacency", ctypes.POINTER(ctypes.c_uint32)),
+        ("tet_offsets", ctypes.POINTER(ctypes.c_uint64)),
+        ("num_failed", ctypes.c_uint64),
+        ("failed_verts", ctypes.POINTER(ctypes.c_uint32)),
+        ("failed_offsets", ctypes.POINTER(ctypes.c_uint64)),
+    ]
+
+
+class _BatchDevResult(ctypes.Structure):
+    _fields_ = [
+        ("num_clouds", ctypes.c_uint32),
+        ("rows", ctypes.c_uint64),
+        ("d_tets", ctypes.c_void_p),
+        ("d_adjacency", ctypes.c_void_p),
+        ("tet_start", ctypes.POINTER(ctypes.c_uint64)),
+        ("tet_count", ctypes.POINTER(ctypes.c_uint32)),
+        ("num_failed", ctypes.c_uint64),
+    ]
+
+
 class _Stats(ctypes.Structure):
     _fields_ = [
         ("pred", ctypes.c_uint64 * 8),
@@ -113,6 +138,13 @@ def lib():
             L.wtet_result_free.argtypes = [ctypes.POINTER(_Result)]
             L.wtet_delaunay_resident.argtypes = [P, P, u32, ctypes.POINTER(_Config),
                                                  ctypes.POINTER(_DevResult)]
+            L.wtet_delaunay_batch.argtypes = [P, P, P, u32, ctypes.POINTER(_Config),
+                                              ctypes.POINTER(_BatchResult)]
+            L.wtet_delaunay_batch_device.argtypes = [P, P, P, u32, ctypes.POINTER(_Config),
+                                                     ctypes.POINTER(_BatchResult)]
+            L.wtet_delaunay_batch_resident.argtypes = [P, P, P, u32, ctypes.POINTER(_Config),
+                                                       ctypes.POINTER(_BatchDevResult)]
+            L.wtet_batch_result_free.argtypes = [ctypes.POINTER(_BatchResult)]
             L.wtet_get_stats.argtypes = [P, ctypes.POINTER(_Stats)]
             L.wtet_check_quality.argtypes = [P, u32, P, P, u32, P, P]
             L.wtet_encode_adjacency.argtypes = [u32, u32]
@@ -261,6 +293,60 @@ class Context:
             raise RuntimeError(f"wtet_delaunay_resident failed ({_STATUS.get(st, st)}): {self.error()}")
         return res.num_tets, res.d_tets, res.d_adjacency
 
+    def _run_batch(self, fn, ptr, offsets, cfg):
+        res = _BatchResult()
+        c = cfg._c() if cfg is not None else Config()._c()
+        off = np.ascontiguousarray(offsets, np.uint64)
+        st = fn(self._h, ptr, _p(off), len(off) - 1, ctypes.byref(c), ctypes.byref(res))
+        if st != 0:
+            msg = self.error()
+            if st in (1, 2):
+                raise ValueError(msg)
+            raise RuntimeError(f"wtet_delaunay_batch failed ({_STATUS.get(st, st)}): {msg}")
+        try:
+            m, C, k = res.num_tets, res.num_clouds, res.num_failed
+            arr = lambda p, n, dt=np.uint32: (np.ctypeslib.as_array(p, shape=(n,)).copy() if n  # noqa: E731
+                                              else np.zeros((0,), dt))
+            tets = arr(res.tets, 4 * m).reshape(-1, 4)
+            adj = arr(res.adjacency, 4 * m).reshape(-1, 4)
+            toff = arr(res.tet_offsets, C + 1, np.uint64)
+            failed = arr(res.failed_verts, k)
+            foff = arr(res.failed_offsets, C + 1, np.uint64)
+        finally:
+            lib().wtet_batch_result_free(ctypes.byref(res))
+        return BatchResult(tets, adj, toff, failed, foff, self.stats())
+
+    def delaunay_batch(self, clouds, offsets=None, config: "Config | None" = None) -> "BatchResult":
+        """Many independent clouds in one pass: ``clouds`` is a list of (n_c, 3)
+        arrays, or one (N, 3) array with ``offsets`` (C+1, offsets[0] = 0).
+        Cloud c's result equals ``delaunay(cloud c)``."""
+        pts, off = _batch_input(clouds, offsets)
+        return self._run_batch(lib().wtet_delaunay_batch, _p(pts), off, config)
+
+    def delaunay_batch_device(self, ptr: int, offsets, config: "Config | None" = None):
+        """Points already in device memory (offsets[-1]*3 f32 at ``ptr``)."""
+        off = _check_offsets(offsets)
+        return self._run_batch(lib().wtet_delaunay_batch_device, ctypes.c_void_p(ptr), off, config)
+
+    def delaunay_batch_resident(self, ptr: int, offsets, config: "Config | None" = None):
+        """Points and result in device memory.  Returns (d_tets, d_adjacency,
+        tet_start, tet_count): device addresses owned by the context and the
+        per-cloud rows (valid until the next call)."""
+        off = _check_offsets(offsets)
+        res = _BatchDevResult()
+        c = config._c() if config is not None else Config()._c()
+        st = lib().wtet_delaunay_batch_resident(self._h, ctypes.c_void_p(ptr), _p(off), len(off) - 1,
+                                                ctypes.byref(c), ctypes.byref(res))
+        if st != 0:
+            msg = self.error()
+            if st in (1, 2):
+                raise ValueError(msg)
+            raise RuntimeError(f"wtet_delaunay_batch_resident failed ({_STATUS.get(st, st)}): {msg}")
+        C = res.num_clouds
+        start = np.ctypeslib.as_array(res.tet_start, shape=(C,)).copy()
+        count = np.ctypeslib.as_array(res.tet_count, shape=(C,)).copy()
+        return res.d_tets, res.d_adjacency, start, count
+
     def predicates(self, kind: int, pts: np.ndarray, ids: np.ndarray | None, mode: int = 0):
         """Device predicate batch; returns (signs, stage counters)."""
         pts = np.ascontiguousarray(pts, np.float32)
@@ -329,6 +415,35 @@ def _validate(points) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def _check_offsets(offsets) -> np.ndarray:
+    off = np.ascontiguousarray(offsets, np.uint64)
+    if off.ndim != 1 or len(off) < 2 or off[0] != 0:
+        raise ValueError("offsets must be a 1-D array of C+1 >= 2 entries starting at 0")
+    d = np.diff(off.astype(np.int64))
+    if (d < 0).any():
+        raise ValueError("offsets must be nondecreasing")
+    if (d < 4).any():
+        c = int(np.argmax(d < 4))
+        raise ValueError(f"Need at least 4 points for 3D Delaunay, got {int(d[c])} in cloud {c}")
+    return off
+
+
+def _batch_input(clouds, offsets):
+    if offsets is None:
+        arrs = [_validate(c) for c in clouds]
+        off = np.zeros(len(arrs) + 1, np.uint64)
+        off[1:] = np.cumsum([len(a) for a in arrs])
+        pts = np.ascontiguousarray(np.concatenate(arrs) if arrs else np.zeros((0, 3), np.float32))
+        return pts, _check_offsets(off)
+    a = np.asarray(clouds)
+    if a.ndim != 2 or a.shape[1] != 3:
+        raise ValueError(f"Points must have shape (N, 3), got {list(a.shape)}")
+    off = _check_offsets(offsets)
+    if int(off[-1]) != a.shape[0]:
+        raise ValueError(f"offsets[-1] = {int(off[-1])} but {a.shape[0]} points were given")
+    return np.ascontiguousarray(a, np.float32), off
+
+
 class Config:
     """Triangulation options (src/python.rs:53-120; GDelConfig src/types.rs:32-61).
 
@@ -393,6 +508,38 @@ class DelaunayResult:
         return f"DelaunayResult(num_tets={self.num_tets}, failed_verts={self.num_failed})"
 
 
+class BatchResult:
+    """Result of a batch: all clouds' rows in cloud order.  ``result[c]`` is
+    cloud c's DelaunayResult (views; cloud-local vertex ids and adjacency)."""
+
+    def __init__(self, tets, adjacency, tet_offsets, failed_verts, failed_offsets, stats=None):
+        self.tets = tets
+        self.adjacency = adjacency
+        self.tet_offsets = tet_offsets
+        self.failed_verts = failed_verts
+        self.failed_offsets = failed_offsets
+        self.stats = stats or {}
+
+    @property
+    def num_clouds(self) -> int:
+        return int(len(self.tet_offsets) - 1)
+
+    def __len__(self) -> int:
+        return self.num_clouds
+
+    def __getitem__(self, c: int) -> DelaunayResult:
+        if c < 0:
+            c += self.num_clouds
+        if not 0 <= c < self.num_clouds:
+            raise IndexError(c)
+        a, b = int(self.tet_offsets[c]), int(self.tet_offsets[c + 1])
+        fa, fb = int(self.failed_offsets[c]), int(self.failed_offsets[c + 1])
+        return DelaunayResult(self.tets[a:b], self.adjacency[a:b], self.failed_verts[fa:fb])
+
+    def __repr__(self) -> str:
+        return f"BatchResult(num_clouds={self.num_clouds}, num_tets={int(self.tets.shape[0])})"
+
+
 _default = None
 _default_lock = threading.Lock()
 
@@ -409,6 +556,12 @@ def delaunay(points, config: Config | None = None) -> DelaunayResult:
     """3D Delaunay triangulation of (N, 3) float32 points (src/python.rs:166-250)."""
     pts = _validate(points)
     return _ctx().delaunay(pts, config)
+
+
+def delaunay_batch(clouds, offsets=None, config: Config | None = None) -> BatchResult:
+    """Triangulate many independent clouds in one pass (a list of (n_c, 3)
+    arrays, or one (N, 3) array with C+1 offsets)."""
+    return _ctx().delaunay_batch(clouds, offsets, config)
 
 
 def check_quality(points, result: DelaunayResult):

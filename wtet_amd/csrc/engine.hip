@@ -40,11 +40,13 @@ struct wtet_ctx {
   std::vector<DevBuf*> all;
 
   // point preparation
-  DevBuf xyz_in, xyzn, keys, ids, keys2, ids2, sort_tmp, mm6, pts;
+  DevBuf xyz_in, xyzn, keys, ids, keys2, ids2, sort_tmp, mm6, pts, doff;
   // mesh (per tet slot)
   DevBuf tet, opp, tinfo, tvote, fvote, tsplit, fit, fround, tmsg, tfwd;
   // lists
-  DevBuf uvert[2], utet[2], wlist, act[2], prop, flips, fout, blk, slot2idx, rec;
+  DevBuf uvert[2], utet[2], wlist, act[2], prop, fout, blk, slot2idx, rec;
+  // output grouping for many clouds
+  DevBuf ord[2], ord_tmp, cstart;
   // counters
   DevBuf ctr, gstats, patch;
   uint32_t* h_ctr = nullptr;  // pinned mirror of the counters
@@ -52,6 +54,9 @@ struct wtet_ctx {
   uint32_t* h_tets = nullptr;
   uint32_t* h_adj = nullptr;
   float* h_xyzn = nullptr;
+  // per-cloud rows of the last resident batch result
+  std::vector<uint64_t> bstart;
+  std::vector<uint32_t> bcount;
   uint32_t* h_patch = nullptr;
   size_t h_cap = 0, h_npts = 0, h_patch_cap = 0;
   uint64_t cap_tets = 0;
@@ -69,10 +74,10 @@ struct wtet_ctx {
 
   wtet_ctx() {
     DevBuf* l[] = {&xyz_in, &xyzn,   &keys,   &ids,   &keys2,  &ids2,     &sort_tmp, &mm6,
-                   &pts,    &tet,    &opp,    &tinfo, &tvote,  &fvote,    &tsplit,   &fit,
-                   &fround, &tmsg,   &tfwd,   &uvert[0], &uvert[1], &utet[0], &utet[1],
-                   &wlist,  &act[0], &act[1], &prop,  &flips,  &fout,     &blk,      &slot2idx,
-                   &rec,    &ctr,    &gstats, &patch};
+                   &pts,    &doff,   &tet,    &opp,   &tinfo,  &tvote,    &fvote,    &tsplit,
+                   &fit,    &fround, &tmsg,   &tfwd,  &uvert[0], &uvert[1], &utet[0], &utet[1],
+                   &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
+                   &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch};
     all.assign(std::begin(l), std::end(l));
   }
 };
@@ -198,7 +203,7 @@ void collect_times(wtet_ctx* c) {
   c->evused = 0;
 }
 
-Mesh mesh_of(wtet_ctx* c, uint32_t n, int mode) {
+Mesh mesh_of(wtet_ctx* c, uint32_t inf, int mode) {
   Mesh M;
   M.pts = P<float4>(c->pts);
   M.tet = P<uint4>(c->tet);
@@ -212,82 +217,13 @@ Mesh mesh_of(wtet_ctx* c, uint32_t n, int mode) {
   M.tmsg = P<uint4>(c->tmsg);
   M.tfwd = P<uint32_t>(c->tfwd);
   M.ctr = P<uint32_t>(c->ctr);
-  M.inf = n + 4;
+  M.inf = inf;
   M.mode = mode;
   return M;
 }
 
 void read_ctr(wtet_ctx* c) {
   HIP_CHECK(hipMemcpyAsync(c->h_ctr, c->ctr.p, C_COUNT * 4, hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
-}
-
-// The five starting tets (src/shaders/init.wgsl:87-148): the super tet
-// (n..n+3, raw orient3d < 0) and one tet joining each of its faces to the
-// infinity vertex n+4, so every face of the mesh has a neighbour.
-void upload_initial_tets(wtet_ctx* c, uint32_t n) {
-  uint32_t s[4] = {n, n + 1, n + 2, n + 3}, inf = n + 4;
-  std::vector<uint32_t> tv(5 * 4), to(5 * 4);
-  auto fv = [](int f, int k) { return face_vi(f, k); };
-  for (int k = 0; k < 4; ++k) tv[k] = s[k];
-  for (int f = 0; f < 4; ++f) {
-    // reversed face + infinity: the infinity tet sees the face from outside
-    uint32_t x = s[fv(f, 0)], y = s[fv(f, 1)], z = s[fv(f, 2)];
-    uint32_t* t = &tv[4 * (1 + f)];
-    t[0] = y; t[1] = x; t[2] = z; t[3] = inf;
-  }
-  // adjacency by face matching over the five tets
-  for (int t = 0; t < 5; ++t)
-    for (int f = 0; f < 4; ++f) {
-      uint32_t a[3];
-      int j = 0;
-      for (int v = 0; v < 4; ++v)
-        if (v != f) a[j++] = tv[4 * t + v];
-      std::sort(a, a + 3);
-      to[4 * t + f] = kInvalid;
-      for (int u = 0; u < 5 && to[4 * t + f] == kInvalid; ++u) {
-        if (u == t) continue;
-        for (int g = 0; g < 4; ++g) {
-          uint32_t b[3];
-          int i = 0;
-          for (int v = 0; v < 4; ++v)
-            if (v != g) b[i++] = tv[4 * u + v];
-          std::sort(b, b + 3);
-          if (a[0] == b[0] && a[1] == b[1] && a[2] == b[2]) {
-            to[4 * t + f] = enc_opp((uint32_t)u, (uint32_t)g);
-            break;
-          }
-        }
-      }
-      if (to[4 * t + f] == kInvalid) throw Fail{WTET_ERR_INTERNAL, "initial tets not closed"};
-    }
-  std::vector<uint32_t> info(5, kAlive), zero32(5, 0), zero64(10, 0);
-  std::vector<uint64_t> none(5, kVoteNone);
-  HIP_CHECK(hipMemcpyAsync(c->tet.p, tv.data(), 80, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->opp.p, to.data(), 80, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->tinfo.p, info.data(), 20, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->tvote.p, none.data(), 40, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->fvote.p, none.data(), 40, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->tsplit.p, zero64.data(), 40, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->fit.p, zero32.data(), 20, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->fround.p, zero32.data(), 20, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
-}
-
-void upload_points_tail(wtet_ctx* c, uint32_t n) {
-  // super vertices src/gpu/mod.rs:152-162 and the infinity vertex; w = id
-  const float sv[5][3] = {{-100.f, -100.f, -100.f}, {400.f, -100.f, -100.f},
-                          {-100.f, 400.f, -100.f}, {-100.f, -100.f, 400.f},
-                          {1000.f, 1000.f, 1000.f}};
-  float4 tail[5];
-  for (int k = 0; k < 5; ++k) {
-    uint32_t id = n + (uint32_t)k;
-    float w;
-    std::memcpy(&w, &id, 4);
-    tail[k] = make_float4(sv[k][0], sv[k][1], sv[k][2], w);
-  }
-  HIP_CHECK(hipMemcpyAsync(P<float4>(c->pts) + n, tail, sizeof(tail), hipMemcpyHostToDevice,
-                           c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -300,14 +236,14 @@ struct Counters {
 // Returns the flips executed; A becomes the next list's length.
 void collect_times(wtet_ctx* c);
 
-uint32_t flip_iteration(wtet_ctx* c, uint32_t n, int mode, uint32_t& ac, uint32_t& A,
+uint32_t flip_iteration(wtet_ctx* c, uint32_t inf, int mode, uint32_t& ac, uint32_t& A,
                         uint32_t& it, uint32_t round, Counters& K, uint64_t* st) {
   if (c->evused > 4096) collect_times(c);
   // worst case every proposal wins and takes one fresh slot (h_ctr holds the
   // counters read at the end of the previous step)
   uint64_t need = (uint64_t)c->h_ctr[C_TETS] + A + 64;
   if (need > c->cap_tets) ensure_tets(c, need + need / 2, true);
-  Mesh M = mesh_of(c, n, mode);
+  Mesh M = mesh_of(c, inf, mode);
   ++it;
   launch_set_ctr(M.ctr, C_ACT_NEXT, 0u, C_FLIPS, 0u, c->stream);
   uint32_t* cur = P<uint32_t>(c->act[ac]);
@@ -328,19 +264,44 @@ uint32_t flip_iteration(wtet_ctx* c, uint32_t n, int mode, uint32_t& ac, uint32_
   return F;
 }
 
-void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& cfg, Counters& K,
-                std::vector<uint32_t>& viol_rec, uint32_t& nviol, bool resident, uint32_t& mout) {
+// Morton bits per axis and the bit where the cloud index starts in a key
+void key_layout(uint32_t C, int* kbits, int* end_bit) {
+  if (C <= 1) {
+    *kbits = 21;
+    *end_bit = 63;
+    return;
+  }
+  int cb = 0;
+  while ((1ull << cb) < C) ++cb;
+  *kbits = std::min(21, (63 - cb) / 3);
+  *end_bit = 3 * *kbits + cb;
+}
+
+// The device phase for clouds off[c]..off[c+1] of d_xyz (one cloud: {0, n}).
+// Leaves the rows in prop/fout (cloud c from cstart[c]), the violation
+// records as rows, and -- unless resident with no violation -- a copy of the
+// rows in the pinned staging buffers.
+void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& off,
+                const wtet_config& cfg, Counters& K, std::vector<uint32_t>& viol_rec,
+                uint32_t& nviol, bool resident, uint32_t& mout, std::vector<uint32_t>& cstart) {
   hipStream_t s = c->stream;
   int mode = cfg.predicate_mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
+  const uint32_t C = (uint32_t)off.size() - 1;
+  const uint32_t n = off[C];
+  const uint32_t V = n + 4 * C;  // infinity vertex
+  int kbits = 21, end_bit = 63;
+  key_layout(C, &kbits, &end_bit);
 
   ensure(c, c->xyzn, (size_t)n * 12);
   ensure(c, c->keys, (size_t)n * 8);
   ensure(c, c->ids, (size_t)n * 4);
   ensure(c, c->keys2, (size_t)n * 8);
   ensure(c, c->ids2, (size_t)n * 4);
-  ensure(c, c->sort_tmp, std::max<size_t>(sort_temp_bytes(n), 16));
-  ensure(c, c->mm6, 64);
-  ensure(c, c->pts, ((size_t)n + 5) * 16);
+  ensure(c, c->sort_tmp, std::max<size_t>(sort_temp_bytes(n, end_bit), 16));
+  ensure(c, c->mm6, (size_t)C * 24);
+  ensure(c, c->doff, ((size_t)C + 1) * 4);
+  ensure(c, c->cstart, ((size_t)C + 1) * 4);
+  ensure(c, c->pts, ((size_t)V + 1) * 16);
   for (int b = 0; b < 2; ++b) {
     ensure(c, c->uvert[b], (size_t)n * 4);
     ensure(c, c->utet[b], (size_t)n * 4);
@@ -349,30 +310,38 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
   ensure(c, c->ctr, C_COUNT * 4);
   ensure(c, c->gstats, 64);
   ensure(c, c->rec, (1u << 20) * 4);
-  ensure_tets(c, 7ull * n + 4096, false);
+  ensure_tets(c, 7ull * n + 4096 + 5ull * C, false);
   uint64_t* st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
 
+  HIP_CHECK(hipMemcpyAsync(c->doff.p, off.data(), ((size_t)C + 1) * 4, hipMemcpyHostToDevice, s));
+  Clouds CL;
+  CL.off = P<uint32_t>(c->doff);
+  CL.C = C;
+  CL.V = V;
+  CL.kbits = kbits;
+  CL.cshift = 3 * kbits;
+
   HIP_CHECK(hipEventRecord(c->ev0, s));
-  // points: normalize (f32, as the reference), Morton keys, sort, gather
-  uint32_t* mm6 = P<uint32_t>(c->mm6);
-  launch_fill_u32(mm6, 0xFFFFFFFFu, 3, s);
-  launch_fill_u32(mm6 + 3, 0u, 3, s);
   launch_fill_u32(P<uint32_t>(c->ctr), 0u, C_COUNT, s);
   launch_fill_u64(P<uint64_t>(c->gstats), 0ull, 8, s);
   timed(c, K_PREP, [&] {
-    launch_minmax(d_xyz, n, mm6, s);
-    launch_normalize(d_xyz, n, mm6, P<float>(c->xyzn), P<uint64_t>(c->keys), P<uint32_t>(c->ids), s);
+    // normalize each cloud (f32, as the reference), (cloud, Morton) keys,
+    // sort, gather into the vertex layout
+    launch_prepare(d_xyz, n, CL, P<uint32_t>(c->mm6), P<float>(c->xyzn), P<uint64_t>(c->keys),
+                   P<uint32_t>(c->ids), s);
     sort_pairs(P<uint64_t>(c->keys), P<uint32_t>(c->ids), P<uint64_t>(c->keys2),
-               P<uint32_t>(c->ids2), n, c->sort_tmp.p, c->sort_tmp.bytes, s);
-    launch_gather_points(P<float>(c->xyzn), P<uint32_t>(c->ids2), n, P<float4>(c->pts), s);
-    // uninserted list: every point (minus redundant duplicates) in the super tet
-    launch_init_uninserted(P<float4>(c->pts), P<uint64_t>(c->keys2), n, P<uint32_t>(c->blk),
-                           P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), P<uint32_t>(c->ctr), s);
+               P<uint32_t>(c->ids2), n, end_bit, c->sort_tmp.p, c->sort_tmp.bytes, s);
+    launch_gather_points(P<float>(c->xyzn), P<uint32_t>(c->ids2), P<uint64_t>(c->keys2), n, CL,
+                         P<float4>(c->pts), s);
+    // uninserted list: every point (minus redundant duplicates) in its
+    // cloud's super tet
+    launch_init_uninserted(P<float4>(c->pts), P<uint64_t>(c->keys2), n, CL, P<uint32_t>(c->blk),
+                           P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), P<uint32_t>(c->ctr),
+                           s);
+    launch_start_tets(mesh_of(c, V, mode), CL, s);
   });
   HIP_CHECK(hipGetLastError());
-  upload_points_tail(c, n);
-  upload_initial_tets(c, n);
-  launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u, -1, 0u, s);
+  launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u * C, -1, 0u, s);
   read_ctr(c);
   uint32_t U = c->h_ctr[C_NPTS];
   c->stats.duplicates = n - U;
@@ -385,7 +354,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
     ++round;
     if (round > std::max<uint32_t>(max_rounds, 4096u))
       throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
-    Mesh M = mesh_of(c, n, mode);
+    Mesh M = mesh_of(c, V, mode);
     launch_set_ctr(M.ctr, C_ULEN, U, -1, 0u, s);
     timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
     timed(c, K_PICK, [&] {
@@ -401,7 +370,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
     uint64_t need = (uint64_t)c->h_ctr[C_TETS] + 64;
     if (need > c->cap_tets) {
       ensure_tets(c, need + need / 2, true);
-      M = mesh_of(c, n, mode);
+      M = mesh_of(c, V, mode);
     }
     K.sumU += U;
     K.rounds++;
@@ -418,9 +387,9 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
     A = c->h_ctr[C_ACT_NEXT];
     if (flipping) {
       for (uint32_t k = 0; k < max_fi && A > 0; ++k)
-        if (flip_iteration(c, n, mode, ac, A, it, round, K, st) == 0) break;
+        if (flip_iteration(c, V, mode, ac, A, it, round, K, st) == 0) break;
       if (U > 0) {
-        M = mesh_of(c, n, mode);
+        M = mesh_of(c, V, mode);
         timed(c, K_RELOCATE, [&] {
           launch_relocate(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st, s);
         });
@@ -435,23 +404,39 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
     uint32_t guard = 0;
     while (A > 0) {
       if (++guard > 100000) throw Fail{WTET_ERR_NOT_CONVERGED, "flipping did not terminate"};
-      if (flip_iteration(c, n, mode, ac, A, it, round, K, st) == 0) break;
+      if (flip_iteration(c, V, mode, ac, A, it, round, K, st) == 0) break;
     }
   }
   // audit every finite face
   read_ctr(c);
   uint32_t T = c->h_ctr[C_TETS];
-  Mesh M = mesh_of(c, n, mode);
+  Mesh M = mesh_of(c, V, mode);
   launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, s);
   const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
   timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, st, s); });
-  // compaction
-  timed(c, K_OUTPUT, [&] {
-    launch_output(M, T, n, P<uint32_t>(c->slot2idx), P<uint32_t>(c->blk), P<uint32_t>(c->prop),
-                  P<uint32_t>(c->fout), s);
-  });
+  // compaction, grouped by cloud
+  OutputBufs OB = {};
+  OB.blk = P<uint32_t>(c->blk);
+  OB.olist = P<uint32_t>(c->act[0]);
+  OB.okey = P<uint32_t>(c->act[1]);
+  OB.slot2pos = P<uint32_t>(c->slot2idx);
+  OB.cstart = P<uint32_t>(c->cstart);
+  if (C > 1) {
+    ensure(c, c->ord[0], (size_t)c->cap_tets * 4);
+    ensure(c, c->ord[1], (size_t)c->cap_tets * 4);
+    ensure(c, c->ord_tmp, std::max<size_t>(sort_temp_bytes_u32(T, cloud_key_bits(C)), 16));
+    OB.olist2 = P<uint32_t>(c->ord[0]);
+    OB.okey2 = P<uint32_t>(c->ord[1]);
+    OB.temp = c->ord_tmp.p;
+    OB.temp_bytes = c->ord_tmp.bytes;
+  }
+  OB.out_tets = P<uint32_t>(c->prop);
+  OB.out_adj = P<uint32_t>(c->fout);
+  timed(c, K_OUTPUT, [&] { launch_output(M, T, CL, OB, s); });
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(c->ev1, s));
+  cstart.resize((size_t)C + 1);
+  HIP_CHECK(hipMemcpyAsync(cstart.data(), c->cstart.p, ((size_t)C + 1) * 4, hipMemcpyDeviceToHost, s));
   read_ctr(c);
   collect_times(c);
   float ms = 0;
@@ -462,6 +447,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
   uint32_t Mout = c->h_ctr[C_ALIVE];
   mout = Mout;
   nviol = c->h_ctr[C_VIOL];
+  if (cstart[C] != Mout) throw Fail{WTET_ERR_INTERNAL, "output row count mismatch"};
   if (!resident || nviol > 0) {
     // room for the repair to append (it rarely adds more than a few tets)
     size_t cap = (size_t)Mout + Mout / 32 + 65536;
@@ -475,7 +461,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
   uint32_t nrec = std::min(c->h_ctr[C_FAILREC], rec_cap);
   viol_rec.resize(nrec);
   if (nrec) {
-    // records are slot-based; map them to output indices on the device
+    // records are slot-based; map them to rows on the device
     launch_map_records(P<uint32_t>(c->rec), nrec, P<uint32_t>(c->slot2idx), s);
     HIP_CHECK(hipMemcpyAsync(viol_rec.data(), c->rec.p, (size_t)nrec * 4, hipMemcpyDeviceToHost, s));
   }
@@ -484,7 +470,6 @@ void run_device(wtet_ctx* c, const float* d_xyz, uint32_t n, const wtet_config& 
   HIP_CHECK(hipStreamSynchronize(s));
   c->stats.tet_slots = T;
 }
-
 
 }  // namespace
 
@@ -535,10 +520,10 @@ void delaunay_impl(void* a_) {
     d_xyz = P<float>(c->xyz_in);
   }
   Counters K;
-  std::vector<uint32_t> rec;
+  std::vector<uint32_t> rec, cstart;
   uint32_t nviol = 0, mout = 0;
   bool resident = a.dout != nullptr;
-  run_device(c, d_xyz, a.n, a.cfg, K, rec, nviol, resident, mout);
+  run_device(c, d_xyz, {0u, a.n}, a.cfg, K, rec, nviol, resident, mout, cstart);
   c->stats.violations_gpu = nviol;
   MeshBuf mb{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
   bool on_host = !resident || nviol > 0;
@@ -622,6 +607,191 @@ void delaunay_impl(void* a_) {
   std::memcpy(out->adjacency, adj.data(), adj.size() * 4);
   if (!failed.empty()) std::memcpy(out->failed_verts, failed.data(), failed.size() * 4);
   c->stats.num_tets_out = out->num_tets;
+  c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+
+struct BatchArgs {
+  wtet_ctx* c;
+  const float* xyz;
+  bool on_device;
+  std::vector<uint32_t> off;
+  wtet_config cfg;
+  wtet_batch_result* out;
+  wtet_batch_device_result* dout;
+};
+
+// One cloud's rows after the device phase: a view of the staged rows, or
+// the repaired copy.
+struct CloudRows {
+  std::vector<uint32_t> tets, adj;  // filled when the cloud was repaired
+  bool repaired = false;
+};
+
+void batch_impl(void* a_) {
+  BatchArgs& a = *static_cast<BatchArgs*>(a_);
+  wtet_ctx* c = a.c;
+  auto t0 = std::chrono::steady_clock::now();
+  HIP_CHECK(hipSetDevice(c->device));
+  std::memset(&c->stats, 0, sizeof(c->stats));
+  const uint32_t C = (uint32_t)a.off.size() - 1;
+  const uint32_t n = a.off[C];
+  const float* d_xyz = a.xyz;
+  if (!a.on_device) {
+    ensure(c, c->xyz_in, (size_t)n * 12);
+    HIP_CHECK(hipMemcpyAsync(c->xyz_in.p, a.xyz, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    d_xyz = P<float>(c->xyz_in);
+  }
+  Counters K;
+  std::vector<uint32_t> rec, cstart;
+  uint32_t nviol = 0, mout = 0;
+  bool resident = a.dout != nullptr;
+  run_device(c, d_xyz, a.off, a.cfg, K, rec, nviol, resident, mout, cstart);
+  c->stats.violations_gpu = nviol;
+  auto t1 = std::chrono::steady_clock::now();
+
+  // violation records (rows) per cloud, as cloud-local (row << 2) | face
+  std::vector<std::pair<uint32_t, uint32_t>> byc;  // (cloud, local record)
+  byc.reserve(rec.size());
+  for (uint32_t r : rec) {
+    uint32_t row = r >> 2;
+    uint32_t cl = (uint32_t)(std::upper_bound(cstart.begin(), cstart.end(), row) - cstart.begin()) - 1;
+    byc.emplace_back(cl, ((row - cstart[cl]) << 2) | (r & 3u));
+  }
+  std::sort(byc.begin(), byc.end());
+  // failed_verts per cloud: the vertex opposite each non-Delaunay real-only face
+  std::vector<std::vector<uint32_t>> failed(C);
+  for (auto& pr : byc) {
+    uint32_t cl = pr.first, nc = a.off[cl + 1] - a.off[cl];
+    const uint32_t* T = c->h_tets + 4 * (size_t)cstart[cl];
+    const uint32_t* A = c->h_adj + 4 * (size_t)cstart[cl];
+    uint32_t t = pr.second >> 2, f = pr.second & 3u;
+    uint32_t e = A[4 * (size_t)t + f];
+    if (e == kInvalid) continue;
+    uint32_t u = e >> 5, g = e & 3u;
+    bool real = true;
+    for (int k = 0; k < 4; ++k) real = real && T[4 * (size_t)t + k] < nc && T[4 * (size_t)u + k] < nc;
+    if (real) failed[cl].push_back(T[4 * (size_t)u + g]);
+  }
+  for (auto& v : failed) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  // host repair of each cloud with violations, on a copy of its rows
+  std::vector<CloudRows> rows(C);
+  if (nviol > 0 && a.cfg.enable_splaying) {
+    ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
+    HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    size_t i = 0;
+    while (i < byc.size()) {
+      uint32_t cl = byc[i].first;
+      std::vector<uint32_t> viol;
+      for (; i < byc.size() && byc[i].first == cl; ++i) viol.push_back(byc[i].second);
+      size_t cnt = cstart[cl + 1] - cstart[cl];
+      size_t cap = cnt + cnt / 8 + 4096;
+      CloudRows& R = rows[cl];
+      R.tets.assign(c->h_tets + 4 * (size_t)cstart[cl], c->h_tets + 4 * (size_t)(cstart[cl] + cnt));
+      R.adj.assign(c->h_adj + 4 * (size_t)cstart[cl], c->h_adj + 4 * (size_t)(cstart[cl] + cnt));
+      R.tets.resize(4 * cap);
+      R.adj.resize(4 * cap);
+      MeshBuf mb{R.tets.data(), R.adj.data(), cnt, cap, {}};
+      RepairReport rr;
+      wtet_status rs = repair_delaunay(c->h_xyzn + 3 * (size_t)a.off[cl], a.off[cl + 1] - a.off[cl],
+                                       mb, viol, a.cfg.predicate_mode, &rr);
+      if (rs != WTET_OK)
+        throw Fail{rs, "host repair of cloud " + std::to_string(cl) + " failed: " + rr.message};
+      R.tets.resize(4 * mb.count);
+      R.adj.resize(4 * mb.count);
+      R.repaired = true;
+      c->stats.repair_regions += rr.regions;
+    }
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  c->stats.ms_repair = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  c->stats.sum_uninserted = K.sumU;
+  c->stats.sum_active = K.sumA;
+  c->stats.sum_flips = K.sumF;
+  c->stats.rounds = K.rounds;
+  c->stats.flip_iterations = K.iters;
+  c->stats.num_points = n;
+  uint64_t nfailed = 0;
+  for (auto& v : failed) nfailed += v.size();
+
+  if (resident) {
+    // repaired clouds move to rows appended after the device phase's rows
+    c->bstart.assign(C, 0);
+    c->bcount.assign(C, 0);
+    uint64_t extra = 0;
+    for (uint32_t cl = 0; cl < C; ++cl)
+      if (rows[cl].repaired) extra += rows[cl].tets.size() / 4;
+    uint64_t total = (uint64_t)mout + extra;
+    if (total >= (1ull << 32)) throw Fail{WTET_ERR_OUT_OF_MEMORY, "too many output rows"};
+    if (extra) {
+      ensure(c, c->prop, total * 16, true);
+      ensure(c, c->fout, total * 16, true);
+    }
+    uint64_t tail = mout;
+    for (uint32_t cl = 0; cl < C; ++cl) {
+      if (rows[cl].repaired) {
+        size_t cnt = rows[cl].tets.size() / 4;
+        HIP_CHECK(hipMemcpyAsync(P<uint32_t>(c->prop) + 4 * tail, rows[cl].tets.data(), cnt * 16,
+                                 hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(P<uint32_t>(c->fout) + 4 * tail, rows[cl].adj.data(), cnt * 16,
+                                 hipMemcpyHostToDevice, c->stream));
+        c->bstart[cl] = tail;
+        c->bcount[cl] = (uint32_t)cnt;
+        tail += cnt;
+      } else {
+        c->bstart[cl] = cstart[cl];
+        c->bcount[cl] = cstart[cl + 1] - cstart[cl];
+      }
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    a.dout->num_clouds = C;
+    a.dout->rows = total;
+    a.dout->d_tets = P<uint32_t>(c->prop);
+    a.dout->d_adjacency = P<uint32_t>(c->fout);
+    a.dout->tet_start = c->bstart.data();
+    a.dout->tet_count = c->bcount.data();
+    a.dout->num_failed = nfailed;
+    uint64_t live = 0;
+    for (uint32_t cl = 0; cl < C; ++cl) live += c->bcount[cl];
+    c->stats.num_tets_out = live;
+    c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return;
+  }
+  // host result: clouds in order, contiguous
+  wtet_batch_result* out = a.out;
+  out->num_clouds = C;
+  out->tet_offsets = (uint64_t*)std::malloc(((size_t)C + 1) * 8);
+  out->failed_offsets = (uint64_t*)std::malloc(((size_t)C + 1) * 8);
+  if (!out->tet_offsets || !out->failed_offsets) throw std::bad_alloc();
+  uint64_t total = 0;
+  for (uint32_t cl = 0; cl < C; ++cl) {
+    out->tet_offsets[cl] = total;
+    total += rows[cl].repaired ? rows[cl].tets.size() / 4 : cstart[cl + 1] - cstart[cl];
+  }
+  out->tet_offsets[C] = total;
+  out->num_tets = total;
+  out->tets = (uint32_t*)std::malloc(std::max<size_t>(total * 16, 16));
+  out->adjacency = (uint32_t*)std::malloc(std::max<size_t>(total * 16, 16));
+  out->num_failed = nfailed;
+  out->failed_verts = (uint32_t*)std::malloc(std::max<size_t>(nfailed * 4, 4));
+  if (!out->tets || !out->adjacency || !out->failed_verts) throw std::bad_alloc();
+  uint64_t fo = 0;
+  for (uint32_t cl = 0; cl < C; ++cl) {
+    uint64_t r0 = out->tet_offsets[cl], cnt = out->tet_offsets[cl + 1] - r0;
+    const uint32_t* st = rows[cl].repaired ? rows[cl].tets.data() : c->h_tets + 4 * (size_t)cstart[cl];
+    const uint32_t* sa = rows[cl].repaired ? rows[cl].adj.data() : c->h_adj + 4 * (size_t)cstart[cl];
+    std::memcpy(out->tets + 4 * r0, st, cnt * 16);
+    std::memcpy(out->adjacency + 4 * r0, sa, cnt * 16);
+    out->failed_offsets[cl] = fo;
+    if (!failed[cl].empty()) std::memcpy(out->failed_verts + fo, failed[cl].data(), failed[cl].size() * 4);
+    fo += failed[cl].size();
+  }
+  out->failed_offsets[C] = fo;
+  c->stats.num_tets_out = total;
   c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -740,6 +910,89 @@ wtet_status wtet_delaunay_resident(wtet_ctx* c, const float* d_xyz, uint32_t n,
                                    const wtet_config* cfg, wtet_device_result* out) {
   if (!out) return WTET_ERR_INVALID_ARGUMENT;
   return delaunay_common(c, d_xyz, true, n, cfg, nullptr, out);
+}
+
+static wtet_status batch_common(wtet_ctx* c, const float* xyz, bool dev, const uint64_t* offsets,
+                                uint32_t num_clouds, const wtet_config* cfg,
+                                wtet_batch_result* out, wtet_batch_device_result* dout) {
+  if (!c || (!out && !dout)) return WTET_ERR_INVALID_ARGUMENT;
+  if (out) std::memset(out, 0, sizeof(*out));
+  if (dout) std::memset(dout, 0, sizeof(*dout));
+  if (!xyz || !offsets) {
+    c->err = "points or offsets pointer is null";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (num_clouds == 0 || num_clouds > (1u << 20)) {
+    c->err = "num_clouds must be in 1..2^20, got " + std::to_string(num_clouds);
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (offsets[0] != 0) {
+    c->err = "offsets[0] must be 0";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (offsets[num_clouds] > (1ull << 26)) {
+    c->err = "too many points for one call (at most 2^26 over all clouds)";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  BatchArgs a;
+  a.off.resize((size_t)num_clouds + 1);
+  for (uint32_t i = 0; i <= num_clouds; ++i) a.off[i] = (uint32_t)offsets[i];
+  for (uint32_t i = 0; i < num_clouds; ++i) {
+    if (offsets[i + 1] < offsets[i]) {
+      c->err = "offsets must be nondecreasing";
+      return WTET_ERR_INVALID_ARGUMENT;
+    }
+    if (offsets[i + 1] - offsets[i] < 4) {
+      c->err = "Need at least 4 points for 3D Delaunay, got " + std::to_string(offsets[i + 1] - offsets[i]) +
+               " in cloud " + std::to_string(i);
+      return WTET_ERR_TOO_FEW_POINTS;
+    }
+  }
+  a.c = c;
+  a.xyz = xyz;
+  a.on_device = dev;
+  if (cfg) a.cfg = *cfg;
+  else wtet_config_default(&a.cfg);
+  if (a.cfg.insertion_rule != 0 && a.cfg.insertion_rule != 1) {
+    c->err = "Invalid insertion_rule. Must be 'circumcenter' or 'centroid'.";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  a.out = out;
+  a.dout = dout;
+  wtet_status s = run_guarded(c, batch_impl, &a);
+  if (s != WTET_OK) {
+    if (out) wtet_batch_result_free(out);
+    if (dout) std::memset(dout, 0, sizeof(*dout));
+  }
+  return s;
+}
+
+wtet_status wtet_delaunay_batch(wtet_ctx* c, const float* xyz, const uint64_t* offsets,
+                                uint32_t num_clouds, const wtet_config* cfg, wtet_batch_result* out) {
+  return batch_common(c, xyz, false, offsets, num_clouds, cfg, out, nullptr);
+}
+
+wtet_status wtet_delaunay_batch_device(wtet_ctx* c, const float* d_xyz, const uint64_t* offsets,
+                                       uint32_t num_clouds, const wtet_config* cfg,
+                                       wtet_batch_result* out) {
+  return batch_common(c, d_xyz, true, offsets, num_clouds, cfg, out, nullptr);
+}
+
+wtet_status wtet_delaunay_batch_resident(wtet_ctx* c, const float* d_xyz, const uint64_t* offsets,
+                                         uint32_t num_clouds, const wtet_config* cfg,
+                                         wtet_batch_device_result* out) {
+  if (!out) return WTET_ERR_INVALID_ARGUMENT;
+  return batch_common(c, d_xyz, true, offsets, num_clouds, cfg, nullptr, out);
+}
+
+void wtet_batch_result_free(wtet_batch_result* r) {
+  if (!r) return;
+  std::free(r->tets);
+  std::free(r->adjacency);
+  std::free(r->tet_offsets);
+  std::free(r->failed_verts);
+  std::free(r->failed_offsets);
+  std::memset(r, 0, sizeof(*r));
 }
 
 void wtet_result_free(wtet_result* r) {

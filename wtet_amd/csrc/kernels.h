@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "ops.h"
+#include "prep.h"
 
 namespace wtet {
 
@@ -12,19 +12,30 @@ constexpr int kBlock = 256;  // 4 wavefronts of 64
 
 inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
-// point preparation: normalization (src/lib.rs:313-337), Morton keys, gather
-void launch_minmax(const float* xyz, uint32_t n, uint32_t* mm6, hipStream_t s);
-void launch_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
-                      uint64_t* keys, uint32_t* ids, hipStream_t s);
-void launch_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts,
-                          hipStream_t s);
+// point preparation for one or many clouds (layout: prep.h):
+// per-cloud normalization (src/lib.rs:313-337) and (cloud, Morton) keys
+void launch_prepare(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, float* xyzn,
+                    uint64_t* keys, uint32_t* ids, hipStream_t s);
+// sorted points -> vertices, plus every cloud's super vertices and infinity
+void launch_gather_points(const float* xyzn, const uint32_t* ids, const uint64_t* keys,
+                          uint32_t n, const Clouds& K, float4* pts, hipStream_t s);
+// five start tets per cloud in slots 5c..5c+4
+void launch_start_tets(const Mesh& M, const Clouds& K, hipStream_t s);
 void sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp,
-                uint32_t n, void* temp, size_t temp_bytes, hipStream_t s);
-size_t sort_temp_bytes(uint32_t n);
+                uint32_t n, int end_bit, void* temp, size_t temp_bytes, hipStream_t s);
+size_t sort_temp_bytes(uint32_t n, int end_bit);
+size_t sort_temp_bytes_u32(uint32_t n, int end_bit);
+// bits of an output sort key: clouds 0..C-1 and C for free slots
+inline int cloud_key_bits(uint32_t C) {
+  int bits = 1;
+  while ((1ull << bits) <= C) ++bits;
+  return bits;
+}
 // uninserted list = every point except redundant exact duplicates;
 // ctr[C_NPTS] = its length
-void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n, uint32_t* blk,
-                            uint32_t* uvert, uint32_t* utet, uint32_t* ctr, hipStream_t s);
+void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n, const Clouds& K,
+                            uint32_t* blk, uint32_t* uvert, uint32_t* utet, uint32_t* ctr,
+                            hipStream_t s);
 
 // rounds
 void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
@@ -48,9 +59,23 @@ void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint3
 void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st,
                   hipStream_t s);
 
-// output: compact alive tets, original vertex ids, (tet << 5) | face adjacency
-void launch_output(const Mesh& M, uint32_t T, uint32_t n, uint32_t* slot2idx, uint32_t* blk,
-                   uint32_t* out_tets, uint32_t* out_adj, hipStream_t s);
+// output: live tets grouped by cloud (rows of cloud c from cstart[c],
+// cstart[C] = ctr[C_ALIVE] = rows), cloud-local vertex ids and
+// (row << 5) | face adjacency; slot2pos maps slots to rows
+struct OutputBufs {
+  uint32_t* blk;
+  uint32_t* olist;     // T
+  uint32_t* okey;      // T (many clouds)
+  uint32_t* olist2;    // T (many clouds)
+  uint32_t* okey2;     // T (many clouds)
+  uint32_t* slot2pos;  // T
+  uint32_t* cstart;    // C + 1
+  void* temp;          // sort temporary (many clouds)
+  size_t temp_bytes;
+  uint32_t* out_tets;  // rows * 4 (null: only slot2pos / cstart)
+  uint32_t* out_adj;
+};
+void launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s);
 
 // violation records (slot << 2 | face) -> output indices
 void launch_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx, hipStream_t s);

@@ -9,8 +9,11 @@
 #include <rocprim/rocprim.hpp>
 
 #include "kernels.h"
+#include "prep.h"
 
 namespace wtet {
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n);
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -93,21 +96,19 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t* blk, uint32_t nb
 }
 
 // ---------------------------------------------------------------------------
-// point preparation
+// point preparation (layout of one or many clouds: prep.h)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t ord_f32(float f) {
-  uint32_t u = f32_bits(f);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float unord_f32(uint32_t o) {
-  uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
-  float f;
-  __builtin_memcpy(&f, &u, 4);
-  return f;
+__global__ void k_minmax_init(uint32_t* mm, uint32_t C) {
+  uint32_t i = gid();
+  if (i < 6u * C) mm[i] = (i % 6u) < 3u ? 0xFFFFFFFFu : 0u;
 }
 
-__global__ void k_minmax(const float* xyz, uint32_t n, uint32_t* mm6) {
+// Per-cloud min/max of each axis.  Lanes in the cloud of the wave's first
+// lane combine with a wave reduction (one atomic per wave); lanes of other
+// clouds (at most a few per cloud boundary) add their own atomics.
+__global__ void k_minmax(const float* xyz, uint32_t n, Clouds K, uint32_t* mm) {
   uint32_t i = gid();
+  uint32_t c = (i < n && K.C > 1) ? cloud_of_point(K.off, K.C, i) : 0u;
   uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
   if (i < n)
     for (int k = 0; k < 3; ++k) {
@@ -118,144 +119,130 @@ __global__ void k_minmax(const float* xyz, uint32_t n, uint32_t* mm6) {
         hi[k] = o;
       }
     }
+  uint32_t c0 = __shfl(c, 0);
+  bool same = c == c0;
   for (int k = 0; k < 3; ++k) {
+    uint32_t l = same ? lo[k] : 0xFFFFFFFFu, h = same ? hi[k] : 0u;
     for (int off = 32; off > 0; off >>= 1) {
-      lo[k] = min(lo[k], (uint32_t)__shfl_xor(lo[k], off));
-      hi[k] = max(hi[k], (uint32_t)__shfl_xor(hi[k], off));
+      l = min(l, (uint32_t)__shfl_xor(l, off));
+      h = max(h, (uint32_t)__shfl_xor(h, off));
     }
     if (__lane_id() == 0) {
-      atomicMin(&mm6[k], lo[k]);
-      atomicMax(&mm6[3 + k], hi[k]);
+      atomicMin(&mm[6 * (size_t)c0 + k], l);
+      atomicMax(&mm[6 * (size_t)c0 + 3 + k], h);
+    }
+    if (!same && i < n) {
+      atomicMin(&mm[6 * (size_t)c + k], lo[k]);
+      atomicMax(&mm[6 * (size_t)c + 3 + k], hi[k]);
     }
   }
 }
 
-// spread the low 21 bits of v to every third bit of a 64-bit word
-__device__ __forceinline__ uint64_t expand21(uint64_t v) {
-  v &= 0x1FFFFFull;
-  v = (v | (v << 32)) & 0x001F00000000FFFFull;
-  v = (v | (v << 16)) & 0x001F0000FF0000FFull;
-  v = (v | (v << 8)) & 0x100F00F00F00F00Full;
-  v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
-  v = (v | (v << 2)) & 0x1249249249249249ull;
-  return v;
-}
-
-#pragma clang fp contract(off)
-__global__ void k_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
-                            uint64_t* keys, uint32_t* ids) {
+__global__ void k_normalize(const float* xyz, uint32_t n, Clouds K, const uint32_t* mm,
+                            float* xyzn, uint64_t* keys, uint32_t* ids) {
   uint32_t i = gid();
-  if (i >= n) return;
-  float mn[3], ext[3];
-  for (int k = 0; k < 3; ++k) {
-    mn[k] = unord_f32(mm6[k]);
-    ext[k] = unord_f32(mm6[3 + k]) - mn[k];
-  }
-  float s = ext[0];
-  if (ext[1] > s) s = ext[1];
-  if (ext[2] > s) s = ext[2];
-  if (!(s >= 1e-10f)) s = 1e-10f;
-  uint32_t q[3];
-  for (int k = 0; k < 3; ++k) {
-    float v = (xyz[3 * (size_t)i + k] - mn[k]) / s;  // IEEE f32 sub and div
-    xyzn[3 * (size_t)i + k] = v;
-    float c = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
-    q[k] = (uint32_t)(c * 2097151.0f);
-  }
-  // 63-bit Morton key: spatial order for memory locality (the insertion
-  // order cannot change the triangulation)
-  keys[i] = (expand21(q[2]) << 2) | (expand21(q[1]) << 1) | expand21(q[0]);
-  ids[i] = i;
+  if (i < n) normalize_body(xyz, i, K, mm, xyzn, keys, ids);
 }
-#pragma clang fp contract(on)
 
-__global__ void k_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts) {
+__global__ void k_gather_points(const float* xyzn, const uint32_t* ids, const uint64_t* keys,
+                                uint32_t n, Clouds K, float4* pts) {
   uint32_t j = gid();
-  if (j >= n) return;
-  uint32_t o = ids[j];
-  float w;
-  __builtin_memcpy(&w, &o, 4);
-  pts[j] = make_float4(xyzn[3 * (size_t)o], xyzn[3 * (size_t)o + 1], xyzn[3 * (size_t)o + 2], w);
+  if (j < n) gather_body(xyzn, ids, keys, j, K, pts);
 }
 
-void launch_minmax(const float* xyz, uint32_t n, uint32_t* mm6, hipStream_t s) {
-  k_minmax<<<nblocks(n), kBlock, 0, s>>>(xyz, n, mm6);
-}
-void launch_normalize(const float* xyz, uint32_t n, const uint32_t* mm6, float* xyzn,
-                      uint64_t* keys, uint32_t* ids, hipStream_t s) {
-  k_normalize<<<nblocks(n), kBlock, 0, s>>>(xyz, n, mm6, xyzn, keys, ids);
-}
-void launch_gather_points(const float* xyzn, const uint32_t* ids, uint32_t n, float4* pts,
-                          hipStream_t s) {
-  k_gather_points<<<nblocks(n), kBlock, 0, s>>>(xyzn, ids, n, pts);
+__global__ void k_tail_points(Clouds K, float4* pts) {
+  uint32_t g = gid();
+  if (g <= 4u * K.C) tail_body(g, K, pts);
 }
 
-size_t sort_temp_bytes(uint32_t n) {
+__global__ void k_start_tets(Mesh M, StartTets S, Clouds K) {
+  uint32_t g = gid();
+  if (g < 5u * K.C) start_tet_body(M, S, g, K);
+}
+
+void launch_prepare(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, float* xyzn,
+                    uint64_t* keys, uint32_t* ids, hipStream_t s) {
+  k_minmax_init<<<nblocks(6u * K.C), kBlock, 0, s>>>(mm, K.C);
+  k_minmax<<<nblocks(n), kBlock, 0, s>>>(xyz, n, K, mm);
+  k_normalize<<<nblocks(n), kBlock, 0, s>>>(xyz, n, K, mm, xyzn, keys, ids);
+}
+
+void launch_gather_points(const float* xyzn, const uint32_t* ids, const uint64_t* keys,
+                          uint32_t n, const Clouds& K, float4* pts, hipStream_t s) {
+  k_gather_points<<<nblocks(n), kBlock, 0, s>>>(xyzn, ids, keys, n, K, pts);
+  k_tail_points<<<nblocks(4u * K.C + 1u), kBlock, 0, s>>>(K, pts);
+}
+
+void launch_start_tets(const Mesh& M, const Clouds& K, hipStream_t s) {
+  static const StartTets S = make_start_tets();
+  k_start_tets<<<nblocks(5u * K.C), kBlock, 0, s>>>(M, S, K);
+}
+
+size_t sort_temp_bytes(uint32_t n, int end_bit) {
   size_t bytes = 0;
   (void)rocprim::radix_sort_pairs((void*)nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, 63);
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
   return bytes;
 }
 
 void sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp, uint32_t n,
-                void* temp, size_t temp_bytes, hipStream_t s) {
-  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, ids, ids_tmp, n, 0, 63, s);
+                int end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
+  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, ids, ids_tmp, n, 0, end_bit, s);
 }
 
-// Exact duplicates (equal normalized coordinates) share a key, so they sit
-// within a run of equal keys after the sort.  All but the highest input id
-// are redundant under the lifted SoS and are not inserted; the uninserted
-// list is built from the rest (stable, one scan).
-__device__ __forceinline__ bool redundant_dup(const float4* pts, const uint64_t* keys, uint32_t n,
-                                              uint32_t j, uint32_t* capped) {
-  float4 p = pts[j];
-  uint32_t id = pt_rank(p);
-  uint64_t kj = keys[j];
-  const int kCap = 64;
-  for (int d = -1; d <= 1; d += 2) {
-    int steps = 0;
-    for (int64_t k = (int64_t)j + d; k >= 0 && k < (int64_t)n && keys[k] == kj; k += d) {
-      if (++steps > kCap) {
-        *capped = 1u;
-        break;
-      }
-      float4 q = pts[k];
-      if (q.x == p.x && q.y == p.y && q.z == p.z && pt_rank(q) > id) return true;
-    }
-  }
-  return false;
+size_t sort_temp_bytes_u32(uint32_t n, int end_bit) {
+  size_t bytes = 0;
+  (void)rocprim::radix_sort_pairs((void*)nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
+  return bytes;
 }
 
-__global__ void k_uninit_count(const float4* pts, const uint64_t* keys, uint32_t n,
+void sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
+                    uint32_t n, int end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
+  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, vals, vals_tmp, n, 0, end_bit, s);
+}
+
+// sorted position j of cloud c: kept unless it is a redundant duplicate;
+// the uninserted list holds (vertex j + 4c, start super tet 5c)
+__device__ __forceinline__ bool keep_point(const float4* pts, const uint64_t* keys, uint32_t n,
+                                           const Clouds& K, uint32_t j, uint32_t* c,
+                                           uint32_t* capped) {
+  if (j >= n) return false;
+  *c = cloud_of_key(K, keys[j]);
+  return !redundant_dup(pts + 4u * *c, keys, n, j, capped);
+}
+
+__global__ void k_uninit_count(const float4* pts, const uint64_t* keys, uint32_t n, Clouds K,
                                uint32_t* blk, uint32_t* ctr) {
   uint32_t j = gid();
-  uint32_t capped = 0;
-  bool keep = j < n && !redundant_dup(pts, keys, n, j, &capped);
+  uint32_t capped = 0, c = 0;
+  bool keep = keep_point(pts, keys, n, K, j, &c, &capped);
   if (capped) atomicAdd(&ctr[C_DUPSCAN], 1u);
   uint32_t tot;
   block_rank(keep, &tot);
   if (threadIdx.x == 0) blk[blockIdx.x] = tot;
 }
 
-__global__ void k_uninit_scatter(const float4* pts, const uint64_t* keys, uint32_t n,
+__global__ void k_uninit_scatter(const float4* pts, const uint64_t* keys, uint32_t n, Clouds K,
                                  const uint32_t* blk, uint32_t* uvert, uint32_t* utet) {
   uint32_t j = gid();
-  uint32_t capped = 0;
-  bool keep = j < n && !redundant_dup(pts, keys, n, j, &capped);
+  uint32_t capped = 0, c = 0;
+  bool keep = keep_point(pts, keys, n, K, j, &c, &capped);
   uint32_t tot;
   uint32_t r = blk[blockIdx.x] + block_rank(keep, &tot);
   if (keep) {
-    uvert[r] = j;
-    utet[r] = 0u;  // every point starts in the super tet (slot 0)
+    uvert[r] = j + 4u * c;
+    utet[r] = 5u * c;  // every point starts in its cloud's super tet
   }
 }
 
-void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n, uint32_t* blk,
-                            uint32_t* uvert, uint32_t* utet, uint32_t* ctr, hipStream_t s) {
+void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n, const Clouds& K,
+                            uint32_t* blk, uint32_t* uvert, uint32_t* utet, uint32_t* ctr,
+                            hipStream_t s) {
   uint32_t nb = nblocks(n);
-  k_uninit_count<<<nb, kBlock, 0, s>>>(pts, keys, n, blk, ctr);
+  k_uninit_count<<<nb, kBlock, 0, s>>>(pts, keys, n, K, blk, ctr);
   k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, ctr, 2);
-  k_uninit_scatter<<<nb, kBlock, 0, s>>>(pts, keys, n, blk, uvert, utet);
+  k_uninit_scatter<<<nb, kBlock, 0, s>>>(pts, keys, n, K, blk, uvert, utet);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,40 +445,72 @@ __global__ void k_alive_count(Mesh M, uint32_t T, uint32_t* blk) {
   if (threadIdx.x == 0) blk[blockIdx.x] = tot;
 }
 
-__global__ void k_alive_scatter(Mesh M, uint32_t T, const uint32_t* blk, uint32_t* slot2idx) {
+// one cloud: live slots in slot order are the rows
+__global__ void k_alive_scatter(Mesh M, uint32_t T, const uint32_t* blk, uint32_t* olist,
+                                uint32_t* slot2pos) {
   uint32_t t = gid();
   bool a = t < T && (M.tinfo[t] & kAlive);
   uint32_t tot;
   uint32_t r = blk[blockIdx.x] + block_rank(a, &tot);
-  if (t < T) slot2idx[t] = a ? r : kInvalid;
+  if (t < T) {
+    slot2pos[t] = a ? r : kInvalid;
+    if (a) olist[r] = t;
+  }
 }
 
-__global__ void k_output_write(Mesh M, uint32_t T, uint32_t n, const uint32_t* slot2idx,
-                               uint32_t* out_tets, uint32_t* out_adj) {
+// many clouds: sort key per slot = cloud of a live tet, or C for a free slot
+__global__ void k_slot_keys(Mesh M, uint32_t T, Clouds K, uint32_t* okey, uint32_t* oslot) {
   uint32_t t = gid();
   if (t >= T) return;
-  uint32_t r = slot2idx[t];
-  if (r == kInvalid) return;
-  uint4 tv = M.tet[t], to = M.opp[t];
-  uint4 ov, oa;
-  for (int k = 0; k < 4; ++k) {
-    uint32_t v = get4(tv, k);
-    set4(ov, k, v < n ? pt_rank(M.pts[v]) : v);
-    uint32_t e = get4(to, k);
-    uint32_t nr = slot2idx[opp_tet(e)];
-    set4(oa, k, nr == kInvalid ? kInvalid : (nr << 5) | opp_face(e));
-  }
-  reinterpret_cast<uint4*>(out_tets)[r] = ov;
-  reinterpret_cast<uint4*>(out_adj)[r] = oa;
+  okey[t] = (M.tinfo[t] & kAlive) ? cloud_of_tet(M, t, K) : K.C;
+  oslot[t] = t;
 }
 
-void launch_output(const Mesh& M, uint32_t T, uint32_t n, uint32_t* slot2idx, uint32_t* blk,
-                   uint32_t* out_tets, uint32_t* out_adj, hipStream_t s) {
+// after the sort: row j holds slot olist[j]; cstart[c] = first row of
+// cloud c (every cloud has live tets), cstart[C] = number of live tets
+__global__ void k_row_positions(const uint32_t* okey, const uint32_t* olist, uint32_t T, uint32_t C,
+                                uint32_t* slot2pos, uint32_t* cstart) {
+  uint32_t j = gid();
+  if (j >= T) return;
+  uint32_t c = okey[j];
+  slot2pos[olist[j]] = c < C ? j : kInvalid;
+  if (c < C && (j == 0 || okey[j - 1] != c)) cstart[c] = j;
+}
+
+__global__ void k_cstart_tail(uint32_t* cstart, uint32_t C, const uint32_t* ctr) {
+  if (threadIdx.x == 0) cstart[C] = ctr[C_ALIVE];
+}
+
+__global__ void k_output_rows(Mesh M, uint32_t T, const uint32_t* olist, const uint32_t* okey,
+                              const uint32_t* cstart, const uint32_t* slot2pos, Clouds K,
+                              uint32_t* out_tets, uint32_t* out_adj) {
+  uint32_t j = gid();
+  if (j >= T || j >= M.ctr[C_ALIVE]) return;
+  output_body(M, j, olist, okey, cstart, slot2pos, K, out_tets, out_adj);
+}
+
+void launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s) {
   uint32_t nb = nblocks(T);
-  k_alive_count<<<nb, kBlock, 0, s>>>(M, T, blk);
-  k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, M.ctr, 1);
-  k_alive_scatter<<<nb, kBlock, 0, s>>>(M, T, blk, slot2idx);
-  if (out_tets) k_output_write<<<nb, kBlock, 0, s>>>(M, T, n, slot2idx, out_tets, out_adj);
+  k_alive_count<<<nb, kBlock, 0, s>>>(M, T, B.blk);
+  k_scan_blocks<<<1, 1024, 0, s>>>(B.blk, nb, M.ctr, 1);
+  const uint32_t* olist;
+  const uint32_t* okey = nullptr;
+  if (K.C <= 1) {
+    k_alive_scatter<<<nb, kBlock, 0, s>>>(M, T, B.blk, B.olist, B.slot2pos);
+    k_fill_u32<<<1, kBlock, 0, s>>>(B.cstart, 0u, 1);
+    k_cstart_tail<<<1, 64, 0, s>>>(B.cstart, 1u, M.ctr);
+    olist = B.olist;
+  } else {
+    k_slot_keys<<<nb, kBlock, 0, s>>>(M, T, K, B.okey, B.olist);
+    sort_pairs_u32(B.okey, B.olist, B.okey2, B.olist2, T, cloud_key_bits(K.C), B.temp, B.temp_bytes, s);
+    k_row_positions<<<nb, kBlock, 0, s>>>(B.okey2, B.olist2, T, K.C, B.slot2pos, B.cstart);
+    k_cstart_tail<<<1, 64, 0, s>>>(B.cstart, K.C, M.ctr);
+    olist = B.olist2;
+    okey = B.okey2;
+  }
+  if (B.out_tets)
+    k_output_rows<<<nb, kBlock, 0, s>>>(M, T, olist, okey, B.cstart, B.slot2pos, K, B.out_tets,
+                                        B.out_adj);
 }
 
 __global__ void k_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx) {

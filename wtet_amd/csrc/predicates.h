@@ -195,9 +195,12 @@ WT_HD BI<D> to_fixed(float v, int E) {
   F32Parts p = f32_parts(v);
   BI<D> r = bi_zero<D>();
   if (p.m == 0u) return r;
-  int s = p.e - E;  // >= 0
+  // drop the mantissa's trailing zeros first: E is the lowest set bit over
+  // all coordinates, so the shift below is >= 0 only from that bit
+  int tz = ctz32(p.m);
+  int s = p.e + tz - E;  // >= 0
   int limb = s >> 5, bit = s & 31;
-  uint64_t x = (uint64_t)p.m << bit;
+  uint64_t x = (uint64_t)(p.m >> tz) << bit;
   if (limb < D) r.w[limb] = (uint32_t)x;
   if (limb + 1 < D) r.w[limb + 1] = (uint32_t)(x >> 32);
   return p.neg ? bi_neg(r) : r;

@@ -5,6 +5,9 @@ lattice      : k^3 grid i/(k-1), ix-major, iz fastest (src/tests.rs:1103-1117)
 sphere       : i.i.d. Gaussian triples (f64) normalized to the unit sphere, cast to f32
 shell        : points in a thin shell of radius 0.4 around (0.5, 0.5, 0.5)
 clusters     : mixture of isotropic Gaussians, samples outside [0,1]^3 redrawn
+mixed_batch  : many clouds, log-normal sizes (median 2000, sigma 0.63, clipped
+               to [64, 50000]), each a ``clusters`` cloud with K ~ U{1..8}
+               Gaussians, sigma_k ~ U[0.02, 0.2]; returns (points, offsets)
 """
 from __future__ import annotations
 
@@ -48,6 +51,18 @@ def clusters(n: int, seed: int, k: int | None = None, sigma=(0.02, 0.2)) -> np.n
         s = s[np.all((s >= 0) & (s <= 1), axis=1)]
         out = np.vstack([out, s])
     return out[:n].astype(np.float32)
+
+
+def mixed_batch(num_clouds: int, seed: int, median: float = 2000.0, sigma: float = 0.63,
+                lo: int = 64, hi: int = 50000):
+    """(points (N, 3) f32, offsets (C+1,) u64): cloud c is points[off[c]:off[c+1]],
+    drawn by ``clusters(size_c, seed * 1_000_003 + c)``."""
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.rint(np.exp(rng.normal(np.log(median), sigma, num_clouds))), lo, hi).astype(np.int64)
+    clouds = [clusters(int(sizes[c]), seed * 1_000_003 + c) for c in range(num_clouds)]
+    off = np.zeros(num_clouds + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    return np.ascontiguousarray(np.concatenate(clouds), np.float32), off
 
 
 def sha256(points: np.ndarray) -> str:
