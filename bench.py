@@ -50,7 +50,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the exact-only comparison")
     ap.add_argument("--ring", type=int, default=1, help="insertion independence radius (0, 1, 2)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-cloud batched leg")
+    ap.add_argument("--no-stress", action="store_true", help="skip the lattice / sphere leg")
+    ap.add_argument("--no-presize", action="store_true", help="skip the learned-presizing leg")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the streaming copy kernel over MIB MiB (5 launches), "
+                         "the known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
@@ -59,6 +64,21 @@ def main():
 
     import wtet_amd
     from wtet_amd import workloads as W
+
+    if args.copy_calibration:
+        ctx = wtet_amd.Context(0)
+        nbytes = args.copy_calibration << 20
+        src = ctx.upload(np.zeros(nbytes, np.uint8))
+        dst = ctx.upload(np.zeros(nbytes, np.uint8))
+        t0 = time.perf_counter()
+        for _ in range(5):
+            ctx.copy(dst, src, nbytes)
+        dt = (time.perf_counter() - t0) / 5
+        ctx.free(src)
+        ctx.free(dst)
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(dt * 1e3, 3),
+                          "GBs_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+        return
 
     pts = W.uniform(args.n, args.seed)
     ctx = wtet_amd.Context(0)
@@ -154,6 +174,56 @@ def main():
                  "violations_gpu": bst["violations_gpu"], "rounds": bst["rounds"],
                  "flip_iterations": bst["flip_iterations"]}
 
+    stress = None
+    if not args.no_stress:
+        # degenerate inputs: 128^3 lattice (2,097,152 points, exact fallback
+        # and SoS on every cell) and 1M points on the unit sphere
+        stress = {}
+        for name, pts_s in (("lattice_128", W.lattice(128)), ("sphere_1000000_seed2", W.sphere(1_000_000, 2))):
+            d_s = ctx.upload(pts_s)
+            ctx.delaunay_resident(d_s, len(pts_s), wtet_amd.Config(collect_stats=True))
+            ss = ctx.stats()
+            t0 = time.perf_counter()
+            ctx.delaunay_resident(d_s, len(pts_s), filt)
+            ms_s = (time.perf_counter() - t0) * 1e3
+            ctx.free(d_s)
+            o3s, ins = max(1, sum(ss["pred"][0:4])), max(1, sum(ss["pred"][4:8]))
+            stress[name] = {"points": len(pts_s), "ms": round(ms_s, 3),
+                            "value": round(len(pts_s) / (ms_s / 1e3), 1),
+                            "orient3d_exact_or_sos": (ss["pred"][2] + ss["pred"][3]) / o3s,
+                            "insphere_exact_or_sos": (ss["pred"][6] + ss["pred"][7]) / ins,
+                            "sos_insphere": ss["pred"][7] / ins,
+                            "violations_gpu": ss["violations_gpu"], "duplicates": ss["duplicates"]}
+            print(f"stress {name}: {ms_s:.1f} ms", file=sys.stderr, flush=True)
+
+    presized = None
+    if not args.no_presize:
+        # 5,000,000 points: 90% from 32 anisotropic Gaussians + 10% uniform;
+        # each mode in a fresh context so allocation and growth are counted
+        pp = W.anisotropic(5_000_000, 5)
+        presized = {"workload": "anisotropic_5000000_seed5", "points": len(pp)}
+        from wtet_amd import presize as PZ
+        for mode in ("learned", "fixed", "grow"):
+            if mode == "learned" and not os.path.exists(PZ.MODEL_PATH):
+                continue
+            c2 = wtet_amd.Context(0)
+            d_p = c2.upload(pp)
+            cfg_p = wtet_amd.Config(presize=mode)
+            t0 = time.perf_counter()
+            c2.delaunay_resident(d_p, len(pp), cfg_p)
+            ms_first = (time.perf_counter() - t0) * 1e3
+            sp = c2.stats()
+            t0 = time.perf_counter()
+            c2.delaunay_resident(d_p, len(pp), cfg_p)
+            ms_again = (time.perf_counter() - t0) * 1e3
+            c2.free(d_p)
+            c2.close()
+            presized[mode] = {"ms_first_call": round(ms_first, 3), "ms_second_call": round(ms_again, 3),
+                              "value_first_call": round(len(pp) / (ms_first / 1e3), 1),
+                              "slots_requested": sp["presize_slots"], "slots_used": sp["tet_slots"],
+                              "grow_events": sp["grow_events"], "peak_active": sp["peak_active"]}
+            print(f"presize {mode}: {ms_first:.1f} ms first call", file=sys.stderr, flush=True)
+
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -191,6 +261,8 @@ def main():
         "plain_exact": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "batched": batch,
+        "stress": stress,
+        "presize": presized,
         "parity_with_oracle": parity,
         "stage_fractions": stages,
         "ms_device": round(st_t["ms_device"], 3),

@@ -28,7 +28,7 @@ typedef enum wtet_status {
   WTET_ERR_INTERNAL = 6          /* invariant violated inside the library     */
 } wtet_status;
 
-/* GDelConfig, src/types.rs:32-61 (defaults :49-61); the two trailing
+/* GDelConfig, src/types.rs:32-61 (defaults :49-61); the three trailing
  * fields are additions of this library. */
 typedef struct wtet_config {
   int32_t insertion_rule;         /* 0 circumcenter, 1 centroid (types.rs:17-28) */
@@ -40,6 +40,8 @@ typedef struct wtet_config {
   uint32_t max_flip_iterations;   /* 10: flip iterations per insertion round   */
   int32_t predicate_mode;         /* 0: f32 filter -> f64 filter -> exact; 1: exact only */
   int32_t collect_stats;          /* 1: count predicate stages (small cost)    */
+  int32_t presize_mode;           /* tet-slot presizing: 0 fixed 7N, 1 learned (model set with
+                                     wtet_set_presize_model), 2 none (grow-and-retry from 2N) */
 } wtet_config;
 
 /* DelaunayResult, src/types.rs:69-79, owned by the caller after the call;
@@ -74,6 +76,11 @@ typedef struct wtet_stats {
   double ms_device;        /* normalized points in HBM -> tets in HBM          */
   double ms_repair;        /* host repair                                      */
   double ms_total;         /* whole call                                       */
+  uint64_t peak_active;    /* longest active (flip-check) list                 */
+  uint64_t presize_slots;  /* tet slots requested before the loop              */
+  uint64_t tet_capacity;   /* tet slots allocated when the loop started        */
+  float presize_pred[3];   /* learned presizing: predicted log(slots / n),
+                              log(peak active / n), log(1 + stuck faces)        */
 } wtet_stats;
 
 typedef struct wtet_ctx wtet_ctx;
@@ -183,6 +190,11 @@ wtet_status wtet_device_info(wtet_ctx *ctx, char *buf, uint32_t len);
 wtet_status wtet_device_upload(wtet_ctx *ctx, const void *host, size_t bytes, void **d_out);
 wtet_status wtet_device_download(wtet_ctx *ctx, const void *d_src, void *host_dst, size_t bytes);
 wtet_status wtet_device_free(wtet_ctx *ctx, void *d_ptr);
+/* Device-to-device copy of `bytes` (a multiple of 16, 16-byte aligned
+ * pointers) by a plain streaming kernel on the context's stream;
+ * synchronous.  Also the known-size kernel that HBM byte counters are
+ * calibrated against. */
+wtet_status wtet_device_copy(wtet_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
 /* Points inserted in one round must lie in tets at least `ring`+1
  * face-steps apart (ring 0: one point per tet, as the reference's vote,
@@ -191,6 +203,44 @@ wtet_status wtet_device_free(wtet_ctx *ctx, void *d_ptr);
  * The triangulation does not depend on it.  No counterpart in the
  * reference. */
 wtet_status wtet_set_insertion_ring(wtet_ctx *ctx, int ring);
+
+/* Learned presizing (no counterpart in the reference; the reference sizes
+ * its buffers with a fixed formula, src/gpu/buffers.rs:112-579).
+ * Features of a cloud: the 512-bin occupancy histogram of the top 9 bits of
+ * its 30-bit Morton code (src/morton.rs:22-35), mean and covariance of its
+ * box-normalized coordinates, log(n).  The MLP (522 -> 128 -> 128 -> 3,
+ * ReLU, FP32) predicts log(tet slots / n), log(peak active list / n) and
+ * log(1 + stuck faces); its parameters are one float array of
+ * WTET_PRESIZE_PARAMS values: feature mean[522], feature std[522], target
+ * mean[3], target std[3], W1[522*128], b1[128], W2[128*128], b2[128],
+ * W3[128*3], b3[3], margin[3] (weights input-major). */
+#define WTET_PRESIZE_FEATURES 522
+#define WTET_PRESIZE_OUTPUTS 3
+#define WTET_PRESIZE_PARAMS 84896
+
+/* features of each cloud (d_xyz in device memory, offsets as for
+ * wtet_delaunay_batch) into feat (host, num_clouds * 522) */
+wtet_status wtet_presize_features(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
+                                  uint32_t num_clouds, float *feat);
+/* GPU inference from host arrays: pred (num_clouds * 3) in log space,
+ * margin included */
+wtet_status wtet_presize_predict(wtet_ctx *ctx, const float *params, const float *feat,
+                                 uint32_t num_clouds, float *pred);
+/* model used by presize_mode 1 (null clears it) */
+wtet_status wtet_set_presize_model(wtet_ctx *ctx, const float *params);
+/* MSE loss and its gradient (trainable range W1..b3 of the parameter
+ * array; the rest is zero) over rows of raw features and raw targets,
+ * standardized with the parameters' mean/std, computed by the training
+ * kernels (rows <= 4096) */
+wtet_status wtet_mlp_loss_grad(wtet_ctx *ctx, const float *params, const float *feat,
+                               const float *targets, uint32_t rows, float *loss, float *grad);
+/* Adam training on the GPU: params (in: initial weights and the
+ * standardization; out: trained), minibatches of `batch` rows drawn by a
+ * seeded shuffle each epoch; loss_curve (may be null) gets each epoch's
+ * mean minibatch loss */
+wtet_status wtet_mlp_train(wtet_ctx *ctx, float *params, const float *feat, const float *targets,
+                           uint32_t rows, uint32_t epochs, uint32_t batch, float lr,
+                           uint64_t seed, float *loss_curve);
 
 /* Per-kernel-group timing with HIP events on the context's stream (the
  * reference's GpuProfiler/CpuProfiler scopes, src/profiler.rs:5-224).

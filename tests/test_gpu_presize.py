@@ -1,0 +1,102 @@
+"""GPU tests of the learned presizing: the device features, inference and
+training kernels against the CPU oracle, and what the trained network gains
+over the method without it."""
+import numpy as np
+import pytest
+
+import wtet_amd
+from wtet_amd import presize
+from wtet_amd import workloads as W
+from oracle import mlp_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = wtet_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _clouds():
+    return [W.uniform(3000, 1), W.clusters(5000, 2), W.lattice(9), W.sphere(2000, 3),
+            W.anisotropic(4000, 4, k=5), (W.uniform(700, 5) * np.float32([1, 1, 1e-3])).astype(np.float32)]
+
+
+def test_device_features_match_oracle(ctx):
+    clouds = _clouds()
+    flat = np.concatenate(clouds)
+    off = np.zeros(len(clouds) + 1, np.uint64)
+    off[1:] = np.cumsum([len(c) for c in clouds])
+    d = ctx.upload(flat)
+    try:
+        got = ctx.presize_features(d, off)
+    finally:
+        ctx.free(d)
+    for c, p in enumerate(clouds):
+        ref = R.features(p)
+        assert np.array_equal(got[c, :512], ref[:512].astype(np.float32)), c  # exact counts / n
+        assert np.allclose(got[c, 512:], ref[512:], rtol=1e-5, atol=1e-7), c
+
+
+def _data(seed, rows):
+    rng = np.random.default_rng(seed)
+    X = np.stack([R.features(W.uniform(int(rng.integers(50, 3000)), seed * 100 + i)) for i in range(rows)])
+    X[:, :512] += rng.random((rows, 512)) * 1e-3  # vary the histogram
+    Y = np.stack([X[:, 521] * 0.1 + X[:, 512], np.sin(X[:, 513] * 6), X[:, 515] * 10], 1)
+    return X.astype(np.float32), Y.astype(np.float32)
+
+
+def test_device_gradient_matches_oracle(ctx):
+    X, Y = _data(1, 48)
+    p = presize.init_params(X, Y, 3)
+    loss, g = ctx.mlp_loss_grad(p, X, Y)
+    lref, gref = R.loss_grad(p.astype(np.float64), X.astype(np.float64), Y.astype(np.float64))
+    assert loss == pytest.approx(lref, rel=1e-4)
+    scale = np.abs(gref).max()
+    assert np.abs(g - gref).max() <= 1e-4 * scale, np.abs(g - gref).max() / scale
+
+
+def test_device_prediction_matches_oracle(ctx):
+    X, Y = _data(2, 20)
+    p = presize.set_margin(presize.init_params(X, Y, 5), [0.05, 0.1, 0.2])
+    got = ctx.presize_predict(p, X)
+    ref = R.predict(p.astype(np.float64), X.astype(np.float64))
+    assert np.allclose(got, ref, rtol=1e-4, atol=1e-4)
+
+
+def test_training_reduces_loss(ctx):
+    X, Y = _data(3, 256)
+    p0 = presize.init_params(X, Y, 0)
+    l0, _ = ctx.mlp_loss_grad(p0, X[:256], Y[:256])
+    p, curve = ctx.mlp_train(p0, X, Y, epochs=60, batch=64, lr=2e-3, seed=1)
+    l1, _ = ctx.mlp_loss_grad(p, X[:256], Y[:256])
+    assert curve[-1] < 0.5 * curve[0] and l1 < 0.5 * l0, (curve[0], curve[-1], l0, l1)
+
+
+def test_shipped_model_presizes_without_growing(oracle, ctx):
+    """The gain of the trained network: with its presizing the run needs no
+    buffer growth and requests far fewer tet slots than the fixed 7 per
+    point, while plain grow-and-retry from a small start grows several
+    times; all three give the same triangulation."""
+    import os
+    if not os.path.exists(presize.MODEL_PATH):
+        pytest.skip("no trained model shipped yet")
+    p = W.anisotropic(300_000, 11)
+    res = {}
+    for mode in ("learned", "fixed", "grow"):
+        c = wtet_amd.Context(0)  # fresh buffers: growth is visible
+        try:
+            r = c.delaunay(p, wtet_amd.Config(presize=mode))
+            res[mode] = (r, c.stats())
+        finally:
+            c.close()
+    ref = oracle.canonical_real(res["fixed"][0].tets, len(p))
+    for mode in ("learned", "grow"):
+        assert np.array_equal(oracle.canonical_real(res[mode][0].tets, len(p)), ref), mode
+    sl, sf, sg = res["learned"][1], res["fixed"][1], res["grow"][1]
+    assert sl["grow_events"] == 0, sl
+    assert sg["grow_events"] >= 2, sg
+    assert sl["presize_slots"] < 0.9 * sf["presize_slots"], (sl["presize_slots"], sf["presize_slots"])
+    assert sl["presize_slots"] >= sl["tet_slots"]

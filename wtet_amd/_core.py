@@ -18,6 +18,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libwtet_amd.so")
 
 INVALID = 0xFFFFFFFF
+PRESIZE_FEATURES = 522
+PRESIZE_OUTPUTS = 3
+PRESIZE_PARAMS = 84896
 __version__ = "0.1.0"
 
 _STATUS = {
@@ -42,6 +45,7 @@ class _Config(ctypes.Structure):
         ("max_flip_iterations", ctypes.c_uint32),
         ("predicate_mode", ctypes.c_int32),
         ("collect_stats", ctypes.c_int32),
+        ("presize_mode", ctypes.c_int32),
     ]
 
 
@@ -108,6 +112,10 @@ class _Stats(ctypes.Structure):
         ("ms_device", ctypes.c_double),
         ("ms_repair", ctypes.c_double),
         ("ms_total", ctypes.c_double),
+        ("peak_active", ctypes.c_uint64),
+        ("presize_slots", ctypes.c_uint64),
+        ("tet_capacity", ctypes.c_uint64),
+        ("presize_pred", ctypes.c_float * 3),
     ]
 
 
@@ -146,6 +154,12 @@ def lib():
                                                        ctypes.POINTER(_BatchDevResult)]
             L.wtet_batch_result_free.argtypes = [ctypes.POINTER(_BatchResult)]
             L.wtet_get_stats.argtypes = [P, ctypes.POINTER(_Stats)]
+            L.wtet_presize_features.argtypes = [P, P, P, u32, P]
+            L.wtet_presize_predict.argtypes = [P, P, P, u32, P]
+            L.wtet_set_presize_model.argtypes = [P, P]
+            L.wtet_mlp_loss_grad.argtypes = [P, P, P, P, u32, P, P]
+            L.wtet_mlp_train.argtypes = [P, P, P, P, u32, u32, u32, ctypes.c_float,
+                                         ctypes.c_uint64, P]
             L.wtet_check_quality.argtypes = [P, u32, P, P, u32, P, P]
             L.wtet_encode_adjacency.argtypes = [u32, u32]
             L.wtet_encode_adjacency.restype = u32
@@ -157,6 +171,7 @@ def lib():
             L.wtet_device_upload.argtypes = [P, P, ctypes.c_size_t, ctypes.POINTER(P)]
             L.wtet_device_download.argtypes = [P, P, P, ctypes.c_size_t]
             L.wtet_device_free.argtypes = [P, P]
+            L.wtet_device_copy.argtypes = [P, P, P, ctypes.c_size_t]
             L.wtet_kernel_times.argtypes = [P, u32, ctypes.POINTER(ctypes.c_char_p),
                                             ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)]
             L.wtet_predicates_host.argtypes = [ctypes.c_int, P, P, u32, ctypes.c_int, P]
@@ -201,9 +216,59 @@ class Context:
     def stats(self) -> dict:
         s = _Stats()
         lib().wtet_get_stats(self._h, ctypes.byref(s))
-        d = {k: getattr(s, k) for k, _ in _Stats._fields_ if k != "pred"}
+        d = {k: getattr(s, k) for k, _ in _Stats._fields_ if k not in ("pred", "presize_pred")}
         d["pred"] = list(s.pred)
+        d["presize_pred"] = list(s.presize_pred)
         return d
+
+    # -- learned presizing ---------------------------------------------------
+    def presize_features(self, ptr: int, offsets) -> np.ndarray:
+        """(C, 522) features of the clouds of device points at ``ptr``."""
+        off = _check_offsets(offsets)
+        C = len(off) - 1
+        out = np.empty((C, PRESIZE_FEATURES), np.float32)
+        st = lib().wtet_presize_features(self._h, ctypes.c_void_p(ptr), _p(off), C, _p(out))
+        if st != 0:
+            raise RuntimeError(f"wtet_presize_features failed: {self.error()}")
+        return out
+
+    def presize_predict(self, params: np.ndarray, feat: np.ndarray) -> np.ndarray:
+        p = _params(params)
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, PRESIZE_FEATURES)
+        out = np.empty((len(f), PRESIZE_OUTPUTS), np.float32)
+        if lib().wtet_presize_predict(self._h, _p(p), _p(f), len(f), _p(out)) != 0:
+            raise RuntimeError(f"wtet_presize_predict failed: {self.error()}")
+        return out
+
+    def set_presize_model(self, params: "np.ndarray | None"):
+        if params is None:
+            lib().wtet_set_presize_model(self._h, None)
+            return
+        p = _params(params)
+        if lib().wtet_set_presize_model(self._h, _p(p)) != 0:
+            raise ValueError(f"presize model rejected: {self.error()}")
+        self._presize_params = p
+
+    def mlp_loss_grad(self, params, feat, targets):
+        p = _params(params)
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, PRESIZE_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, PRESIZE_OUTPUTS)
+        loss = np.zeros(1, np.float32)
+        grad = np.zeros(PRESIZE_PARAMS, np.float32)
+        if lib().wtet_mlp_loss_grad(self._h, _p(p), _p(f), _p(t), len(f), _p(loss), _p(grad)) != 0:
+            raise RuntimeError(f"wtet_mlp_loss_grad failed: {self.error()}")
+        return float(loss[0]), grad
+
+    def mlp_train(self, params, feat, targets, epochs: int, batch: int, lr: float, seed: int = 0):
+        p = _params(params).copy()
+        f = np.ascontiguousarray(feat, np.float32).reshape(-1, PRESIZE_FEATURES)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, PRESIZE_OUTPUTS)
+        curve = np.zeros(max(epochs, 1), np.float32)
+        st = lib().wtet_mlp_train(self._h, _p(p), _p(f), _p(t), len(f), epochs, batch,
+                                  ctypes.c_float(lr), ctypes.c_uint64(seed), _p(curve))
+        if st != 0:
+            raise RuntimeError(f"wtet_mlp_train failed: {self.error()}")
+        return p, curve[:epochs]
 
     def upload(self, arr: np.ndarray) -> int:
         """Copy a host array to a new device allocation; returns its address."""
@@ -218,6 +283,11 @@ class Context:
         if lib().wtet_device_download(self._h, ctypes.c_void_p(ptr), _p(out), out.nbytes) != 0:
             raise RuntimeError(f"device download failed: {self.error()}")
         return out
+
+    def copy(self, dst: int, src: int, nbytes: int):
+        """Device-to-device copy by the library's streaming kernel (synchronous)."""
+        if lib().wtet_device_copy(self._h, ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes) != 0:
+            raise RuntimeError(f"device copy failed: {self.error()}")
 
     def free(self, ptr: int):
         lib().wtet_device_free(self._h, ctypes.c_void_p(ptr))
@@ -246,7 +316,14 @@ class Context:
         lib().wtet_device_info(self._h, buf, 512)
         return buf.value.decode()
 
+    def _prepare(self, cfg):
+        # presize="learned" uses the shipped model unless one was set
+        if cfg is not None and cfg.presize == "learned" and getattr(self, "_presize_params", None) is None:
+            from . import presize
+            self.set_presize_model(presize.load_model())
+
     def _run(self, fn, ptr, n, cfg):
+        self._prepare(cfg)
         res = _Result()
         c = cfg._c() if cfg is not None else Config()._c()
         st = fn(self._h, ptr, n, ctypes.byref(c), ctypes.byref(res))
@@ -285,6 +362,7 @@ class Context:
         next call)."""
         if n < 4:
             raise ValueError(f"Need at least 4 points for 3D Delaunay, got {n}")
+        self._prepare(config)
         res = _DevResult()
         c = config._c() if config is not None else Config()._c()
         st = lib().wtet_delaunay_resident(self._h, ctypes.c_void_p(ptr), n, ctypes.byref(c),
@@ -294,6 +372,7 @@ class Context:
         return res.num_tets, res.d_tets, res.d_adjacency
 
     def _run_batch(self, fn, ptr, offsets, cfg):
+        self._prepare(cfg)
         res = _BatchResult()
         c = cfg._c() if cfg is not None else Config()._c()
         off = np.ascontiguousarray(offsets, np.uint64)
@@ -333,6 +412,7 @@ class Context:
         tet_start, tet_count): device addresses owned by the context and the
         per-cloud rows (valid until the next call)."""
         off = _check_offsets(offsets)
+        self._prepare(config)
         res = _BatchDevResult()
         c = config._c() if config is not None else Config()._c()
         st = lib().wtet_delaunay_batch_resident(self._h, ctypes.c_void_p(ptr), _p(off), len(off) - 1,
@@ -415,6 +495,13 @@ def _validate(points) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def _params(params) -> np.ndarray:
+    p = np.ascontiguousarray(params, np.float32).ravel()
+    if p.shape != (PRESIZE_PARAMS,):
+        raise ValueError(f"presize parameters must hold {PRESIZE_PARAMS} floats, got {p.size}")
+    return p
+
+
 def _check_offsets(offsets) -> np.ndarray:
     off = np.ascontiguousarray(offsets, np.uint64)
     if off.ndim != 1 or len(off) < 2 or off[0] != 0:
@@ -444,17 +531,22 @@ def _batch_input(clouds, offsets):
     return np.ascontiguousarray(a, np.float32), off
 
 
+_PRESIZE = {"fixed": 0, "learned": 1, "grow": 2}
+
+
 class Config:
     """Triangulation options (src/python.rs:53-120; GDelConfig src/types.rs:32-61).
 
-    ``predicate_mode`` ("filtered" | "exact") and ``collect_stats`` are
+    ``predicate_mode`` ("filtered" | "exact"), ``collect_stats`` and
+    ``presize`` ("fixed": 7 tet slots per point; "learned": the presizing
+    network's prediction; "grow": grow-and-retry from 2 per point) are
     additions of this library."""
 
     def __init__(self, insertion_rule: str = "circumcenter", enable_flipping: bool = True,
                  enable_sorting: bool = False, enable_hilbert_sorting: bool = False,
                  enable_splaying: bool = True, max_insert_iterations: int = 100,
                  max_flip_iterations: int = 10, predicate_mode: str = "filtered",
-                 collect_stats: bool = False):
+                 collect_stats: bool = False, presize: str = "fixed"):
         rule = str(insertion_rule).lower()
         if rule not in ("circumcenter", "centroid"):
             raise ValueError(
@@ -468,15 +560,18 @@ class Config:
         self.enable_splaying = bool(enable_splaying)
         self.max_insert_iterations = int(max_insert_iterations)
         self.max_flip_iterations = int(max_flip_iterations)
+        if presize not in _PRESIZE:
+            raise ValueError(f"Invalid presize '{presize}'. Must be 'fixed', 'learned' or 'grow'.")
         self.predicate_mode = predicate_mode
         self.collect_stats = bool(collect_stats)
+        self.presize = presize
 
     def _c(self) -> _Config:
         return _Config(0 if self.insertion_rule == "circumcenter" else 1, int(self.enable_flipping),
                        int(self.enable_sorting), int(self.enable_hilbert_sorting),
                        int(self.enable_splaying), self.max_insert_iterations,
                        self.max_flip_iterations, 1 if self.predicate_mode == "exact" else 0,
-                       int(self.collect_stats))
+                       int(self.collect_stats), _PRESIZE[self.presize])
 
     def __repr__(self) -> str:
         b = lambda v: "true" if v else "false"  # noqa: E731 (Rust bool formatting)

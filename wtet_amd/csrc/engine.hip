@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +20,7 @@
 
 #include "../../include/wtet.h"
 #include "kernels.h"
+#include "mlp.h"
 #include "repair.h"
 
 using namespace wtet;
@@ -47,6 +49,10 @@ struct wtet_ctx {
   DevBuf uvert[2], utet[2], wlist, act[2], prop, fout, blk, slot2idx, rec;
   // output grouping for many clouds
   DevBuf ord[2], ord_tmp, cstart;
+  // learned presizing
+  DevBuf presize_params, feat_acc, feat, pred;
+  bool presize_loaded = false;
+  std::vector<float> h_pred;
   // counters
   DevBuf ctr, gstats, patch;
   uint32_t* h_ctr = nullptr;  // pinned mirror of the counters
@@ -77,7 +83,8 @@ struct wtet_ctx {
                    &pts,    &doff,   &tet,    &opp,   &tinfo,  &tvote,    &fvote,    &tsplit,
                    &fit,    &fround, &tmsg,   &tfwd,  &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
-                   &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch};
+                   &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
+                   &presize_params, &feat_acc, &feat, &pred};
     all.assign(std::begin(l), std::end(l));
   }
 };
@@ -228,7 +235,7 @@ void read_ctr(wtet_ctx* c) {
 }
 
 struct Counters {
-  uint64_t sumU = 0, sumA = 0, sumF = 0;
+  uint64_t sumU = 0, sumA = 0, sumF = 0, peakA = 0;
   uint32_t rounds = 0, iters = 0;
 };
 
@@ -258,6 +265,7 @@ uint32_t flip_iteration(wtet_ctx* c, uint32_t inf, int mode, uint32_t& ac, uint3
   uint32_t F = c->h_ctr[C_FLIPS];
   K.sumA += A;
   K.sumF += F;
+  K.peakA = std::max<uint64_t>(K.peakA, A);
   K.iters++;
   A = c->h_ctr[C_ACT_NEXT];
   ac ^= 1;
@@ -310,7 +318,15 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   ensure(c, c->ctr, C_COUNT * 4);
   ensure(c, c->gstats, 64);
   ensure(c, c->rec, (1u << 20) * 4);
-  ensure_tets(c, 7ull * n + 4096 + 5ull * C, false);
+  ensure(c, c->blk, ((size_t)nblocks(n) + 2) * 4);
+  const bool learned = cfg.presize_mode == 1;
+  if (learned) {
+    if (!c->presize_loaded) throw Fail{WTET_ERR_INVALID_ARGUMENT, "presize_mode 1 needs a model (wtet_set_presize_model)"};
+    ensure(c, c->feat_acc, (size_t)C * (kHistBins / 2 + 9) * 8 + 64);
+    ensure(c, c->feat, (size_t)C * kFeat * 4);
+    ensure(c, c->pred, (size_t)C * kOut * 4);
+    c->h_pred.resize((size_t)C * kOut);
+  }
   uint64_t* st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
 
   HIP_CHECK(hipMemcpyAsync(c->doff.p, off.data(), ((size_t)C + 1) * 4, hipMemcpyHostToDevice, s));
@@ -338,13 +354,42 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
     launch_init_uninserted(P<float4>(c->pts), P<uint64_t>(c->keys2), n, CL, P<uint32_t>(c->blk),
                            P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), P<uint32_t>(c->ctr),
                            s);
-    launch_start_tets(mesh_of(c, V, mode), CL, s);
+    if (learned) {
+      // features from the raw points and this call's per-cloud boxes, then
+      // the network's predictions (log space, margin included)
+      launch_features(d_xyz, n, CL.off, C, P<uint32_t>(c->mm6), P<double>(c->feat_acc),
+                      P<float>(c->feat), s);
+      launch_mlp_predict(P<float>(c->presize_params), P<float>(c->feat), C, P<float>(c->pred), s);
+      HIP_CHECK(hipMemcpyAsync(c->h_pred.data(), c->pred.p, (size_t)C * kOut * 4,
+                               hipMemcpyDeviceToHost, s));
+    }
   });
   HIP_CHECK(hipGetLastError());
-  launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u * C, -1, 0u, s);
   read_ctr(c);
   uint32_t U = c->h_ctr[C_NPTS];
   c->stats.duplicates = n - U;
+  // tet slots for the whole call: fixed 7 per point, the learned
+  // prediction, or a small start that the grow-and-retry path extends
+  uint64_t slots;
+  if (learned) {
+    double sum = 0.0;
+    for (uint32_t cl = 0; cl < C; ++cl) {
+      double lp = std::min<double>(c->h_pred[(size_t)cl * kOut], 4.0);  // <= e^4 slots per point
+      sum += (double)(off[cl + 1] - off[cl]) * std::exp(lp);
+    }
+    for (int k = 0; k < 3; ++k) c->stats.presize_pred[k] = C == 1 ? c->h_pred[k] : 0.0f;
+    slots = (uint64_t)std::ceil(sum) + 5ull * C + 1024;
+  } else if (cfg.presize_mode == 2) {
+    slots = 2ull * n + 4096 + 5ull * C;
+  } else {
+    slots = 7ull * n + 4096 + 5ull * C;
+  }
+  c->stats.presize_slots = slots;
+  ensure_tets(c, slots, false);
+  c->stats.tet_capacity = c->cap_tets;
+  launch_start_tets(mesh_of(c, V, mode), CL, s);
+  launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u * C, -1, 0u, s);
+  HIP_CHECK(hipGetLastError());
 
   uint32_t cur = 0, ac = 0, A = 0, it = 0, round = 0;
   uint32_t max_rounds = std::max<uint32_t>(cfg.max_insert_iterations, 1u);
@@ -562,6 +607,7 @@ void delaunay_impl(void* a_) {
   c->stats.sum_flips = K.sumF;
   c->stats.rounds = K.rounds;
   c->stats.flip_iterations = K.iters;
+  c->stats.peak_active = K.peakA;
   c->stats.num_points = a.n;
   if (resident) {
     if (repaired) {
@@ -714,6 +760,7 @@ void batch_impl(void* a_) {
   c->stats.sum_flips = K.sumF;
   c->stats.rounds = K.rounds;
   c->stats.flip_iterations = K.iters;
+  c->stats.peak_active = K.peakA;
   c->stats.num_points = n;
   uint64_t nfailed = 0;
   for (auto& v : failed) nfailed += v.size();
@@ -855,6 +902,7 @@ void wtet_config_default(wtet_config* cfg) {
   cfg->max_flip_iterations = 10;
   cfg->predicate_mode = 0;
   cfg->collect_stats = 0;
+  cfg->presize_mode = 0;
 }
 
 static wtet_status delaunay_common(wtet_ctx* c, const float* xyz, bool dev, uint32_t n,
@@ -884,6 +932,10 @@ static wtet_status delaunay_common(wtet_ctx* c, const float* xyz, bool dev, uint
   else wtet_config_default(&a.cfg);
   if (a.cfg.insertion_rule != 0 && a.cfg.insertion_rule != 1) {
     c->err = "Invalid insertion_rule. Must be 'circumcenter' or 'centroid'.";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (a.cfg.presize_mode < 0 || a.cfg.presize_mode > 2) {
+    c->err = "presize_mode must be 0 (fixed), 1 (learned) or 2 (grow)";
     return WTET_ERR_INVALID_ARGUMENT;
   }
   a.out = out;
@@ -955,6 +1007,10 @@ static wtet_status batch_common(wtet_ctx* c, const float* xyz, bool dev, const u
   else wtet_config_default(&a.cfg);
   if (a.cfg.insertion_rule != 0 && a.cfg.insertion_rule != 1) {
     c->err = "Invalid insertion_rule. Must be 'circumcenter' or 'centroid'.";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (a.cfg.presize_mode < 0 || a.cfg.presize_mode > 2) {
+    c->err = "presize_mode must be 0 (fixed), 1 (learned) or 2 (grow)";
     return WTET_ERR_INVALID_ARGUMENT;
   }
   a.out = out;
@@ -1051,6 +1107,305 @@ wtet_status wtet_device_free(wtet_ctx* c, void* d_ptr) {
   if (!c) return WTET_ERR_INVALID_ARGUMENT;
   if (d_ptr && hipFree(d_ptr) != hipSuccess) {
     (void)hipGetLastError();
+    return WTET_ERR_DEVICE;
+  }
+  return WTET_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// learned presizing
+// ---------------------------------------------------------------------------
+static_assert(kParamCount == WTET_PRESIZE_PARAMS, "parameter layout");
+static_assert(kFeat == WTET_PRESIZE_FEATURES && kOut == WTET_PRESIZE_OUTPUTS, "model shape");
+
+struct FeatArgs {
+  wtet_ctx* c;
+  const float* d_xyz;
+  std::vector<uint32_t> off;
+  float* feat;
+};
+
+static void features_impl(void* a_) {
+  FeatArgs& a = *static_cast<FeatArgs*>(a_);
+  wtet_ctx* c = a.c;
+  HIP_CHECK(hipSetDevice(c->device));
+  uint32_t C = (uint32_t)a.off.size() - 1, n = a.off[C];
+  hipStream_t s = c->stream;
+  ensure(c, c->doff, ((size_t)C + 1) * 4);
+  ensure(c, c->mm6, (size_t)C * 24);
+  ensure(c, c->feat_acc, (size_t)C * (kHistBins / 2 + 9) * 8 + 64);
+  ensure(c, c->feat, (size_t)C * kFeat * 4);
+  ensure(c, c->xyzn, (size_t)n * 12 + 16);
+  ensure(c, c->keys, (size_t)n * 8 + 16);
+  ensure(c, c->ids, (size_t)n * 4 + 16);
+  HIP_CHECK(hipMemcpyAsync(c->doff.p, a.off.data(), ((size_t)C + 1) * 4, hipMemcpyHostToDevice, s));
+  Clouds CL;
+  CL.off = P<uint32_t>(c->doff);
+  CL.C = C;
+  CL.V = n + 4 * C;
+  int kbits, end_bit;
+  key_layout(C, &kbits, &end_bit);
+  CL.kbits = kbits;
+  CL.cshift = 3 * kbits;
+  launch_prepare(a.d_xyz, n, CL, P<uint32_t>(c->mm6), P<float>(c->xyzn), P<uint64_t>(c->keys),
+                 P<uint32_t>(c->ids), s);
+  launch_features(a.d_xyz, n, CL.off, C, P<uint32_t>(c->mm6), P<double>(c->feat_acc),
+                  P<float>(c->feat), s);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(a.feat, c->feat.p, (size_t)C * kFeat * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+struct PredictArgs {
+  wtet_ctx* c;
+  const float* params;
+  const float* feat;
+  uint32_t C;
+  float* pred;
+};
+
+static void predict_impl(void* a_) {
+  PredictArgs& a = *static_cast<PredictArgs*>(a_);
+  wtet_ctx* c = a.c;
+  HIP_CHECK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  DevBuf p, f, o;
+  ensure(c, p, kParamCount * 4);
+  ensure(c, f, (size_t)a.C * kFeat * 4);
+  ensure(c, o, (size_t)a.C * kOut * 4);
+  HIP_CHECK(hipMemcpyAsync(p.p, a.params, kParamCount * 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(f.p, a.feat, (size_t)a.C * kFeat * 4, hipMemcpyHostToDevice, s));
+  launch_mlp_predict(P<float>(p), P<float>(f), a.C, P<float>(o), s);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(a.pred, o.p, (size_t)a.C * kOut * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  (void)hipFree(p.p);
+  (void)hipFree(f.p);
+  (void)hipFree(o.p);
+}
+
+// device buffers of one training / gradient call
+struct MlpBufs {
+  DevBuf params, grad, m, v, xraw, yraw, idx, x, y, h1, h2, out, d3, d2, d1, loss;
+  std::vector<DevBuf*> all() {
+    return {&params, &grad, &m, &v, &xraw, &yraw, &idx, &x, &y, &h1, &h2, &out, &d3, &d2, &d1, &loss};
+  }
+  ~MlpBufs() {
+    for (DevBuf* b : all())
+      if (b->p) (void)hipFree(b->p);
+  }
+};
+
+static MlpWork mlp_alloc(wtet_ctx* c, MlpBufs& B, uint32_t rows, uint32_t batch) {
+  ensure(c, B.params, kParamCount * 4);
+  ensure(c, B.grad, kParamCount * 4);
+  ensure(c, B.xraw, (size_t)rows * kFeat * 4);
+  ensure(c, B.yraw, (size_t)rows * kOut * 4);
+  ensure(c, B.idx, (size_t)std::max(rows, batch) * 4);
+  ensure(c, B.x, (size_t)batch * kFeat * 4);
+  ensure(c, B.y, (size_t)batch * kOut * 4);
+  ensure(c, B.h1, (size_t)batch * kHidden * 4);
+  ensure(c, B.h2, (size_t)batch * kHidden * 4);
+  ensure(c, B.out, (size_t)batch * kOut * 4);
+  ensure(c, B.d3, (size_t)batch * kOut * 4);
+  ensure(c, B.d2, (size_t)batch * kHidden * 4);
+  ensure(c, B.d1, (size_t)batch * kHidden * 4);
+  ensure(c, B.loss, 64);
+  MlpWork w;
+  w.x = P<float>(B.x);
+  w.y = P<float>(B.y);
+  w.h1 = P<float>(B.h1);
+  w.h2 = P<float>(B.h2);
+  w.out = P<float>(B.out);
+  w.d3 = P<float>(B.d3);
+  w.d2 = P<float>(B.d2);
+  w.d1 = P<float>(B.d1);
+  w.grad = P<float>(B.grad);
+  w.loss = P<float>(B.loss);
+  return w;
+}
+
+struct GradArgs {
+  wtet_ctx* c;
+  const float* params;
+  const float* feat;
+  const float* tgt;
+  uint32_t rows;
+  float* loss;
+  float* grad;
+};
+
+static void grad_impl(void* a_) {
+  GradArgs& a = *static_cast<GradArgs*>(a_);
+  wtet_ctx* c = a.c;
+  HIP_CHECK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  MlpBufs B;
+  MlpWork w = mlp_alloc(c, B, a.rows, a.rows);
+  std::vector<uint32_t> idx(a.rows);
+  for (uint32_t i = 0; i < a.rows; ++i) idx[i] = i;
+  HIP_CHECK(hipMemcpyAsync(B.params.p, a.params, kParamCount * 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.xraw.p, a.feat, (size_t)a.rows * kFeat * 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.yraw.p, a.tgt, (size_t)a.rows * kOut * 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.idx.p, idx.data(), (size_t)a.rows * 4, hipMemcpyHostToDevice, s));
+  launch_fill_u32(P<uint32_t>(B.grad), 0u, kParamCount, s);
+  const float* Pp = P<float>(B.params);
+  launch_gather_standardize(P<float>(B.xraw), P<uint32_t>(B.idx), a.rows, kFeat, Pp + kOffFeatMean,
+                            Pp + kOffFeatStd, w.x, s);
+  launch_gather_standardize(P<float>(B.yraw), P<uint32_t>(B.idx), a.rows, kOut, Pp + kOffTgtMean,
+                            Pp + kOffTgtStd, w.y, s);
+  launch_mlp_forward_backward(Pp, w, a.rows, s);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(a.grad, B.grad.p, kParamCount * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipMemcpyAsync(a.loss, B.loss.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+struct TrainArgs {
+  wtet_ctx* c;
+  float* params;
+  const float* feat;
+  const float* tgt;
+  uint32_t rows, epochs, batch;
+  float lr;
+  uint64_t seed;
+  float* curve;
+};
+
+static void train_impl(void* a_) {
+  TrainArgs& a = *static_cast<TrainArgs*>(a_);
+  wtet_ctx* c = a.c;
+  HIP_CHECK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  MlpBufs B;
+  MlpWork w = mlp_alloc(c, B, a.rows, a.batch);
+  ensure(c, B.m, kParamCount * 4);
+  ensure(c, B.v, kParamCount * 4);
+  HIP_CHECK(hipMemcpyAsync(B.params.p, a.params, kParamCount * 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.xraw.p, a.feat, (size_t)a.rows * kFeat * 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.yraw.p, a.tgt, (size_t)a.rows * kOut * 4, hipMemcpyHostToDevice, s));
+  launch_fill_u32(P<uint32_t>(B.m), 0u, kParamCount, s);
+  launch_fill_u32(P<uint32_t>(B.v), 0u, kParamCount, s);
+  launch_fill_u32(P<uint32_t>(B.grad), 0u, kParamCount, s);
+  float* Pp = P<float>(B.params);
+  std::vector<uint32_t> perm(a.rows);
+  std::vector<float> losses;
+  uint64_t rng = a.seed * 0x9E3779B97F4A7C15ull + 1;
+  auto next = [&]() {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return rng;
+  };
+  uint32_t nb = a.rows / a.batch;
+  ensure(c, B.loss, (size_t)std::max<uint32_t>(nb, 1) * 4 + 64);
+  uint32_t step = 0;
+  for (uint32_t e = 0; e < a.epochs; ++e) {
+    for (uint32_t i = 0; i < a.rows; ++i) perm[i] = i;
+    for (uint32_t i = a.rows - 1; i > 0; --i) std::swap(perm[i], perm[next() % (i + 1)]);
+    HIP_CHECK(hipMemcpyAsync(B.idx.p, perm.data(), (size_t)a.rows * 4, hipMemcpyHostToDevice, s));
+    for (uint32_t b = 0; b < nb; ++b) {
+      const uint32_t* bi = P<uint32_t>(B.idx) + (size_t)b * a.batch;
+      launch_gather_standardize(P<float>(B.xraw), bi, a.batch, kFeat, Pp + kOffFeatMean,
+                                Pp + kOffFeatStd, w.x, s);
+      launch_gather_standardize(P<float>(B.yraw), bi, a.batch, kOut, Pp + kOffTgtMean,
+                                Pp + kOffTgtStd, w.y, s);
+      w.loss = P<float>(B.loss) + b;  // one slot per minibatch of this epoch
+      launch_mlp_forward_backward(Pp, w, a.batch, s);
+      launch_adam(Pp, w.grad, P<float>(B.m), P<float>(B.v), a.lr, 0.9f, 0.999f, 1e-8f, ++step, s);
+    }
+    HIP_CHECK(hipGetLastError());
+    if (a.curve && nb) {
+      losses.resize(nb);
+      HIP_CHECK(hipMemcpyAsync(losses.data(), B.loss.p, (size_t)nb * 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      double sum = 0;
+      for (float l : losses) sum += l;
+      a.curve[e] = (float)(sum / nb);
+    }
+  }
+  HIP_CHECK(hipMemcpyAsync(a.params, B.params.p, kParamCount * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+static std::vector<uint32_t> offsets32(const uint64_t* offsets, uint32_t C) {
+  std::vector<uint32_t> off((size_t)C + 1);
+  for (uint32_t i = 0; i <= C; ++i) off[i] = (uint32_t)offsets[i];
+  return off;
+}
+
+wtet_status wtet_presize_features(wtet_ctx* c, const float* d_xyz, const uint64_t* offsets,
+                                  uint32_t num_clouds, float* feat) {
+  if (!c || !d_xyz || !offsets || !feat || num_clouds == 0 || offsets[0] != 0 ||
+      offsets[num_clouds] > (1ull << 26))
+    return WTET_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < num_clouds; ++i)
+    if (offsets[i + 1] <= offsets[i]) return WTET_ERR_INVALID_ARGUMENT;
+  FeatArgs a{c, d_xyz, offsets32(offsets, num_clouds), feat};
+  return run_guarded(c, features_impl, &a);
+}
+
+wtet_status wtet_presize_predict(wtet_ctx* c, const float* params, const float* feat,
+                                 uint32_t num_clouds, float* pred) {
+  if (!c || !params || !feat || !pred || num_clouds == 0) return WTET_ERR_INVALID_ARGUMENT;
+  PredictArgs a{c, params, feat, num_clouds, pred};
+  return run_guarded(c, predict_impl, &a);
+}
+
+struct SetModelArgs {
+  wtet_ctx* c;
+  const float* params;
+};
+
+static void set_model_impl(void* a_) {
+  SetModelArgs& a = *static_cast<SetModelArgs*>(a_);
+  HIP_CHECK(hipSetDevice(a.c->device));
+  ensure(a.c, a.c->presize_params, kParamCount * 4);
+  HIP_CHECK(hipMemcpy(a.c->presize_params.p, a.params, kParamCount * 4, hipMemcpyHostToDevice));
+  a.c->presize_loaded = true;
+}
+
+wtet_status wtet_set_presize_model(wtet_ctx* c, const float* params) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  if (!params) {
+    c->presize_loaded = false;
+    return WTET_OK;
+  }
+  for (int k = 0; k < kFeat; ++k)
+    if (!(params[kOffFeatStd + k] > 0.0f)) {
+      c->err = "presize model: feature std must be positive";
+      return WTET_ERR_INVALID_ARGUMENT;
+    }
+  SetModelArgs a{c, params};
+  return run_guarded(c, set_model_impl, &a);
+}
+
+wtet_status wtet_mlp_loss_grad(wtet_ctx* c, const float* params, const float* feat,
+                               const float* targets, uint32_t rows, float* loss, float* grad) {
+  if (!c || !params || !feat || !targets || !loss || !grad || rows == 0 || rows > 4096)
+    return WTET_ERR_INVALID_ARGUMENT;
+  GradArgs a{c, params, feat, targets, rows, loss, grad};
+  return run_guarded(c, grad_impl, &a);
+}
+
+wtet_status wtet_mlp_train(wtet_ctx* c, float* params, const float* feat, const float* targets,
+                           uint32_t rows, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                           float* loss_curve) {
+  if (!c || !params || !feat || !targets || batch == 0 || batch > 4096 || rows < batch ||
+      !(lr > 0.0f))
+    return WTET_ERR_INVALID_ARGUMENT;
+  TrainArgs a{c, params, feat, targets, rows, epochs, batch, lr, seed, loss_curve};
+  return run_guarded(c, train_impl, &a);
+}
+
+wtet_status wtet_device_copy(wtet_ctx* c, void* d_dst, const void* d_src, size_t bytes) {
+  if (!c || !d_dst || !d_src || bytes % 16 || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16)
+    return WTET_ERR_INVALID_ARGUMENT;
+  if (hipSetDevice(c->device) != hipSuccess) return WTET_ERR_DEVICE;
+  launch_copy16(static_cast<uint4*>(d_dst), static_cast<const uint4*>(d_src), bytes / 16, c->stream);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+    c->err = "device copy failed";
     return WTET_ERR_DEVICE;
   }
   return WTET_OK;

@@ -89,6 +89,8 @@ void launch_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2, h
 // fills
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 void launch_fill_u64(uint64_t* p, uint64_t v, uint64_t n, hipStream_t s);
+// plain streaming copy (16 B per lane), also the HBM-counter calibration kernel
+void launch_copy16(uint4* dst, const uint4* src, uint64_t n, hipStream_t s);
 
 // predicate batches (for checking the device predicates)
 void launch_pred_batch(int kind, const float* pts, const uint32_t* idx, uint32_t n, int mode,

@@ -4,6 +4,7 @@
 // wave ballots: each wavefront ranks its lanes with popcount(ballot & lt),
 // wave totals are combined in LDS, block totals are scanned by one block,
 // and a scatter pass writes every element once (stable order).
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -547,6 +548,15 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_copy16(uint4* dst, const uint4* src, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+void launch_copy16(uint4* dst, const uint4* src, uint64_t n, hipStream_t s) {
+  if (n) k_copy16<<<(uint32_t)std::min<uint64_t>(nblocks(n), 8192), kBlock, 0, s>>>(dst, src, n);
+}
+
 void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
   if (n) k_fill_u32<<<nblocks(n), kBlock, 0, s>>>(p, v, n);
 }

@@ -302,7 +302,7 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
   if (kind == 0) {
     uint32_t x = get4(tv, pos[0]), y = get4(tv, pos[1]), z = get4(tv, pos[2]);
     int qx = find4(nv, x), qy = find4(nv, y), qz = find4(nv, z);
-    uint32_t s = at_add_u32(&M.ctr[C_TETS], 1u);
+    uint32_t s = append_slot(&M.ctr[C_TETS]);  // one atomic per wavefront
     uint32_t Txy = t, Tyz = n, Tzx = s;
     M.tet[Txy] = make_uint4(x, y, o, a);
     M.tet[Tyz] = make_uint4(y, z, o, a);

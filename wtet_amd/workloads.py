@@ -5,6 +5,9 @@ lattice      : k^3 grid i/(k-1), ix-major, iz fastest (src/tests.rs:1103-1117)
 sphere       : i.i.d. Gaussian triples (f64) normalized to the unit sphere, cast to f32
 shell        : points in a thin shell of radius 0.4 around (0.5, 0.5, 0.5)
 clusters     : mixture of isotropic Gaussians, samples outside [0,1]^3 redrawn
+anisotropic  : mixture of k anisotropic Gaussians (random rotations,
+               covariance eigenvalues U[1e-4, 1e-2]) plus a uniform share,
+               samples outside [0,1]^3 redrawn
 mixed_batch  : many clouds, log-normal sizes (median 2000, sigma 0.63, clipped
                to [64, 50000]), each a ``clusters`` cloud with K ~ U{1..8}
                Gaussians, sigma_k ~ U[0.02, 0.2]; returns (points, offsets)
@@ -51,6 +54,27 @@ def clusters(n: int, seed: int, k: int | None = None, sigma=(0.02, 0.2)) -> np.n
         s = s[np.all((s >= 0) & (s <= 1), axis=1)]
         out = np.vstack([out, s])
     return out[:n].astype(np.float32)
+
+
+def anisotropic(n: int, seed: int, k: int = 32, uniform_share: float = 0.1) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    nu = int(round(n * uniform_share))
+    centers = rng.random((k, 3))
+    rots = []
+    for _ in range(k):
+        q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+        rots.append(q * np.sign(np.diag(r)))
+    scales = np.sqrt(rng.uniform(1e-4, 1e-2, (k, 3)))
+    out = np.empty((0, 3))
+    m = n - nu
+    while len(out) < m:
+        c = rng.integers(0, k, m)
+        z = rng.standard_normal((m, 3)) * scales[c]
+        s = centers[c] + np.einsum("nij,nj->ni", np.stack(rots)[c], z)
+        s = s[np.all((s >= 0) & (s <= 1), axis=1)]
+        out = np.vstack([out, s])
+    pts = np.vstack([out[:m], rng.random((nu, 3))])
+    return pts[rng.permutation(n)].astype(np.float32)
 
 
 def mixed_batch(num_clouds: int, seed: int, median: float = 2000.0, sigma: float = 0.63,
