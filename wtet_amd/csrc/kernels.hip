@@ -161,10 +161,49 @@ __global__ void k_start_tets(Mesh M, StartTets S, Clouds K) {
   if (g < 5u * K.C) start_tet_body(M, S, g, K);
 }
 
+// One cloud: every thread folds many points (grid-stride, coalesced), the
+// block combines in LDS, and one lane per block issues the 6 atomics.
+__global__ void __launch_bounds__(kBlock) k_minmax_single(const float* xyz, uint32_t n, uint32_t* mm) {
+  __shared__ uint32_t red[6][kBlock / 64];
+  uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = gid(); i < n; i += stride)
+    for (int k = 0; k < 3; ++k) {
+      float v = xyz[3 * (size_t)i + k];
+      if (v == v) {
+        uint32_t o = ord_f32(v);
+        lo[k] = min(lo[k], o);
+        hi[k] = max(hi[k], o);
+      }
+    }
+  int wave = threadIdx.x >> 6;
+  for (int k = 0; k < 3; ++k) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[k] = min(lo[k], (uint32_t)__shfl_xor(lo[k], off));
+      hi[k] = max(hi[k], (uint32_t)__shfl_xor(hi[k], off));
+    }
+    if (__lane_id() == 0) {
+      red[k][wave] = lo[k];
+      red[3 + k][wave] = hi[k];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    uint32_t v = k < 3 ? 0xFFFFFFFFu : 0u;
+    for (int w = 0; w < kBlock / 64; ++w) v = k < 3 ? min(v, red[k][w]) : max(v, red[k][w]);
+    if (k < 3) atomicMin(&mm[k], v);
+    else atomicMax(&mm[k], v);
+  }
+}
+
 void launch_prepare(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, float* xyzn,
                     uint64_t* keys, uint32_t* ids, hipStream_t s) {
   k_minmax_init<<<nblocks(6u * K.C), kBlock, 0, s>>>(mm, K.C);
-  k_minmax<<<nblocks(n), kBlock, 0, s>>>(xyz, n, K, mm);
+  if (K.C <= 1)
+    k_minmax_single<<<std::max(1u, std::min(nblocks(n) / 16u, 1024u)), kBlock, 0, s>>>(xyz, n, mm);
+  else
+    k_minmax<<<nblocks(n), kBlock, 0, s>>>(xyz, n, K, mm);
   k_normalize<<<nblocks(n), kBlock, 0, s>>>(xyz, n, K, mm, xyzn, keys, ids);
 }
 
@@ -372,10 +411,58 @@ __global__ void k_flip_check(Mesh M, const uint32_t* act, uint4* prop, uint32_t 
   flush_stats(ls, st);
 }
 
-__global__ void k_flip_decide_exec(Mesh M, const uint4* prop, uint32_t A, uint32_t it,
-                                   uint32_t round, uint4* out, uint32_t* act_next) {
+// exclusive prefix of a small count v (< 4) within the block; *tot = block
+// total.  Every thread of the block must call it; w is LDS of kBlock/64.
+__device__ __forceinline__ uint32_t block_prefix_small(uint32_t v, uint32_t* tot, uint32_t* w) {
+  uint64_t lt = (1ull << __lane_id()) - 1ull;
+  uint64_t b0 = __ballot(v & 1u), b1 = __ballot(v & 2u);
+  uint32_t r = __popcll(b0 & lt) + 2u * __popcll(b1 & lt);
+  int wave = threadIdx.x >> 6;
+  if (__lane_id() == 0) w[wave] = __popcll(b0) + 2u * __popcll(b1);
+  __syncthreads();
+  uint32_t off = 0, t = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    off += k < wave ? w[k] : 0u;
+    t += w[k];
+  }
+  __syncthreads();
+  *tot = t;
+  return off + r;
+}
+
+// Decide every proposal, then claim list space for the whole block with
+// one atomic per counter (flips, fresh slots, next active entries), then
+// execute the winners.  Counters share a cache line, so per-wave atomics
+// from every wave serialize; per-block claims keep them few.
+__global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, const uint4* prop, uint32_t A,
+                                                             uint32_t it, uint32_t round, uint4* out,
+                                                             uint32_t* act_next) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  __shared__ uint32_t base[3];
   uint32_t j = gid();
-  if (j < A) flip_decide_exec_body(M, prop, j, it, round, out, act_next);
+  uint4 pr = make_uint4(kInvalid, 0u, 0u, 0u);
+  if (j < A) pr = prop[j];
+  uint32_t qid[2], nq = 0, nslot = 0;
+  bool win = flip_decide_body(M, pr, it, qid, &nq, &nslot);
+  uint32_t nqa = nq + nslot;  // the fresh slot is queued too
+  uint32_t tf, ts, tq;
+  uint32_t rf = block_prefix_small(win ? 1u : 0u, &tf, wsum);
+  uint32_t rs = block_prefix_small(nslot, &ts, wsum);
+  uint32_t rq = block_prefix_small(nqa, &tq, wsum);
+  if (threadIdx.x == 0) {
+    base[0] = tf ? atomicAdd(&M.ctr[C_FLIPS], tf) : 0u;
+    base[1] = ts ? atomicAdd(&M.ctr[C_TETS], ts) : 0u;
+    base[2] = tq ? atomicAdd(&M.ctr[C_ACT_NEXT], tq) : 0u;
+  }
+  __syncthreads();
+  uint32_t q = base[2] + rq;
+  for (uint32_t k = 0; k < nq; ++k) act_next[q + k] = qid[k];
+  if (win) {
+    uint32_t s = nslot ? base[1] + rs : kInvalid;
+    flip_exec_body(M, pr, it, round, s, &out[base[0] + rf]);
+    if (nslot) act_next[q + nq] = s;
+  }
 }
 
 // F is an upper bound (the active-list length); the flip count itself is

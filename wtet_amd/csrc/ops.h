@@ -278,20 +278,17 @@ WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint
   }
 }
 
-WT_HD void queue_tet(const Mesh& M, uint32_t s, uint32_t* act_next) {
-  uint32_t old = at_or_u32(&M.tinfo[s], kAlive | kQueued);
-  if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = s;
-}
-
 // ---------------------------------------------------------------------------
 // execute a winning flip (src/shaders/flip.wgsl:531-845).  New tets reuse
 // the old slots; a 2-3 flip takes one fresh slot, a 3-2 flip frees one.
 // External faces keep the old neighbour pointer; flip_update_body fixes
 // them once every flip of the iteration is written.
+// s: the fresh slot of a 2-3 flip.  The caller queues the new tets (the
+// reused slots were claimed by flip_decide_body).
 // out: x,y,z = slots of the new tets (z unused for 3-2), w = kind
 // ---------------------------------------------------------------------------
 WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t round,
-                          uint4* outk, uint32_t* act_next) {
+                          uint32_t s, uint4* outk) {
   uint32_t t = pr.x, f = pr.y & 3u, kind = (pr.y >> 2) & 1u, edge = (pr.y >> 3) & 3u;
   uint32_t n = pr.z;
   uint4 tv = M.tet[t], to = M.opp[t];
@@ -302,7 +299,6 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
   if (kind == 0) {
     uint32_t x = get4(tv, pos[0]), y = get4(tv, pos[1]), z = get4(tv, pos[2]);
     int qx = find4(nv, x), qy = find4(nv, y), qz = find4(nv, z);
-    uint32_t s = append_slot(&M.ctr[C_TETS]);  // one atomic per wavefront
     uint32_t Txy = t, Tyz = n, Tzx = s;
     M.tet[Txy] = make_uint4(x, y, o, a);
     M.tet[Tyz] = make_uint4(y, z, o, a);
@@ -329,10 +325,7 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
     M.tsplit[s] = make_uint2(0u, 0u);
     M.fit[s] = 0u;
     M.fround[s] = 0u;
-    M.tinfo[s] = kAlive | kQueued;  // fresh slot, cannot be queued yet
-    queue_tet(M, Txy, act_next);
-    queue_tet(M, Tyz, act_next);
-    act_next[append_slot(&M.ctr[C_ACT_NEXT])] = Tzx;
+    M.tinfo[s] = kAlive | kQueued;  // fresh slot: queued by the caller
     *outk = make_uint4(Txy, Tyz, Tzx, 0u);
   } else {
     // rotate the face triple so the offending edge is (x, y), third z
@@ -368,8 +361,6 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
     M.fround[m] = round;
     M.tinfo[m] = kDead;
     M.tfwd[m] = A;
-    queue_tet(M, A, act_next);
-    queue_tet(M, B, act_next);
     *outk = make_uint4(A, B, kInvalid, 1u);
   }
 }
@@ -378,20 +369,46 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
 // winner executes at once (its tets belong to no other winner), losers go
 // back to the active list (src/shaders/mark_rejected_flips.wgsl:61-178 and
 // src/shaders/flip.wgsl:531-845, one pass here).
-WT_HD void flip_decide_exec_body(const Mesh& M, const uint4* prop, uint32_t j, uint32_t it,
-                                 uint32_t round, uint4* out, uint32_t* act_next) {
-  uint4 pr = prop[j];
-  if (pr.x == kInvalid) return;
+// Decide proposal j and claim the queue entries it needs: a winner's two
+// reused slots (and, for a 2-3 flip, the fresh slot, counted in *nslot),
+// or a loser's own tet.  qid[0..*nq) are the tets to append to the next
+// active list (kQueued is set here, so each tet is appended once); for a
+// 2-3 winner the fresh slot is appended by the caller after qid.
+WT_HD bool flip_decide_body(const Mesh& M, const uint4& pr, uint32_t it, uint32_t qid[2],
+                            uint32_t* nq, uint32_t* nslot) {
+  *nq = 0;
+  *nslot = 0;
+  if (pr.x == kInvalid) return false;
   uint64_t key = flip_key(it, pr.x);
   bool win = M.fvote[pr.x] == key && M.fvote[pr.z] == key;
   if (((pr.y >> 2) & 1u) && M.fvote[pr.w] != key) win = false;
   if (win) {
-    uint32_t k = append_slot(&M.ctr[C_FLIPS]);
-    flip_exec_body(M, pr, it, round, &out[k], act_next);
+    uint32_t reuse[2] = {pr.x, pr.z};
+    for (int k = 0; k < 2; ++k) {
+      uint32_t old = at_or_u32(&M.tinfo[reuse[k]], kAlive | kQueued);
+      if (!(old & kQueued)) qid[(*nq)++] = reuse[k];
+    }
+    if (((pr.y >> 2) & 1u) == 0u) *nslot = 1;
   } else {
     uint32_t old = at_or_u32(&M.tinfo[pr.x], kQueued);
-    if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = pr.x;
+    if (!(old & kQueued)) qid[(*nq)++] = pr.x;
   }
+  return win;
+}
+
+// Sequential form (one proposal at a time, host callers): counters advance
+// with one atomic each.
+WT_HD void flip_decide_exec_body(const Mesh& M, const uint4* prop, uint32_t j, uint32_t it,
+                                 uint32_t round, uint4* out, uint32_t* act_next) {
+  uint4 pr = prop[j];
+  uint32_t qid[2], nq, nslot;
+  bool win = flip_decide_body(M, pr, it, qid, &nq, &nslot);
+  for (uint32_t k = 0; k < nq; ++k) act_next[at_add_u32(&M.ctr[C_ACT_NEXT], 1u)] = qid[k];
+  if (!win) return;
+  uint32_t s = nslot ? at_add_u32(&M.ctr[C_TETS], 1u) : kInvalid;
+  uint32_t f = at_add_u32(&M.ctr[C_FLIPS], 1u);
+  flip_exec_body(M, pr, it, round, s, &out[f]);
+  if (nslot) act_next[at_add_u32(&M.ctr[C_ACT_NEXT], 1u)] = s;
 }
 
 WT_HD void fix_face(const Mesh& M, uint32_t T, uint32_t e, uint32_t it) {

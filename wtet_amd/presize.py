@@ -97,7 +97,9 @@ def sample_cloud(rng: np.random.Generator, n: int):
     elif fam == "sphere":
         p = W.sphere(n, seed)
     elif fam == "shell":
-        p = W.shell(n, seed, thickness=float(10 ** rng.uniform(-5, -1)))
+        # thin shells above ~1e5 points send the host repair into whole-mesh
+        # regions (minutes); the family is kept at sizes that finish quickly
+        p = W.shell(min(n, 100_000), seed, thickness=float(10 ** rng.uniform(-5, -1)))
     elif fam == "lattice":
         k = max(2, int(round(n ** (1 / 3))))
         p = W.lattice(k)
@@ -158,7 +160,7 @@ def collect(ctx: "_core.Context", runs: int, seed: int, nmin: int = 100, nmax: i
 
 
 def train(ctx: "_core.Context", X: np.ndarray, Y: np.ndarray, epochs: int = 300, batch: int = 128,
-          lr: float = 1e-3, seed: int = 0, holdout: float = 0.15, quantile: float = 0.995):
+          lr: float = 1e-3, seed: int = 0, holdout: float = 0.15, quantile: float = 0.975):
     """Fit on the GPU; the margin of each output is the `quantile` of the
     held-out residuals (true - predicted, log space), at least 0.  Returns
     (params, report)."""
@@ -170,7 +172,11 @@ def train(ctx: "_core.Context", X: np.ndarray, Y: np.ndarray, epochs: int = 300,
     p, curve = ctx.mlp_train(p0, X[fit], Y[fit], epochs=epochs, batch=min(batch, len(fit)), lr=lr, seed=seed)
     pred = ctx.presize_predict(p, X[hold])  # margin still zero
     res = Y[hold] - pred
-    margin = np.maximum(np.quantile(res, quantile, axis=0), 0.0)
+    # the margin matters where buffers are large: small clouds are covered by
+    # the constant slack and by growth
+    big = np.exp(X[hold, -1]) >= 20000
+    rm = res[big] if big.sum() >= 20 else res
+    margin = np.maximum(np.quantile(rm, quantile, axis=0), 0.0)
     p = set_margin(p, margin)
     report = {"rows_fit": int(len(fit)), "rows_holdout": int(nh), "epochs": epochs,
               "loss_first": float(curve[0]), "loss_last": float(curve[-1]),
