@@ -121,8 +121,15 @@ def main():
     launches, tot_ms = kt[dom]
     bytes_dom = BYTES_PER_UNIT[dom] * st_p["sum_active"]
     achieved = bytes_dom / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else 0.0
+    # measured HBM bytes per checked tet (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # copy-kernel calibrated) times this run's checked tets per launch
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "flip_check_traffic.json")
+    if os.path.exists(tpath):
+        per_tet = json.load(open(tpath))["traffic_bytes_per_checked_tet"]
+        traffic = int(per_tet * st_p["sum_active"] / max(launches, 1))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_flip_check", "launches": launches,
                 "avg_launch_ms": round(tot_ms / max(launches, 1), 4),
                 "bytes_per_launch_avg": int(bytes_dom / max(launches, 1))}
