@@ -206,6 +206,23 @@ wtet_status wtet_device_copy(wtet_ctx *ctx, void *d_dst, const void *d_src, size
  * reference. */
 wtet_status wtet_set_insertion_ring(wtet_ctx *ctx, int ring);
 
+/* The flip iterations of a round run on the device without host reads
+ * (counts, iteration tag and list parity live in device counters); they
+ * are captured once in a HIP graph and replayed every round (on by
+ * default; off: direct launches).  The three list kernels are persistent
+ * grid-stride kernels of `blocks` workgroups (default 1024).  No
+ * counterpart in the reference, which reads counters back every iteration
+ * (src/phase1.rs:16-1431). */
+wtet_status wtet_set_flip_graphs(wtet_ctx *ctx, int on);
+wtet_status wtet_set_flip_grid(wtet_ctx *ctx, uint32_t blocks);
+wtet_status wtet_get_flip_grid(const wtet_ctx *ctx, uint32_t *blocks, uint32_t *graph_captures);
+/* Autotuner: triangulates the given clouds (device points) at each
+ * candidate grid (256 .. 4096 workgroups), keeps the fastest device time
+ * for this context and reports it. */
+wtet_status wtet_autotune(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
+                          uint32_t num_clouds, const wtet_config *cfg, uint32_t *best_grid,
+                          double *best_ms);
+
 /* Learned presizing (no counterpart in the reference; the reference sizes
  * its buffers with a fixed formula, src/gpu/buffers.rs:112-579).
  * Features of a cloud: the 512-bin occupancy histogram of the top 9 bits of

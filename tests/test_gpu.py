@@ -194,3 +194,49 @@ def test_resident_matches_host_result(oracle, ctx):
     back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
     idx = (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m)
     assert np.array_equal(back, idx)
+
+
+def test_flip_graph_replays_match_direct_launches(oracle, ctx):
+    """Each round replays the captured graph of its flip iterations, with
+    eager launches and host reads between replays; every call must equal
+    the direct-launch result and the oracle."""
+    p = W.uniform(50000, 61)
+    ref = oracle.canonical_real(_check(oracle, ctx, p).tets, len(p))
+    ctx.set_flip_graphs(False)
+    try:
+        direct = ctx.delaunay(p)
+    finally:
+        ctx.set_flip_graphs(True)
+    assert np.array_equal(oracle.canonical_real(direct.tets, len(p)), ref)
+    _, before = ctx.flip_grid()
+    small = W.uniform(7000, 62)
+    small_ref = oracle.canonical_real(oracle.delaunay(oracle.normalize_points(small)[0]), len(small))
+    for q, want in ((p, ref), (small, small_ref), (p, ref)):  # the smaller call recaptures
+        r = ctx.delaunay(q)
+        assert np.array_equal(oracle.canonical_real(r.tets, len(q)), want)
+    _, after = ctx.flip_grid()
+    assert after > before
+
+
+@pytest.mark.parametrize("blocks", [1, 64, 4096])
+def test_flip_grid_sizes(oracle, ctx, blocks):
+    g0, _ = ctx.flip_grid()
+    ctx.set_flip_grid(blocks)
+    try:
+        _check(oracle, ctx, W.uniform(20000, 63))
+    finally:
+        ctx.set_flip_grid(g0)
+
+
+def test_autotune_picks_a_candidate(oracle, ctx):
+    p = W.uniform(100000, 64)
+    d = ctx.upload(p)
+    g0, _ = ctx.flip_grid()
+    try:
+        g, ms = ctx.autotune(d, [0, len(p)])
+        assert g in (256, 512, 1024, 2048, 4096) and ms > 0
+        assert ctx.flip_grid()[0] == g
+    finally:
+        ctx.free(d)
+        ctx.set_flip_grid(g0)
+    _check(oracle, ctx, p)

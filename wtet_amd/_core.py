@@ -168,6 +168,10 @@ def lib():
             L.wtet_device_info.argtypes = [P, ctypes.c_char_p, u32]
             L.wtet_set_profiling.argtypes = [P, ctypes.c_int]
             L.wtet_set_insertion_ring.argtypes = [P, ctypes.c_int]
+            L.wtet_set_flip_graphs.argtypes = [P, ctypes.c_int]
+            L.wtet_set_flip_grid.argtypes = [P, u32]
+            L.wtet_get_flip_grid.argtypes = [P, P, P]
+            L.wtet_autotune.argtypes = [P, P, P, u32, ctypes.POINTER(_Config), P, P]
             L.wtet_device_upload.argtypes = [P, P, ctypes.c_size_t, ctypes.POINTER(P)]
             L.wtet_device_download.argtypes = [P, P, P, ctypes.c_size_t]
             L.wtet_device_free.argtypes = [P, P]
@@ -296,6 +300,35 @@ class Context:
         """Insertion independence radius in face-steps (0, 1 or 2)."""
         if lib().wtet_set_insertion_ring(self._h, int(ring)) != 0:
             raise ValueError("ring must be 0, 1 or 2")
+
+    def set_flip_graphs(self, on: bool):
+        """Replay the captured graph of a round's flip iterations (default) or
+        launch them directly."""
+        lib().wtet_set_flip_graphs(self._h, int(bool(on)))
+
+    def set_flip_grid(self, blocks: int):
+        if lib().wtet_set_flip_grid(self._h, int(blocks)) != 0:
+            raise ValueError(f"invalid grid {blocks}")
+
+    def flip_grid(self):
+        """(persistent grid in workgroups, graph captures so far)"""
+        b = np.zeros(2, np.uint32)
+        lib().wtet_get_flip_grid(self._h, _p(b[:1]), _p(b[1:]))
+        return int(b[0]), int(b[1])
+
+    def autotune(self, ptr: int, offsets, config: "Config | None" = None):
+        """Time the candidate persistent grids on these device points; keeps the
+        fastest.  Returns (grid, device ms)."""
+        off = _check_offsets(offsets)
+        self._prepare(config)
+        c = config._c() if config is not None else Config()._c()
+        g = np.zeros(1, np.uint32)
+        ms = np.zeros(1, np.float64)
+        st = lib().wtet_autotune(self._h, ctypes.c_void_p(ptr), _p(off), len(off) - 1, ctypes.byref(c),
+                                 _p(g), _p(ms))
+        if st != 0:
+            raise RuntimeError(f"wtet_autotune failed: {self.error()}")
+        return int(g[0]), float(ms[0])
 
     def set_profiling(self, on: bool = True):
         lib().wtet_set_profiling(self._h, int(on))

@@ -53,6 +53,14 @@ struct wtet_ctx {
   DevBuf presize_params, feat_acc, feat, pred;
   bool presize_loaded = false;
   std::vector<float> h_pred;
+  // device-driven flip iterations: sums (checked tets, flips, iterations),
+  // persistent grid, and the captured graph of one round's iterations
+  DevBuf isums, slow_t, slow_p;
+  uint32_t grid = 1024;
+  bool use_graphs = true;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gkey;
+  uint32_t graph_captures = 0;
   // counters
   DevBuf ctr, gstats, patch;
   uint32_t* h_ctr = nullptr;  // pinned mirror of the counters
@@ -84,7 +92,7 @@ struct wtet_ctx {
                    &fit,    &fround, &tmsg,   &tfwd,  &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
                    &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
-                   &presize_params, &feat_acc, &feat, &pred};
+                   &presize_params, &feat_acc, &feat, &pred, &isums, &slow_t, &slow_p};
     all.assign(std::begin(l), std::end(l));
   }
 };
@@ -160,6 +168,7 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
   ensure(c, c->tfwd, cap * 4, keep);
   ensure(c, c->act[0], cap * 4, keep);
   ensure(c, c->act[1], cap * 4, keep);
+  ensure(c, c->slow_t, cap * 4, false);
   ensure(c, c->prop, cap * 16, false);
   ensure(c, c->fout, cap * 16, false);
   ensure(c, c->slot2idx, cap * 4, false);
@@ -239,37 +248,75 @@ struct Counters {
   uint32_t rounds = 0, iters = 0;
 };
 
-// One flip iteration over the active list act[ac] (length A).
-// Returns the flips executed; A becomes the next list's length.
 void collect_times(wtet_ctx* c);
 
-uint32_t flip_iteration(wtet_ctx* c, uint32_t inf, int mode, uint32_t& ac, uint32_t& A,
-                        uint32_t& it, uint32_t round, Counters& K, uint64_t* st) {
-  if (c->evused > 4096) collect_times(c);
-  // worst case every proposal wins and takes one fresh slot (h_ctr holds the
-  // counters read at the end of the previous step)
-  uint64_t need = (uint64_t)c->h_ctr[C_TETS] + A + 64;
-  if (need > c->cap_tets) ensure_tets(c, need + need / 2, true);
+FlipLists lists_of(wtet_ctx* c) {
+  FlipLists L;
+  L.act[0] = P<uint32_t>(c->act[0]);
+  L.act[1] = P<uint32_t>(c->act[1]);
+  L.prop = P<uint4>(c->prop);
+  L.out = P<uint4>(c->fout);
+  L.slow = P<uint32_t>(c->slow_t);
+  return L;
+}
+
+void enqueue_flip_iterations(wtet_ctx* c, const Mesh& M, const FlipLists& L, uint64_t* st, int k) {
+  hipStream_t s = c->stream;
+  uint64_t* sums = P<uint64_t>(c->isums);
+  launch_iter_begin(M.ctr, s);
+  for (int i = 0; i < k; ++i) {
+    timed(c, K_FLIP_CHECK, [&] { launch_flip_check(M, L, c->grid, st, s); });
+    timed(c, K_FLIP_EXEC, [&] { launch_flip_decide_exec(M, L, c->grid, s); });
+    timed(c, K_FLIP_UPDATE, [&] { launch_flip_update(M, L, c->grid, s); });
+    if (i + 1 < k) launch_iter_step(M.ctr, sums, s);
+    else launch_iter_end(M.ctr, sums, s);
+  }
+}
+
+// k flip iterations without the host: replays a captured graph (captured
+// again whenever a buffer, the grid or k changed), or launches directly
+// when profiling (per-kernel events) or when graphs are off.
+void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k) {
   Mesh M = mesh_of(c, inf, mode);
-  ++it;
-  launch_set_ctr(M.ctr, C_ACT_NEXT, 0u, C_FLIPS, 0u, c->stream);
-  uint32_t* cur = P<uint32_t>(c->act[ac]);
-  uint32_t* nxt = P<uint32_t>(c->act[ac ^ 1]);
-  timed(c, K_FLIP_CHECK, [&] { launch_flip_check(M, cur, P<uint4>(c->prop), A, it, st, c->stream); });
-  timed(c, K_FLIP_EXEC, [&] {
-    launch_flip_decide_exec(M, P<uint4>(c->prop), A, it, round, P<uint4>(c->fout), nxt, c->stream);
-  });
-  timed(c, K_FLIP_UPDATE, [&] { launch_flip_update(M, P<uint4>(c->fout), A, it, c->stream); });
-  HIP_CHECK(hipGetLastError());
-  read_ctr(c);
-  uint32_t F = c->h_ctr[C_FLIPS];
-  K.sumA += A;
-  K.sumF += F;
-  K.peakA = std::max<uint64_t>(K.peakA, A);
-  K.iters++;
-  A = c->h_ctr[C_ACT_NEXT];
-  ac ^= 1;
-  return F;
+  FlipLists L = lists_of(c);
+  if (!c->use_graphs || c->profiling) {
+    if (c->evused > 4096) collect_times(c);
+    enqueue_flip_iterations(c, M, L, st, k);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
+  std::vector<uint64_t> key = {
+      (uint64_t)(uintptr_t)M.pts,   (uint64_t)(uintptr_t)M.tet,    (uint64_t)(uintptr_t)M.opp,
+      (uint64_t)(uintptr_t)M.tinfo, (uint64_t)(uintptr_t)M.tvote,  (uint64_t)(uintptr_t)M.fvote,
+      (uint64_t)(uintptr_t)M.tsplit, (uint64_t)(uintptr_t)M.fit,   (uint64_t)(uintptr_t)M.fround,
+      (uint64_t)(uintptr_t)M.tmsg,  (uint64_t)(uintptr_t)M.tfwd,   (uint64_t)(uintptr_t)M.ctr,
+      (uint64_t)M.inf,              (uint64_t)M.mode,              (uint64_t)(uintptr_t)L.act[0],
+      (uint64_t)(uintptr_t)L.act[1], (uint64_t)(uintptr_t)L.prop,  (uint64_t)(uintptr_t)L.out,
+      (uint64_t)(uintptr_t)L.slow,
+      (uint64_t)(uintptr_t)st,      (uint64_t)(uintptr_t)c->isums.p, (uint64_t)c->grid,
+      (uint64_t)k};
+  if (!c->gexec || key != c->gkey) {
+    if (c->gexec) {
+      HIP_CHECK(hipGraphExecDestroy(c->gexec));
+      c->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_flip_iterations(c, M, L, st, k);
+    HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+    hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_CHECK(e);
+    c->gkey = key;
+    c->graph_captures++;
+  }
+  HIP_CHECK(hipGraphLaunch(c->gexec, c->stream));
+}
+
+// tet slots after growth by `need`; the flip iterations read the capacity
+void grow_for(wtet_ctx* c, uint64_t need) {
+  if (need > c->cap_tets) ensure_tets(c, need + need / 2, true);
+  launch_set_ctr(P<uint32_t>(c->ctr), C_CAP, (uint32_t)c->cap_tets, C_STOP, 0u, c->stream);
 }
 
 // Morton bits per axis and the bit where the cloud index starts in a key
@@ -315,8 +362,10 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
     ensure(c, c->utet[b], (size_t)n * 4);
   }
   ensure(c, c->wlist, (size_t)n * 8);
+  ensure(c, c->slow_p, (size_t)n * 4 + 16);
   ensure(c, c->ctr, C_COUNT * 4);
   ensure(c, c->gstats, 64);
+  ensure(c, c->isums, 64);
   ensure(c, c->rec, (1u << 20) * 4);
   ensure(c, c->blk, ((size_t)nblocks(n) + 2) * 4);
   const bool learned = cfg.presize_mode == 1;
@@ -396,16 +445,18 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u * C, -1, 0u, s);
   HIP_CHECK(hipGetLastError());
 
-  uint32_t cur = 0, ac = 0, A = 0, it = 0, round = 0;
+  uint32_t cur = 0, round = 0;
   uint32_t max_rounds = std::max<uint32_t>(cfg.max_insert_iterations, 1u);
-  uint32_t max_fi = std::max<uint32_t>(cfg.max_flip_iterations, 1u);
+  int max_fi = (int)std::min<uint32_t>(std::max<uint32_t>(cfg.max_flip_iterations, 1u), 64u);
   bool flipping = cfg.enable_flipping != 0;
+  launch_fill_u64(P<uint64_t>(c->isums), 0ull, 4, s);
+  launch_set_ctr(P<uint32_t>(c->ctr), C_CAP, (uint32_t)c->cap_tets, -1, 0u, s);
   while (U > 0) {
     ++round;
     if (round > std::max<uint32_t>(max_rounds, 4096u))
       throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
     Mesh M = mesh_of(c, V, mode);
-    launch_set_ctr(M.ctr, C_ULEN, U, -1, 0u, s);
+    launch_set_ctr(M.ctr, C_ULEN, U, C_ROUND, round, s);
     timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
     timed(c, K_PICK, [&] {
       launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round,
@@ -413,50 +464,56 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
                   P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), c->ring, s);
     });
     HIP_CHECK(hipGetLastError());
+    // the one host read of a round: winners, reserved slots, and whether the
+    // previous round's flips stopped for capacity
     read_ctr(c);
     uint32_t W = c->h_ctr[C_WIN];
     if (c->h_ctr[C_BADTET]) throw Fail{WTET_ERR_INTERNAL, "a point was located outside the super tet"};
     if (W == 0) throw Fail{WTET_ERR_INTERNAL, "no insertion possible in a round"};
-    uint64_t need = (uint64_t)c->h_ctr[C_TETS] + 64;
-    if (need > c->cap_tets) {
-      ensure_tets(c, need + need / 2, true);
+    uint64_t need = (uint64_t)c->h_ctr[C_TETS] + 64 + (c->h_ctr[C_STOP] ? c->h_ctr[C_A] : 0u);
+    if (need > c->cap_tets || c->h_ctr[C_STOP]) {
+      grow_for(c, need);
       M = mesh_of(c, V, mode);
     }
     K.sumU += U;
     K.rounds++;
-    // splits append their children after the active entries carried over
-    launch_set_ctr(M.ctr, C_ACT_NEXT, A, -1, 0u, s);
-    timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), W, round, P<uint32_t>(c->act[ac]), s); });
+    timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), W, round, lists_of(c), s); });
     U -= W;
     cur ^= 1;
     timed(c, K_SPLIT_POINTS, [&] {
-      launch_split_points(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st, s);
+      launch_split_points(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st,
+                          P<uint32_t>(c->slow_p), s);
     });
     HIP_CHECK(hipGetLastError());
-    read_ctr(c);
-    A = c->h_ctr[C_ACT_NEXT];
     if (flipping) {
-      for (uint32_t k = 0; k < max_fi && A > 0; ++k)
-        if (flip_iteration(c, V, mode, ac, A, it, round, K, st) == 0) break;
+      run_flip_batch(c, V, mode, st, max_fi);
       if (U > 0) {
-        M = mesh_of(c, V, mode);
         timed(c, K_RELOCATE, [&] {
-          launch_relocate(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st, s);
+          launch_relocate(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st,
+                          P<uint32_t>(c->slow_p), s);
         });
         HIP_CHECK(hipGetLastError());
       }
     } else {
-      A = 0;
+      launch_set_ctr(M.ctr, C_A, 0u, -1, 0u, s);
     }
   }
-  // finish flipping
+  // finish flipping: batches of iterations until the active list is empty
   if (flipping) {
-    uint32_t guard = 0;
-    while (A > 0) {
-      if (++guard > 100000) throw Fail{WTET_ERR_NOT_CONVERGED, "flipping did not terminate"};
-      if (flip_iteration(c, V, mode, ac, A, it, round, K, st) == 0) break;
+    for (uint32_t guard = 0;; ++guard) {
+      if (guard > 20000) throw Fail{WTET_ERR_NOT_CONVERGED, "flipping did not terminate"};
+      read_ctr(c);
+      if (c->h_ctr[C_STOP]) grow_for(c, (uint64_t)c->h_ctr[C_TETS] + c->h_ctr[C_A] + 64);
+      else if (c->h_ctr[C_A] == 0) break;
+      run_flip_batch(c, V, mode, st, 16);
     }
   }
+  uint64_t sums[4] = {0, 0, 0, 0};
+  HIP_CHECK(hipMemcpyAsync(sums, c->isums.p, 32, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  K.sumA = sums[0];
+  K.sumF = sums[1];
+  K.iters = (uint32_t)sums[2];
   // audit every finite face
   read_ctr(c);
   uint32_t T = c->h_ctr[C_TETS];
@@ -493,6 +550,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->stats.ms_device = ms;
   c->stats.walk_failures = c->h_ctr[C_WALK_FAIL];
+  K.peakA = c->h_ctr[C_PEAKA];
   if (c->h_ctr[C_WALK_FAIL]) throw Fail{WTET_ERR_INTERNAL, "point relocation walk did not terminate"};
   uint32_t Mout = c->h_ctr[C_ALIVE];
   mout = Mout;
@@ -877,6 +935,7 @@ void wtet_destroy(wtet_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   for (DevBuf* b : c->all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
@@ -1413,6 +1472,55 @@ wtet_status wtet_device_copy(wtet_ctx* c, void* d_dst, const void* d_src, size_t
     c->err = "device copy failed";
     return WTET_ERR_DEVICE;
   }
+  return WTET_OK;
+}
+
+
+wtet_status wtet_set_flip_graphs(wtet_ctx* c, int on) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  c->use_graphs = on != 0;
+  return WTET_OK;
+}
+
+wtet_status wtet_set_flip_grid(wtet_ctx* c, uint32_t blocks) {
+  if (!c || blocks == 0 || blocks > 65536) return WTET_ERR_INVALID_ARGUMENT;
+  c->grid = blocks;
+  return WTET_OK;
+}
+
+wtet_status wtet_get_flip_grid(const wtet_ctx* c, uint32_t* blocks, uint32_t* graph_captures) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  if (blocks) *blocks = c->grid;
+  if (graph_captures) *graph_captures = c->graph_captures;
+  return WTET_OK;
+}
+
+wtet_status wtet_autotune(wtet_ctx* c, const float* d_xyz, const uint64_t* offsets,
+                          uint32_t num_clouds, const wtet_config* cfg, uint32_t* best_grid,
+                          double* best_ms) {
+  if (!c || !d_xyz || !offsets || num_clouds == 0) return WTET_ERR_INVALID_ARGUMENT;
+  static const uint32_t cand[] = {256, 512, 1024, 2048, 4096};
+  uint32_t keep = c->grid, best = keep;
+  double bms = 1e300;
+  for (uint32_t g : cand) {
+    c->grid = g;
+    wtet_batch_device_result r;
+    // the first call at a grid size captures its graph: time the second
+    for (int rep = 0; rep < 2; ++rep) {
+      wtet_status st = wtet_delaunay_batch_resident(c, d_xyz, offsets, num_clouds, cfg, &r);
+      if (st != WTET_OK) {
+        c->grid = keep;
+        return st;
+      }
+    }
+    if (c->stats.ms_device < bms) {
+      bms = c->stats.ms_device;
+      best = g;
+    }
+  }
+  c->grid = best;
+  if (best_grid) *best_grid = best;
+  if (best_ms) *best_ms = bms;
   return WTET_OK;
 }
 

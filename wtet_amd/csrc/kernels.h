@@ -44,18 +44,36 @@ void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uin
 void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                  uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
                  uint32_t* blk, int ring, hipStream_t s);
+// the two active lists (current = act[ctr[C_PAR]]), flip proposals and the
+// executed flips of one iteration
+struct FlipLists {
+  uint32_t* act[2];
+  uint4* prop;
+  uint4* out;
+  uint32_t* slow;  // entries the FP32-only check deferred (ctr[C_SLOW_T])
+};
+// appends the children to the current active list after its ctr[C_A]
+// entries and updates ctr[C_A]
 void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round,
-                  uint32_t* act_next, hipStream_t s);
+                  const FlipLists& L, hipStream_t s);
+// filtered mode: an FP32-only kernel, then the full predicates on the points
+// it deferred (slow, U entries of scratch); exact mode: one full kernel
 void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                         uint32_t round, uint64_t* st, hipStream_t s);
-void launch_flip_check(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t A,
-                       uint32_t it, uint64_t* st, hipStream_t s);
-void launch_flip_decide_exec(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it,
-                             uint32_t round, uint4* out, uint32_t* act_next, hipStream_t s);
-void launch_flip_update(const Mesh& M, const uint4* out, uint32_t F, uint32_t it,
-                        hipStream_t s);
+                         uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s);
+// one flip iteration = begin, check, decide+execute, update, end; counts,
+// tag, round and parity are read from ctr (device-driven, graph-capturable);
+// grid = persistent blocks of the three list kernels
+void launch_iter_begin(uint32_t* ctr, hipStream_t s);
+void launch_flip_check(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_t* st,
+                       hipStream_t s);
+void launch_flip_decide_exec(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s);
+void launch_flip_update(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s);
+// sums[0] += checked tets, sums[1] += flips, sums[2] += iterations
+void launch_iter_end(uint32_t* ctr, uint64_t* sums, hipStream_t s);
+// end of one iteration followed by the begin of the next
+void launch_iter_step(uint32_t* ctr, uint64_t* sums, hipStream_t s);
 void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                     uint32_t round, uint64_t* st, hipStream_t s);
+                     uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s);
 void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st,
                   hipStream_t s);
 

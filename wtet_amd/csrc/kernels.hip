@@ -301,7 +301,7 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
 // their keys with a wave min first (a few passes), so early rounds -- when
 // most points still share a few tets -- issue one atomic per wave instead of
 // one per point; the remaining lanes vote directly.
-__global__ void k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U) {
+__global__ void __launch_bounds__(kBlock) k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U) {
   uint32_t i = gid();
   uint64_t key = kVoteNone;
   uint32_t t = 0;
@@ -328,7 +328,7 @@ __global__ void k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint
   if (pending) at_min_u64(&M.tvote[t], key);
 }
 
-__global__ void k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t* blk, int ring) {
+__global__ void __launch_bounds__(kBlock) k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t* blk, int ring) {
   uint32_t i = gid();
   bool w = i < U && is_winner(M, utet, i, ring);
   uint32_t tot;
@@ -336,7 +336,7 @@ __global__ void k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t*
   if (threadIdx.x == 0) blk[blockIdx.x] = tot;
 }
 
-__global__ void k_pick_scatter(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
+__global__ void __launch_bounds__(kBlock) k_pick_scatter(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                                uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
                                const uint32_t* blk, int ring) {
   uint32_t i = gid();
@@ -376,38 +376,128 @@ void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uin
   if (ring > 0) k_clear_votes<<<nb, kBlock, 0, s>>>(M, utet2, M.ctr);
 }
 
-__global__ void k_split(Mesh M, const uint2* wlist, uint32_t W, uint32_t round,
-                        uint32_t* act_next) {
+__global__ void __launch_bounds__(kBlock) k_split(Mesh M, const uint2* wlist, uint32_t W, uint32_t round, FlipLists L) {
   uint32_t k = gid();
-  if (k < W) split_body(M, wlist, k, round, act_next);
+  if (k < W) split_body(M, wlist, k, round, L.act[M.ctr[C_PAR] & 1u]);
 }
 
-void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round,
-                  uint32_t* act_next, hipStream_t s) {
-  if (W) k_split<<<nblocks(W), kBlock, 0, s>>>(M, wlist, W, round, act_next);
+// ctr[dst] = ctr[src]
+__global__ void k_ctr_copy(uint32_t* ctr, int dst, int src) {
+  if (threadIdx.x == 0) ctr[dst] = ctr[src];
 }
 
-__global__ void k_split_points(Mesh M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                               uint32_t round, uint64_t* st) {
-  uint32_t i = gid();
+void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round, const FlipLists& L,
+                  hipStream_t s) {
+  // the children are appended after the active entries carried over
+  k_ctr_copy<<<1, 64, 0, s>>>(M.ctr, C_ACT_NEXT, C_A);
+  if (W) k_split<<<nblocks(W), kBlock, 0, s>>>(M, wlist, W, round, L);
+  k_ctr_copy<<<1, 64, 0, s>>>(M.ctr, C_A, C_ACT_NEXT);
+}
+
+// Point kernels in three forms: LEAN runs the FP32 filter alone and defers
+// the points it cannot decide to `slow` (ctr[C_SLOW_P]); SLOW runs the full
+// staged predicates on those; FULL runs them on every point (exact mode).
+enum KForm : int { LEAN = 0, FULL = 1, SLOW = 2 };
+
+template <int FORM>
+__global__ void __launch_bounds__(kBlock) k_split_points(Mesh M, const uint32_t* uvert,
+                                                         uint32_t* utet, uint32_t U, uint32_t round,
+                                                         uint64_t* st, uint32_t* slow) {
   PredStats ls = {};
-  if (i < U) split_points_body(M, uvert, utet, i, round, &ls);
+  if (FORM == SLOW) {
+    uint32_t n = M.ctr[C_SLOW_P], stride = gridDim.x * blockDim.x;
+    for (uint32_t k = gid(); k < n; k += stride)
+      (void)split_points_body_t<false>(M, uvert, utet, slow[k], round, &ls);
+  } else {
+    uint32_t i = gid();
+    bool defer = false;
+    if (i < U) defer = split_points_body_t<FORM == LEAN>(M, uvert, utet, i, round, &ls);
+    if (defer) slow[append_slot(&M.ctr[C_SLOW_P])] = i;
+  }
   flush_stats(ls, st);
 }
 
 void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                         uint32_t round, uint64_t* st, hipStream_t s) {
-  if (U) k_split_points<<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st);
+                         uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s) {
+  if (!U) return;
+  if (M.mode == PRED_FILTERED) {
+    launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
+    k_split_points<LEAN><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_split_points<SLOW><<<std::min(nblocks(U), 1024u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+  } else {
+    k_split_points<FULL><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // flips
 // ---------------------------------------------------------------------------
-__global__ void k_flip_check(Mesh M, const uint32_t* act, uint4* prop, uint32_t A, uint32_t it,
-                             uint64_t* st) {
-  uint32_t j = gid();
+// ---------------------------------------------------------------------------
+// flip iterations, driven from the device: every count, the iteration tag,
+// the round and the list parity are read from ctr, so a sequence of
+// iterations runs (and can be captured in a graph) without the host.  The
+// kernels are persistent: a fixed grid strides over the current list.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void iter_begin(uint32_t* ctr) {
+  uint32_t A = ctr[C_A];
+  uint32_t run = 0u;
+  if (!ctr[C_STOP] && A > 0u) {
+    // worst case every proposal wins a 2-3 flip and takes a fresh slot
+    if ((uint64_t)ctr[C_TETS] + A + 64u > (uint64_t)ctr[C_CAP]) ctr[C_STOP] = 1u;
+    else run = 1u;
+  }
+  ctr[C_RUN] = run;
+  if (run) {
+    ctr[C_IT] += 1u;
+    ctr[C_ACT_NEXT] = 0u;
+    ctr[C_FLIPS] = 0u;
+    ctr[C_SLOW_T] = 0u;
+    if (A > ctr[C_PEAKA]) ctr[C_PEAKA] = A;
+  }
+}
+
+__global__ void k_iter_begin(uint32_t* ctr) {
+  if (threadIdx.x == 0) iter_begin(ctr);
+}
+
+__device__ __forceinline__ void iter_end(uint32_t* ctr, unsigned long long* sums) {
+  if (!ctr[C_RUN]) return;
+  sums[0] += ctr[C_A];
+  sums[1] += ctr[C_FLIPS];
+  sums[2] += 1ull;
+  ctr[C_A] = ctr[C_ACT_NEXT];
+  ctr[C_PAR] ^= 1u;
+}
+
+__global__ void k_iter_end(uint32_t* ctr, unsigned long long* sums) {
+  if (threadIdx.x == 0) iter_end(ctr, sums);
+}
+
+// end of one iteration and begin of the next in one launch
+__global__ void k_iter_step(uint32_t* ctr, unsigned long long* sums) {
+  if (threadIdx.x) return;
+  iter_end(ctr, sums);
+  iter_begin(ctr);
+}
+
+template <int FORM>
+__global__ void __launch_bounds__(kBlock) k_flip_check(Mesh M, FlipLists L, uint64_t* st) {
+  if (!M.ctr[C_RUN]) return;
+  uint32_t it = M.ctr[C_IT];
+  const uint32_t* cur = L.act[M.ctr[C_PAR] & 1u];
+  uint32_t stride = gridDim.x * blockDim.x;
   PredStats ls = {};
-  if (j < A) flip_check_body(M, act, prop, j, it, &ls);
+  if (FORM == SLOW) {
+    uint32_t n = M.ctr[C_SLOW_T];
+    for (uint32_t k = gid(); k < n; k += stride)
+      (void)flip_check_body_t<false>(M, cur, L.prop, L.slow[k], it, &ls);
+  } else {
+    uint32_t A = M.ctr[C_A];
+    for (uint32_t j = gid(); j < A; j += stride) {
+      bool defer = flip_check_body_t<FORM == LEAN>(M, cur, L.prop, j, it, &ls);
+      if (defer) L.slow[append_slot(&M.ctr[C_SLOW_T])] = j;
+    }
+  }
   flush_stats(ls, st);
 }
 
@@ -435,44 +525,47 @@ __device__ __forceinline__ uint32_t block_prefix_small(uint32_t v, uint32_t* tot
 // one atomic per counter (flips, fresh slots, next active entries), then
 // execute the winners.  Counters share a cache line, so per-wave atomics
 // from every wave serialize; per-block claims keep them few.
-__global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, const uint4* prop, uint32_t A,
-                                                             uint32_t it, uint32_t round, uint4* out,
-                                                             uint32_t* act_next) {
+__global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L) {
   __shared__ uint32_t wsum[kBlock / 64];
   __shared__ uint32_t base[3];
-  uint32_t j = gid();
-  uint4 pr = make_uint4(kInvalid, 0u, 0u, 0u);
-  if (j < A) pr = prop[j];
-  uint32_t qid[2], nq = 0, nslot = 0;
-  bool win = flip_decide_body(M, pr, it, qid, &nq, &nslot);
-  uint32_t nqa = nq + nslot;  // the fresh slot is queued too
-  uint32_t tf, ts, tq;
-  uint32_t rf = block_prefix_small(win ? 1u : 0u, &tf, wsum);
-  uint32_t rs = block_prefix_small(nslot, &ts, wsum);
-  uint32_t rq = block_prefix_small(nqa, &tq, wsum);
-  if (threadIdx.x == 0) {
-    base[0] = tf ? atomicAdd(&M.ctr[C_FLIPS], tf) : 0u;
-    base[1] = ts ? atomicAdd(&M.ctr[C_TETS], ts) : 0u;
-    base[2] = tq ? atomicAdd(&M.ctr[C_ACT_NEXT], tq) : 0u;
-  }
-  __syncthreads();
-  uint32_t q = base[2] + rq;
-  for (uint32_t k = 0; k < nq; ++k) act_next[q + k] = qid[k];
-  if (win) {
-    uint32_t s = nslot ? base[1] + rs : kInvalid;
-    flip_exec_body(M, pr, it, round, s, &out[base[0] + rf]);
-    if (nslot) act_next[q + nq] = s;
+  if (!M.ctr[C_RUN]) return;
+  uint32_t A = M.ctr[C_A], it = M.ctr[C_IT], round = M.ctr[C_ROUND];
+  uint32_t* act_next = L.act[(M.ctr[C_PAR] & 1u) ^ 1u];
+  for (uint32_t b0 = blockIdx.x * kBlock; b0 < A; b0 += gridDim.x * kBlock) {
+    uint32_t j = b0 + threadIdx.x;
+    uint4 pr = make_uint4(kInvalid, 0u, 0u, 0u);
+    if (j < A) pr = L.prop[j];
+    uint32_t qid[2], nq = 0, nslot = 0;
+    bool win = flip_decide_body(M, pr, it, qid, &nq, &nslot);
+    uint32_t nqa = nq + nslot;  // the fresh slot is queued too
+    uint32_t tf, ts, tq;
+    uint32_t rf = block_prefix_small(win ? 1u : 0u, &tf, wsum);
+    uint32_t rs = block_prefix_small(nslot, &ts, wsum);
+    uint32_t rq = block_prefix_small(nqa, &tq, wsum);
+    if (threadIdx.x == 0) {
+      base[0] = tf ? atomicAdd(&M.ctr[C_FLIPS], tf) : 0u;
+      base[1] = ts ? atomicAdd(&M.ctr[C_TETS], ts) : 0u;
+      base[2] = tq ? atomicAdd(&M.ctr[C_ACT_NEXT], tq) : 0u;
+    }
+    __syncthreads();
+    uint32_t q = base[2] + rq;
+    for (uint32_t k = 0; k < nq; ++k) act_next[q + k] = qid[k];
+    if (win) {
+      uint32_t sl = nslot ? base[1] + rs : kInvalid;
+      flip_exec_body(M, pr, it, round, sl, &L.out[base[0] + rf]);
+      if (nslot) act_next[q + nq] = sl;
+    }
+    // the next tile's prefix sums synchronize before base[] is rewritten
   }
 }
 
-// F is an upper bound (the active-list length); the flip count itself is
-// read from the device counter, so the host does not wait for it.
-__global__ void k_flip_update(Mesh M, const uint4* out, uint32_t F, uint32_t it) {
-  uint32_t k = gid();
-  if (k < F && k < M.ctr[C_FLIPS]) flip_update_body(M, out, k, it);
+__global__ void __launch_bounds__(kBlock) k_flip_update(Mesh M, FlipLists L) {
+  if (!M.ctr[C_RUN]) return;
+  uint32_t F = M.ctr[C_FLIPS], it = M.ctr[C_IT];
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t k = gid(); k < F; k += stride) flip_update_body(M, L.out, k, it);
 }
 
-// ctr[idx] = v, as a kernel so that a captured graph never records a memset
 __global__ void k_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2) {
   if (threadIdx.x == 0) {
     ctr[idx] = v;
@@ -484,33 +577,59 @@ void launch_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2, h
   k_set_ctr<<<1, 64, 0, s>>>(ctr, idx, v, idx2, v2);
 }
 
-void launch_flip_check(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t A, uint32_t it,
-                       uint64_t* st, hipStream_t s) {
-  if (A) k_flip_check<<<nblocks(A), kBlock, 0, s>>>(M, act, prop, A, it, st);
+void launch_iter_begin(uint32_t* ctr, hipStream_t s) { k_iter_begin<<<1, 64, 0, s>>>(ctr); }
+void launch_iter_end(uint32_t* ctr, uint64_t* sums, hipStream_t s) {
+  k_iter_end<<<1, 64, 0, s>>>(ctr, reinterpret_cast<unsigned long long*>(sums));
 }
-void launch_flip_decide_exec(const Mesh& M, const uint4* prop, uint32_t A, uint32_t it,
-                             uint32_t round, uint4* out, uint32_t* act_next, hipStream_t s) {
-  if (A) k_flip_decide_exec<<<nblocks(A), kBlock, 0, s>>>(M, prop, A, it, round, out, act_next);
+void launch_iter_step(uint32_t* ctr, uint64_t* sums, hipStream_t s) {
+  k_iter_step<<<1, 64, 0, s>>>(ctr, reinterpret_cast<unsigned long long*>(sums));
 }
-void launch_flip_update(const Mesh& M, const uint4* out, uint32_t F, uint32_t it,
-                        hipStream_t s) {
-  if (F) k_flip_update<<<nblocks(F), kBlock, 0, s>>>(M, out, F, it);
+void launch_flip_check(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_t* st, hipStream_t s) {
+  if (M.mode == PRED_FILTERED) {
+    k_flip_check<LEAN><<<grid, kBlock, 0, s>>>(M, L, st);
+    k_flip_check<SLOW><<<grid, kBlock, 0, s>>>(M, L, st);
+  } else {
+    k_flip_check<FULL><<<grid, kBlock, 0, s>>>(M, L, st);
+  }
+}
+void launch_flip_decide_exec(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s) {
+  k_flip_decide_exec<<<grid, kBlock, 0, s>>>(M, L);
+}
+void launch_flip_update(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s) {
+  k_flip_update<<<grid, kBlock, 0, s>>>(M, L);
 }
 
-__global__ void k_relocate(Mesh M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                           uint32_t round, uint64_t* st) {
-  uint32_t i = gid();
+template <int FORM>
+__global__ void __launch_bounds__(kBlock) k_relocate(Mesh M, const uint32_t* uvert, uint32_t* utet,
+                                                     uint32_t U, uint32_t round, uint64_t* st,
+                                                     uint32_t* slow) {
   PredStats ls = {};
-  if (i < U) relocate_body(M, uvert, utet, i, round, &ls);
+  if (FORM == SLOW) {
+    uint32_t n = M.ctr[C_SLOW_P], stride = gridDim.x * blockDim.x;
+    for (uint32_t k = gid(); k < n; k += stride)
+      (void)relocate_body_t<false>(M, uvert, utet, slow[k], round, &ls);
+  } else {
+    uint32_t i = gid();
+    bool defer = false;
+    if (i < U) defer = relocate_body_t<FORM == LEAN>(M, uvert, utet, i, round, &ls);
+    if (defer) slow[append_slot(&M.ctr[C_SLOW_P])] = i;
+  }
   flush_stats(ls, st);
 }
 
 void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                     uint32_t round, uint64_t* st, hipStream_t s) {
-  if (U) k_relocate<<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st);
+                     uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s) {
+  if (!U) return;
+  if (M.mode == PRED_FILTERED) {
+    launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
+    k_relocate<LEAN><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_relocate<SLOW><<<std::min(nblocks(U), 1024u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+  } else {
+    k_relocate<FULL><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+  }
 }
 
-__global__ void k_audit(Mesh M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st) {
+__global__ void __launch_bounds__(kBlock) k_audit(Mesh M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st) {
   uint32_t t = gid();
   PredStats ls = {};
   if (t < T) audit_body(M, t, rec, cap, &ls);

@@ -28,7 +28,18 @@ enum Ctr : int {
   C_DUPSCAN = 11,  // duplicate scans that hit their length cap
   C_ULEN = 12,     // uninserted list length at the start of the round
   C_SURV = 13,     // points left after this round's insertions
-  C_COUNT = 16
+  // device-driven flip iterations (the host sets C_ROUND and C_CAP)
+  C_A = 14,        // length of the current active list
+  C_IT = 15,       // flip iteration tag (fvote keys, fit stamps)
+  C_PAR = 16,      // which of the two active lists is current
+  C_RUN = 17,      // 1 while this iteration runs
+  C_STOP = 18,     // set when an iteration could overflow the tet slots
+  C_ROUND = 19,    // insertion round (fround stamps)
+  C_CAP = 20,      // tet slot capacity
+  C_PEAKA = 21,    // longest active list seen
+  C_SLOW_T = 22,   // tets deferred by the lean flip check this iteration
+  C_SLOW_P = 23,   // points deferred by the lean point kernels
+  C_COUNT = 32
 };
 
 struct Mesh {
@@ -70,6 +81,36 @@ WT_HD int o3(const Mesh& M, uint32_t a, uint32_t b, uint32_t c, uint32_t d, Pred
   load_pt(M, c, C, &rc);
   load_pt(M, d, D, &rd);
   return orient3d_sos(A, B, C, D, ra, rb, rc, rd, M.mode, st);
+}
+
+// Predicates of the lean (FAST) and full bodies.  FAST runs the FP32 filter
+// alone and returns 2 when it cannot decide; the body then stops with no
+// side effect and reports the element as deferred, and a second kernel
+// runs the full staged predicates on the deferred elements only.
+template <bool FAST>
+WT_HD int o3s(const Mesh& M, const float* a, const float* b, const float* c, const float* d,
+              uint32_t ra, uint32_t rb, uint32_t rc, uint32_t rd, PredStats* st) {
+  if (FAST) return orient3d_sos_fast(a, b, c, d, st);
+  return orient3d_sos(a, b, c, d, ra, rb, rc, rd, M.mode, st);
+}
+
+template <bool FAST>
+WT_HD int o4s(const Mesh& M, const float* a, const float* b, const float* c, const float* d,
+              const float* e, uint32_t ra, uint32_t rb, uint32_t rc, uint32_t rd, uint32_t re,
+              PredStats* st) {
+  if (FAST) return orient4d_fast(a, b, c, d, e, st);
+  return orient4d(a, b, c, d, e, ra, rb, rc, rd, re, M.mode, st);
+}
+
+template <bool FAST>
+WT_HD int o3v(const Mesh& M, uint32_t a, uint32_t b, uint32_t c, uint32_t d, PredStats* st) {
+  float A[3], B[3], C[3], D[3];
+  uint32_t ra, rb, rc, rd;
+  load_pt(M, a, A, &ra);
+  load_pt(M, b, B, &rb);
+  load_pt(M, c, C, &rc);
+  load_pt(M, d, D, &rd);
+  return o3s<FAST>(M, A, B, C, D, ra, rb, rc, rd, st);
 }
 
 WT_HD uint32_t* opp_word(const Mesh& M, uint32_t t, uint32_t f) {
@@ -187,26 +228,38 @@ WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t ro
 // move an uninserted point of a split tet into its child
 // (src/shaders/split_points.wgsl:222-335): three orient3d_sos tests.
 // ---------------------------------------------------------------------------
-WT_HD void split_points_body(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
-                             uint32_t round, PredStats* st) {
+template <bool FAST>
+WT_HD bool split_points_body_t(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
+                               uint32_t round, PredStats* st) {
   uint32_t t = utet[i];
   uint2 sp = M.tsplit[t];
-  if (sp.x != round) return;
+  if (sp.x != round) return false;
   uint32_t base = sp.y;
   uint4 c0 = M.tet[t];      // (p, v1, v2, v3)
   uint32_t v0 = M.tet[base].x;  // child 1 = (v0, p, v2, v3)
   uint32_t p = c0.x, v1 = c0.y, v2 = c0.z, v3 = c0.w;
   uint32_t q = uvert[i];
   // children 0 | 1 across plane (p, v2, v3): +1 -> v0's side -> not child 0
-  int a = o3(M, p, v3, v2, q, st) > 0 ? 1 : 0;
+  int sa = o3v<FAST>(M, p, v3, v2, q, st);
+  if (sa == 2) return true;
+  int a = sa > 0 ? 1 : 0;
   // children 2 | 3 across plane (v0, p, v1): +1 -> v2's side -> not child 2
-  int b = o3(M, v0, p, v1, q, st) > 0 ? 3 : 2;
+  int sb = o3v<FAST>(M, v0, p, v1, q, st);
+  if (sb == 2) return true;
+  int b = sb > 0 ? 3 : 2;
   // decide between a and b on their shared face (child a's face b)
   uint32_t vv[4] = {v0, v1, v2, v3};
   vv[a] = p;
-  int s = o3(M, vv[face_vi(b, 0)], vv[face_vi(b, 1)], vv[face_vi(b, 2)], q, st);
+  int s = o3v<FAST>(M, vv[face_vi(b, 0)], vv[face_vi(b, 1)], vv[face_vi(b, 2)], q, st);
+  if (s == 2) return true;
   int child = s > 0 ? a : b;
   utet[i] = child_slot(t, base, (uint32_t)child);
+  return false;
+}
+
+WT_HD void split_points_body(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
+                             uint32_t round, PredStats* st) {
+  (void)split_points_body_t<false>(M, uvert, utet, i, round, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -219,14 +272,18 @@ WT_HD uint64_t flip_key(uint32_t it, uint32_t t) {
   return ((uint64_t)(0xFFFFFFFFu - it) << 32) | t;
 }
 
-WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t j,
-                           uint32_t it, PredStats* st) {
+template <bool FAST>
+WT_HD bool flip_check_body_t(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t j,
+                             uint32_t it, PredStats* st) {
   uint32_t t = act[j];
-  uint32_t info = at_and_u32(&M.tinfo[t], ~kQueued);
+  // each tet is on the list once and no other thread of this kernel writes
+  // its info word: a plain read-modify-write clears the queued bit
+  uint32_t info = M.tinfo[t];
+  M.tinfo[t] = info & ~kQueued;
   prop[j].x = kInvalid;
-  if (!(info & kAlive)) return;
+  if (!(info & kAlive)) return false;
   uint4 tv = M.tet[t];
-  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return;
+  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return false;
   uint4 to = M.opp[t];
   float P[4][3];
   uint32_t R[4];
@@ -239,8 +296,9 @@ WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint
     float O[3];
     uint32_t ro;
     load_pt(M, o, O, &ro);
-    if (orient4d(P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, M.mode, st) > 0)
-      continue;
+    int ins = o4s<FAST>(M, P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, st);
+    if (ins == 2) return true;
+    if (ins > 0) continue;
     // o lies inside the circumsphere: classify the flip
     int px = face_vi(f, 0), py = face_vi(f, 1), pz = face_vi(f, 2);
     int pos[3] = {px, py, pz};
@@ -248,7 +306,8 @@ WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint
     for (int k = 0; k < 3; ++k) {
       int u = pos[k], w = pos[(k + 1) % 3];
       // edge (u, w): the line a-o should pass on the face side -> raw > 0
-      int s = orient3d_sos(P[u], P[w], P[f], O, R[u], R[w], R[f], ro, M.mode, st);
+      int s = o3s<FAST>(M, P[u], P[w], P[f], O, R[u], R[w], R[f], ro, st);
+      if (s == 2) return true;
       if (s > 0) {
         bad = k;
         ++nbad;
@@ -274,8 +333,14 @@ WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint
     at_min_u64(&M.fvote[nt], key);
     if (kind == 1) at_min_u64(&M.fvote[third], key);
     prop[j] = make_uint4(t, (uint32_t)f | kind << 2 | (uint32_t)(kind ? bad : 0) << 3, nt, third);
-    return;
+    return false;
   }
+  return false;
+}
+
+WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t j,
+                           uint32_t it, PredStats* st) {
+  (void)flip_check_body_t<false>(M, act, prop, j, it, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,8 +509,10 @@ WT_HD void flip_update_body(const Mesh& M, const uint4* out, uint32_t k, uint32_
 // ---------------------------------------------------------------------------
 constexpr uint32_t kWalkLimit = 1u << 16;
 
-WT_HD uint32_t walk_to(const Mesh& M, uint32_t t, uint32_t q, uint32_t seed, PredStats* st,
-                       bool* ok) {
+// *state: 0 arrived, 1 step limit, 2 deferred (FAST filter undecided)
+template <bool FAST>
+WT_HD uint32_t walk_to_t(const Mesh& M, uint32_t t, uint32_t q, uint32_t seed, PredStats* st,
+                         int* state) {
   float Q[3];
   uint32_t rq;
   load_pt(M, q, Q, &rq);
@@ -462,30 +529,43 @@ WT_HD uint32_t walk_to(const Mesh& M, uint32_t t, uint32_t q, uint32_t seed, Pre
       load_pt(M, x, X, &rx);
       load_pt(M, y, Y, &ry);
       load_pt(M, z, Z, &rz);
-      if (orient3d_sos(X, Y, Z, Q, rx, ry, rz, rq, M.mode, st) < 0) {
+      int o = o3s<FAST>(M, X, Y, Z, Q, rx, ry, rz, rq, st);
+      if (o == 2) {
+        *state = 2;
+        return t;
+      }
+      if (o < 0) {
         t = opp_tet(get4(to, f));
         moved = true;
         break;
       }
     }
     if (!moved) {
-      *ok = true;
+      *state = 0;
       return t;
     }
   }
-  *ok = false;
+  *state = 1;
   return t;
+}
+
+template <bool FAST>
+WT_HD bool relocate_body_t(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
+                           uint32_t round, PredStats* st) {
+  uint32_t t = utet[i];
+  if (M.fround[t] != round) return false;
+  for (int guard = 0; guard < 64 && (M.tinfo[t] & kDead); ++guard) t = M.tfwd[t];
+  int state = 0;
+  t = walk_to_t<FAST>(M, t, uvert[i], i * 0x85EBCA6Bu, st, &state);
+  if (state == 2) return true;  // the full walk restarts from utet[i]
+  if (state == 1) at_add_u32(&M.ctr[C_WALK_FAIL], 1u);
+  utet[i] = t;
+  return false;
 }
 
 WT_HD void relocate_body(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
                          uint32_t round, PredStats* st) {
-  uint32_t t = utet[i];
-  if (M.fround[t] != round) return;
-  for (int guard = 0; guard < 64 && (M.tinfo[t] & kDead); ++guard) t = M.tfwd[t];
-  bool ok = false;
-  t = walk_to(M, t, uvert[i], i * 0x85EBCA6Bu, st, &ok);
-  if (!ok) at_add_u32(&M.ctr[C_WALK_FAIL], 1u);
-  utet[i] = t;
+  (void)relocate_body_t<false>(M, uvert, utet, i, round, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -553,6 +553,17 @@ WT_HD int orient3d_sos(const float* a, const float* b, const float* c, const flo
   return orient3d_sos_only(a, b, c, d, ia, ib, ic, id);
 }
 
+// FP32 filter stage alone, for the lean kernels: the staged functions
+// sign conventions, or 2 when the filter cannot decide (the caller then
+// defers the whole element to a kernel that runs the full stages).
+WT_HD int orient3d_sos_fast(const float* a, const float* b, const float* c, const float* d,
+                            PredStats* st) {
+  int s = orient3d_f32(a, b, c, d);
+  if (s == 2) return 2;
+  stat_inc(st, 0);
+  return s < 0 ? 1 : -1;
+}
+
 // The insphere cascade as data: kind (3: 4-point, 2: 3-point, 1: 2-point,
 // 0: constant +1), lifted, coordinate order (0 xyz, 1 xzy, 2 yzx, 3 yxz,
 // 4 zxy), ranks of the points used.  Depths 1..49 of src/predicates.rs:513-652.
@@ -655,6 +666,14 @@ WT_HD int orient4d(const float* a, const float* b, const float* c, const float* 
   int s = insphere_raw(a, b, c, d, e, mode, st);
   if (s != 0) return s;
   return orient4d_sos_only(a, b, c, d, e, ia, ib, ic, id, ie);
+}
+
+WT_HD int orient4d_fast(const float* a, const float* b, const float* c, const float* d,
+                        const float* e, PredStats* st) {
+  int s = insphere_f32(a, b, c, d, e);
+  if (s == 2) return 2;
+  stat_inc(st, 4);
+  return s;
 }
 
 }  // namespace wtet
