@@ -25,8 +25,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # algorithmic bytes per unit (minimal SoA traffic):
-#   flip_check: per active tet checked -- own quad + adjacency 32, four
-#   neighbour quads 64, eight vertex xyz 96, vote atomic 8
+#   flip_check (the FP32-only check kernel): per active tet checked -- own
+#   quad + adjacency 32, four neighbour quads 64, eight vertex xyz 96, vote
+#   atomic 8
 BYTES_PER_UNIT = {"flip_check": 200}
 
 
@@ -52,6 +53,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-cloud batched leg")
     ap.add_argument("--no-stress", action="store_true", help="skip the lattice / sphere leg")
     ap.add_argument("--no-presize", action="store_true", help="skip the learned-presizing leg")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the flip iterations directly instead of replaying HIP graphs "
+                         "(the same kernels; used under rocprofv3)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (5 launches), "
@@ -84,6 +88,7 @@ def main():
     ctx = wtet_amd.Context(0)
     d_pts = ctx.upload(pts)  # resident in HBM before any timed region
     ctx.set_insertion_ring(args.ring)
+    ctx.set_flip_graphs(not args.no_graphs)
     filt = wtet_amd.Config(predicate_mode="filtered")
     exact = wtet_amd.Config(predicate_mode="exact")
 
@@ -105,6 +110,8 @@ def main():
             ctx.delaunay_resident(d_pts, args.n, cfg)
         return (time.perf_counter() - t0) * 1e3 / max(k, 1)
 
+    # autotuned persistent grid sizing (entries per thread), outside the timed region
+    items, _ = ctx.autotune(d_pts, [0, args.n], filt)
     run(filt, args.warmup)
     ms = run(filt, args.steps)
     value = args.n / (ms / 1e3)
@@ -130,7 +137,7 @@ def main():
         traffic = int(per_tet * st_p["sum_active"] / max(launches, 1))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_flip_check", "launches": launches,
+                "kernel": "k_flip_check<LEAN> (FP32-only flip check)", "launches": launches,
                 "avg_launch_ms": round(tot_ms / max(launches, 1), 4),
                 "bytes_per_launch_avg": int(bytes_dom / max(launches, 1))}
 
@@ -148,6 +155,7 @@ def main():
         bp, boff = W.mixed_batch(4096, 4)
         d_b = ctx.upload(bp)
         nb = len(bp)
+        items_b, _ = ctx.autotune(d_b, boff, filt)
         ctx.delaunay_batch_resident(d_b, boff, filt)
         t0 = time.perf_counter()
         for _ in range(args.batch_steps):
@@ -177,6 +185,7 @@ def main():
                  "single_instance_plain_exact": round(single_plain, 1),
                  "single_instance_sample": "clouds 0..255, one resident call each",
                  "speedup_vs_single_plain": round(nb / (ms_b / 1e3) / single_plain, 2),
+                 "autotuned_items_per_thread": items_b,
                  "ms_device": round(bst["ms_device"], 3), "ms_repair": round(bst["ms_repair"], 3),
                  "violations_gpu": bst["violations_gpu"], "rounds": bst["rounds"],
                  "flip_iterations": bst["flip_iterations"]}
@@ -262,7 +271,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"uniform_{args.n}_seed{args.seed}", "points": args.n,
                    "distribution": "i.i.d. U[0,1)^3 f32, numpy PCG64", "predicates": "f32->f64->exact",
-                   "batched": False, "insertion_ring": args.ring},
+                   "batched": False, "insertion_ring": args.ring, "autotuned_items_per_thread": items,
+                   "flip_graphs": not args.no_graphs},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_exact": plain,

@@ -208,19 +208,20 @@ wtet_status wtet_set_insertion_ring(wtet_ctx *ctx, int ring);
 
 /* The flip iterations of a round run on the device without host reads
  * (counts, iteration tag and list parity live in device counters); they
- * are captured once in a HIP graph and replayed every round (on by
- * default; off: direct launches).  The three list kernels are persistent
- * grid-stride kernels of `blocks` workgroups (default 1024).  No
- * counterpart in the reference, which reads counters back every iteration
- * (src/phase1.rs:16-1431). */
+ * are captured in HIP graphs and replayed every round (on by default; off:
+ * direct launches).  The list kernels are persistent grid-stride kernels
+ * whose grid is sized per round from the host's bound on the first list,
+ * for `items` entries per thread (default 2; powers of two from 16 to 4096
+ * workgroups, one graph per size).  No counterpart in the reference, which
+ * reads counters back every iteration (src/phase1.rs:16-1431). */
 wtet_status wtet_set_flip_graphs(wtet_ctx *ctx, int on);
-wtet_status wtet_set_flip_grid(wtet_ctx *ctx, uint32_t blocks);
-wtet_status wtet_get_flip_grid(const wtet_ctx *ctx, uint32_t *blocks, uint32_t *graph_captures);
-/* Autotuner: triangulates the given clouds (device points) at each
- * candidate grid (256 .. 4096 workgroups), keeps the fastest device time
- * for this context and reports it. */
+wtet_status wtet_set_flip_items(wtet_ctx *ctx, uint32_t items);
+wtet_status wtet_get_flip_items(const wtet_ctx *ctx, uint32_t *items, uint32_t *graph_captures);
+/* Autotuner: triangulates the given clouds (device points) with 1, 2, 4
+ * and 8 entries per thread, keeps the fastest device time for this
+ * context and reports it. */
 wtet_status wtet_autotune(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
-                          uint32_t num_clouds, const wtet_config *cfg, uint32_t *best_grid,
+                          uint32_t num_clouds, const wtet_config *cfg, uint32_t *best_items,
                           double *best_ms);
 
 /* Learned presizing (no counterpart in the reference; the reference sizes
@@ -264,8 +265,10 @@ wtet_status wtet_mlp_train(wtet_ctx *ctx, float *params, const float *feat, cons
 /* Per-kernel-group timing with HIP events on the context's stream (the
  * reference's GpuProfiler/CpuProfiler scopes, src/profiler.rs:5-224).
  * wtet_set_profiling resets the totals; kinds 0..11 are prepare_points,
- * vote, pick, split, split_points, flip_check, (unused), flip_decide_exec,
- * flip_update, relocate, audit, output.  Totals accumulate over calls. */
+ * vote, pick, split, split_points, flip_check (the FP32-only check),
+ * flip_check_deferred (full predicates on the tets it deferred),
+ * flip_decide_exec, flip_update, relocate, audit, output.  Totals
+ * accumulate over calls. */
 wtet_status wtet_set_profiling(wtet_ctx *ctx, int on);
 wtet_status wtet_kernel_times(const wtet_ctx *ctx, uint32_t kind, const char **name,
                               uint32_t *launches, double *total_ms);

@@ -208,35 +208,35 @@ def test_flip_graph_replays_match_direct_launches(oracle, ctx):
     finally:
         ctx.set_flip_graphs(True)
     assert np.array_equal(oracle.canonical_real(direct.tets, len(p)), ref)
-    _, before = ctx.flip_grid()
+    _, before = ctx.flip_items()
     small = W.uniform(7000, 62)
     small_ref = oracle.canonical_real(oracle.delaunay(oracle.normalize_points(small)[0]), len(small))
     for q, want in ((p, ref), (small, small_ref), (p, ref)):  # the smaller call recaptures
         r = ctx.delaunay(q)
         assert np.array_equal(oracle.canonical_real(r.tets, len(q)), want)
-    _, after = ctx.flip_grid()
+    _, after = ctx.flip_items()
     assert after > before
 
 
-@pytest.mark.parametrize("blocks", [1, 64, 4096])
-def test_flip_grid_sizes(oracle, ctx, blocks):
-    g0, _ = ctx.flip_grid()
-    ctx.set_flip_grid(blocks)
+@pytest.mark.parametrize("items", [1, 8, 64])
+def test_flip_grid_sizes(oracle, ctx, items):
+    i0, _ = ctx.flip_items()
+    ctx.set_flip_items(items)
     try:
         _check(oracle, ctx, W.uniform(20000, 63))
     finally:
-        ctx.set_flip_grid(g0)
+        ctx.set_flip_items(i0)
 
 
 def test_autotune_picks_a_candidate(oracle, ctx):
     p = W.uniform(100000, 64)
     d = ctx.upload(p)
-    g0, _ = ctx.flip_grid()
+    i0, _ = ctx.flip_items()
     try:
         g, ms = ctx.autotune(d, [0, len(p)])
-        assert g in (256, 512, 1024, 2048, 4096) and ms > 0
-        assert ctx.flip_grid()[0] == g
+        assert g in (1, 2, 4, 8) and ms > 0
+        assert ctx.flip_items()[0] == g
     finally:
         ctx.free(d)
-        ctx.set_flip_grid(g0)
+        ctx.set_flip_items(i0)
     _check(oracle, ctx, p)

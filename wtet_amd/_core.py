@@ -169,8 +169,8 @@ def lib():
             L.wtet_set_profiling.argtypes = [P, ctypes.c_int]
             L.wtet_set_insertion_ring.argtypes = [P, ctypes.c_int]
             L.wtet_set_flip_graphs.argtypes = [P, ctypes.c_int]
-            L.wtet_set_flip_grid.argtypes = [P, u32]
-            L.wtet_get_flip_grid.argtypes = [P, P, P]
+            L.wtet_set_flip_items.argtypes = [P, u32]
+            L.wtet_get_flip_items.argtypes = [P, P, P]
             L.wtet_autotune.argtypes = [P, P, P, u32, ctypes.POINTER(_Config), P, P]
             L.wtet_device_upload.argtypes = [P, P, ctypes.c_size_t, ctypes.POINTER(P)]
             L.wtet_device_download.argtypes = [P, P, P, ctypes.c_size_t]
@@ -306,19 +306,20 @@ class Context:
         launch them directly."""
         lib().wtet_set_flip_graphs(self._h, int(bool(on)))
 
-    def set_flip_grid(self, blocks: int):
-        if lib().wtet_set_flip_grid(self._h, int(blocks)) != 0:
-            raise ValueError(f"invalid grid {blocks}")
+    def set_flip_items(self, items: int):
+        """List entries per thread that the persistent flip grids are sized for."""
+        if lib().wtet_set_flip_items(self._h, int(items)) != 0:
+            raise ValueError(f"invalid items per thread {items}")
 
-    def flip_grid(self):
-        """(persistent grid in workgroups, graph captures so far)"""
+    def flip_items(self):
+        """(entries per thread, graph captures so far)"""
         b = np.zeros(2, np.uint32)
-        lib().wtet_get_flip_grid(self._h, _p(b[:1]), _p(b[1:]))
+        lib().wtet_get_flip_items(self._h, _p(b[:1]), _p(b[1:]))
         return int(b[0]), int(b[1])
 
     def autotune(self, ptr: int, offsets, config: "Config | None" = None):
-        """Time the candidate persistent grids on these device points; keeps the
-        fastest.  Returns (grid, device ms)."""
+        """Time 1, 2, 4 and 8 list entries per thread on these device points;
+        keeps the fastest.  Returns (entries per thread, device ms)."""
         off = _check_offsets(offsets)
         self._prepare(config)
         c = config._c() if config is not None else Config()._c()

@@ -56,10 +56,13 @@ struct wtet_ctx {
   // device-driven flip iterations: sums (checked tets, flips, iterations),
   // persistent grid, and the captured graph of one round's iterations
   DevBuf isums, slow_t, slow_p;
-  uint32_t grid = 1024;
+  uint32_t items = 2;  // list entries per thread the persistent grids are sized for
   bool use_graphs = true;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<uint64_t> gkey;
+  struct GraphEntry {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;  // one per (buffers, grid, iterations)
   uint32_t graph_captures = 0;
   // counters
   DevBuf ctr, gstats, patch;
@@ -178,11 +181,11 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
 }
 
 enum KernelKind : int {
-  K_PREP = 0, K_VOTE, K_PICK, K_SPLIT, K_SPLIT_POINTS, K_FLIP_CHECK, K_FLIP_DECIDE,
+  K_PREP = 0, K_VOTE, K_PICK, K_SPLIT, K_SPLIT_POINTS, K_FLIP_CHECK, K_FLIP_CHECK_DEFERRED,
   K_FLIP_EXEC, K_FLIP_UPDATE, K_RELOCATE, K_AUDIT, K_OUTPUT, K_NKINDS
 };
 const char* kKernelNames[K_NKINDS] = {"prepare_points", "vote", "pick", "split", "split_points",
-                                      "flip_check", "unused", "flip_decide_exec", "flip_update",
+                                      "flip_check", "flip_check_deferred", "flip_decide_exec", "flip_update",
                                       "relocate", "audit", "output"};
 
 // brackets the launches issued by fn with an event pair when profiling
@@ -260,28 +263,46 @@ FlipLists lists_of(wtet_ctx* c) {
   return L;
 }
 
-void enqueue_flip_iterations(wtet_ctx* c, const Mesh& M, const FlipLists& L, uint64_t* st, int k) {
+void enqueue_flip_iterations(wtet_ctx* c, const Mesh& M, const FlipLists& L, uint64_t* st, int k,
+                             uint32_t grid, uint32_t dgrid) {
   hipStream_t s = c->stream;
   uint64_t* sums = P<uint64_t>(c->isums);
   launch_iter_begin(M.ctr, s);
   for (int i = 0; i < k; ++i) {
-    timed(c, K_FLIP_CHECK, [&] { launch_flip_check(M, L, c->grid, st, s); });
-    timed(c, K_FLIP_EXEC, [&] { launch_flip_decide_exec(M, L, c->grid, s); });
-    timed(c, K_FLIP_UPDATE, [&] { launch_flip_update(M, L, c->grid, s); });
+    timed(c, K_FLIP_CHECK, [&] { launch_flip_check(M, L, grid, st, s); });
+    timed(c, K_FLIP_CHECK_DEFERRED, [&] { launch_flip_check_deferred(M, L, dgrid, st, s); });
+    timed(c, K_FLIP_EXEC, [&] { launch_flip_decide_exec(M, L, grid, s); });
+    timed(c, K_FLIP_UPDATE, [&] { launch_flip_update(M, L, grid, s); });
     if (i + 1 < k) launch_iter_step(M.ctr, sums, s);
     else launch_iter_end(M.ctr, sums, s);
   }
 }
 
-// k flip iterations without the host: replays a captured graph (captured
-// again whenever a buffer, the grid or k changed), or launches directly
-// when profiling (per-kernel events) or when graphs are off.
-void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k) {
+// persistent grid for a list of about `a` entries: a power of two of
+// workgroups in [lo, 4096]
+uint32_t grid_for(const wtet_ctx* c, uint64_t a, uint32_t lo) {
+  uint64_t want = (a + (uint64_t)kBlock * c->items - 1) / ((uint64_t)kBlock * c->items);
+  uint32_t g = lo;
+  while (g < want && g < 4096) g <<= 1;
+  return g;
+}
+
+// k flip iterations without the host: replays a captured graph (one per
+// buffer set, grid and k; captured on first use), or launches directly when
+// profiling (per-kernel events) or when graphs are off.
+// a_hint: bound on the first list; d_hint: deferred tets per iteration
+// expected (from the previous round).  The deferred check runs the full
+// predicates, whose exact stage keeps a thread busy for microseconds, so its
+// grid follows the deferred count rather than the list.
+void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, uint64_t a_hint,
+                    uint64_t d_hint) {
   Mesh M = mesh_of(c, inf, mode);
   FlipLists L = lists_of(c);
+  uint32_t grid = grid_for(c, a_hint, 16);
+  uint32_t dgrid = std::min(grid_for(c, d_hint, 1), grid);
   if (!c->use_graphs || c->profiling) {
     if (c->evused > 4096) collect_times(c);
-    enqueue_flip_iterations(c, M, L, st, k);
+    enqueue_flip_iterations(c, M, L, st, k, grid, dgrid);
     HIP_CHECK(hipGetLastError());
     return;
   }
@@ -292,25 +313,32 @@ void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k) {
       (uint64_t)(uintptr_t)M.tmsg,  (uint64_t)(uintptr_t)M.tfwd,   (uint64_t)(uintptr_t)M.ctr,
       (uint64_t)M.inf,              (uint64_t)M.mode,              (uint64_t)(uintptr_t)L.act[0],
       (uint64_t)(uintptr_t)L.act[1], (uint64_t)(uintptr_t)L.prop,  (uint64_t)(uintptr_t)L.out,
-      (uint64_t)(uintptr_t)L.slow,
-      (uint64_t)(uintptr_t)st,      (uint64_t)(uintptr_t)c->isums.p, (uint64_t)c->grid,
-      (uint64_t)k};
-  if (!c->gexec || key != c->gkey) {
-    if (c->gexec) {
-      HIP_CHECK(hipGraphExecDestroy(c->gexec));
-      c->gexec = nullptr;
+      (uint64_t)(uintptr_t)L.slow,  (uint64_t)(uintptr_t)st,       (uint64_t)(uintptr_t)c->isums.p,
+      (uint64_t)dgrid,              (uint64_t)grid,                (uint64_t)k};
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : c->graphs)
+    if (g.key == key) exec = g.exec;
+  if (!exec) {
+    // graphs of older buffer sets are stale: keep only this buffer set's
+    for (size_t i = 0; i < c->graphs.size();) {
+      if (!std::equal(key.begin(), key.end() - 3, c->graphs[i].key.begin())) {
+        HIP_CHECK(hipGraphExecDestroy(c->graphs[i].exec));
+        c->graphs.erase(c->graphs.begin() + (long)i);
+      } else {
+        ++i;
+      }
     }
     hipGraph_t g = nullptr;
     HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_flip_iterations(c, M, L, st, k);
+    enqueue_flip_iterations(c, M, L, st, k, grid, dgrid);
     HIP_CHECK(hipStreamEndCapture(c->stream, &g));
-    hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HIP_CHECK(e);
-    c->gkey = key;
+    c->graphs.push_back({key, exec});
     c->graph_captures++;
   }
-  HIP_CHECK(hipGraphLaunch(c->gexec, c->stream));
+  HIP_CHECK(hipGraphLaunch(exec, c->stream));
 }
 
 // tet slots after growth by `need`; the flip iterations read the capacity
@@ -446,6 +474,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   HIP_CHECK(hipGetLastError());
 
   uint32_t cur = 0, round = 0;
+  uint64_t dsum_prev = 0;
   uint32_t max_rounds = std::max<uint32_t>(cfg.max_insert_iterations, 1u);
   int max_fi = (int)std::min<uint32_t>(std::max<uint32_t>(cfg.max_flip_iterations, 1u), 64u);
   bool flipping = cfg.enable_flipping != 0;
@@ -486,7 +515,12 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
     });
     HIP_CHECK(hipGetLastError());
     if (flipping) {
-      run_flip_batch(c, V, mode, st, max_fi);
+      // the first list holds the entries carried over plus four per split;
+      // the deferred count per iteration follows the previous round's
+      uint64_t dsum = c->h_ctr[C_SLOW_SUM];
+      uint64_t dper = (2 * (dsum - dsum_prev) + max_fi - 1) / (uint64_t)max_fi;
+      dsum_prev = dsum;
+      run_flip_batch(c, V, mode, st, max_fi, (uint64_t)c->h_ctr[C_A] + 4ull * W, dper);
       if (U > 0) {
         timed(c, K_RELOCATE, [&] {
           launch_relocate(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st,
@@ -505,7 +539,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
       read_ctr(c);
       if (c->h_ctr[C_STOP]) grow_for(c, (uint64_t)c->h_ctr[C_TETS] + c->h_ctr[C_A] + 64);
       else if (c->h_ctr[C_A] == 0) break;
-      run_flip_batch(c, V, mode, st, 16);
+      run_flip_batch(c, V, mode, st, 16, c->h_ctr[C_A], c->h_ctr[C_A] / 64);
     }
   }
   uint64_t sums[4] = {0, 0, 0, 0};
@@ -935,7 +969,7 @@ void wtet_destroy(wtet_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   for (DevBuf* b : c->all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
@@ -1482,34 +1516,34 @@ wtet_status wtet_set_flip_graphs(wtet_ctx* c, int on) {
   return WTET_OK;
 }
 
-wtet_status wtet_set_flip_grid(wtet_ctx* c, uint32_t blocks) {
-  if (!c || blocks == 0 || blocks > 65536) return WTET_ERR_INVALID_ARGUMENT;
-  c->grid = blocks;
+wtet_status wtet_set_flip_items(wtet_ctx* c, uint32_t items) {
+  if (!c || items == 0 || items > 64) return WTET_ERR_INVALID_ARGUMENT;
+  c->items = items;
   return WTET_OK;
 }
 
-wtet_status wtet_get_flip_grid(const wtet_ctx* c, uint32_t* blocks, uint32_t* graph_captures) {
+wtet_status wtet_get_flip_items(const wtet_ctx* c, uint32_t* items, uint32_t* graph_captures) {
   if (!c) return WTET_ERR_INVALID_ARGUMENT;
-  if (blocks) *blocks = c->grid;
+  if (items) *items = c->items;
   if (graph_captures) *graph_captures = c->graph_captures;
   return WTET_OK;
 }
 
 wtet_status wtet_autotune(wtet_ctx* c, const float* d_xyz, const uint64_t* offsets,
-                          uint32_t num_clouds, const wtet_config* cfg, uint32_t* best_grid,
+                          uint32_t num_clouds, const wtet_config* cfg, uint32_t* best_items,
                           double* best_ms) {
   if (!c || !d_xyz || !offsets || num_clouds == 0) return WTET_ERR_INVALID_ARGUMENT;
-  static const uint32_t cand[] = {256, 512, 1024, 2048, 4096};
-  uint32_t keep = c->grid, best = keep;
+  static const uint32_t cand[] = {1, 2, 4, 8};
+  uint32_t keep = c->items, best = keep;
   double bms = 1e300;
   for (uint32_t g : cand) {
-    c->grid = g;
+    c->items = g;
     wtet_batch_device_result r;
-    // the first call at a grid size captures its graph: time the second
+    // the first call at a setting captures its graphs: time the second
     for (int rep = 0; rep < 2; ++rep) {
       wtet_status st = wtet_delaunay_batch_resident(c, d_xyz, offsets, num_clouds, cfg, &r);
       if (st != WTET_OK) {
-        c->grid = keep;
+        c->items = keep;
         return st;
       }
     }
@@ -1518,8 +1552,8 @@ wtet_status wtet_autotune(wtet_ctx* c, const float* d_xyz, const uint64_t* offse
       best = g;
     }
   }
-  c->grid = best;
-  if (best_grid) *best_grid = best;
+  c->items = best;
+  if (best_items) *best_items = best;
   if (best_ms) *best_ms = bms;
   return WTET_OK;
 }

@@ -64,8 +64,12 @@ void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, u
 // tag, round and parity are read from ctr (device-driven, graph-capturable);
 // grid = persistent blocks of the three list kernels
 void launch_iter_begin(uint32_t* ctr, hipStream_t s);
+// filtered mode: the FP32-only check (deferring undecided tets), then the
+// full check of the deferred tets; exact mode: the full check, nothing deferred
 void launch_flip_check(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_t* st,
                        hipStream_t s);
+void launch_flip_check_deferred(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_t* st,
+                                hipStream_t s);
 void launch_flip_decide_exec(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s);
 void launch_flip_update(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s);
 // sums[0] += checked tets, sums[1] += flips, sums[2] += iterations

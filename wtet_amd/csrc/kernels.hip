@@ -423,7 +423,7 @@ void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, u
   if (M.mode == PRED_FILTERED) {
     launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
     k_split_points<LEAN><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
-    k_split_points<SLOW><<<std::min(nblocks(U), 1024u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_split_points<SLOW><<<std::min(nblocks(U) / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
   } else {
     k_split_points<FULL><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
   }
@@ -465,6 +465,7 @@ __device__ __forceinline__ void iter_end(uint32_t* ctr, unsigned long long* sums
   sums[0] += ctr[C_A];
   sums[1] += ctr[C_FLIPS];
   sums[2] += 1ull;
+  ctr[C_SLOW_SUM] += ctr[C_SLOW_T];
   ctr[C_A] = ctr[C_ACT_NEXT];
   ctr[C_PAR] ^= 1u;
 }
@@ -585,12 +586,12 @@ void launch_iter_step(uint32_t* ctr, uint64_t* sums, hipStream_t s) {
   k_iter_step<<<1, 64, 0, s>>>(ctr, reinterpret_cast<unsigned long long*>(sums));
 }
 void launch_flip_check(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_t* st, hipStream_t s) {
-  if (M.mode == PRED_FILTERED) {
-    k_flip_check<LEAN><<<grid, kBlock, 0, s>>>(M, L, st);
-    k_flip_check<SLOW><<<grid, kBlock, 0, s>>>(M, L, st);
-  } else {
-    k_flip_check<FULL><<<grid, kBlock, 0, s>>>(M, L, st);
-  }
+  if (M.mode == PRED_FILTERED) k_flip_check<LEAN><<<grid, kBlock, 0, s>>>(M, L, st);
+  else k_flip_check<FULL><<<grid, kBlock, 0, s>>>(M, L, st);
+}
+void launch_flip_check_deferred(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_t* st,
+                                hipStream_t s) {
+  if (M.mode == PRED_FILTERED) k_flip_check<SLOW><<<grid, kBlock, 0, s>>>(M, L, st);
 }
 void launch_flip_decide_exec(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s) {
   k_flip_decide_exec<<<grid, kBlock, 0, s>>>(M, L);
@@ -623,7 +624,7 @@ void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint3
   if (M.mode == PRED_FILTERED) {
     launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
     k_relocate<LEAN><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
-    k_relocate<SLOW><<<std::min(nblocks(U), 1024u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_relocate<SLOW><<<std::min(nblocks(U) / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
   } else {
     k_relocate<FULL><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
   }

@@ -39,6 +39,7 @@ enum Ctr : int {
   C_PEAKA = 21,    // longest active list seen
   C_SLOW_T = 22,   // tets deferred by the lean flip check this iteration
   C_SLOW_P = 23,   // points deferred by the lean point kernels
+  C_SLOW_SUM = 24, // tets deferred by the lean flip check, summed over iterations
   C_COUNT = 32
 };
 
