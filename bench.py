@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-cloud batched leg")
     ap.add_argument("--no-stress", action="store_true", help="skip the lattice / sphere leg")
     ap.add_argument("--no-presize", action="store_true", help="skip the learned-presizing leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default entries per thread instead of autotuning them")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the flip iterations directly instead of replaying HIP graphs "
                          "(the same kernels; used under rocprofv3)")
@@ -111,7 +113,7 @@ def main():
         return (time.perf_counter() - t0) * 1e3 / max(k, 1)
 
     # autotuned persistent grid sizing (entries per thread), outside the timed region
-    items, _ = ctx.autotune(d_pts, [0, args.n], filt)
+    items = ctx.flip_items()[0] if args.no_autotune else ctx.autotune(d_pts, [0, args.n], filt)[0]
     run(filt, args.warmup)
     ms = run(filt, args.steps)
     value = args.n / (ms / 1e3)

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -590,7 +592,9 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   mout = Mout;
   nviol = c->h_ctr[C_VIOL];
   if (cstart[C] != Mout) throw Fail{WTET_ERR_INTERNAL, "output row count mismatch"};
-  if (!resident || nviol > 0) {
+  // the host needs the rows: for its result, or for the repair of one cloud
+  // (a batch downloads only the clouds with violations)
+  if (!resident || (nviol > 0 && C == 1)) {
     // room for the repair to append (it rarely adds more than a few tets)
     size_t cap = (size_t)Mout + Mout / 32 + 65536;
     size_t c1 = c->h_cap, c2 = c->h_cap;
@@ -802,12 +806,41 @@ void batch_impl(void* a_) {
     byc.emplace_back(cl, ((row - cstart[cl]) << 2) | (r & 3u));
   }
   std::sort(byc.begin(), byc.end());
+  // where each cloud's rows sit in the pinned staging: the whole mesh for a
+  // host result; for a resident result only the clouds with violations,
+  // downloaded here one segment each
+  std::vector<uint64_t> hoff(C, 0);
+  if (!resident) {
+    for (uint32_t cl = 0; cl < C; ++cl) hoff[cl] = cstart[cl];
+  } else if (!byc.empty()) {
+    uint64_t tot = 0;
+    for (size_t i = 0; i < byc.size(); ++i)
+      if (i == 0 || byc[i].first != byc[i - 1].first) {
+        uint32_t cl = byc[i].first;
+        hoff[cl] = tot;
+        tot += cstart[cl + 1] - cstart[cl];
+      }
+    size_t c1 = c->h_cap, c2 = c->h_cap;
+    ensure_pinned(c->h_tets, c1, (size_t)tot * 4 + 4);
+    ensure_pinned(c->h_adj, c2, (size_t)tot * 4 + 4);
+    c->h_cap = std::min(c1, c2);
+    for (size_t i = 0; i < byc.size(); ++i)
+      if (i == 0 || byc[i].first != byc[i - 1].first) {
+        uint32_t cl = byc[i].first;
+        size_t cnt = cstart[cl + 1] - cstart[cl];
+        HIP_CHECK(hipMemcpyAsync(c->h_tets + 4 * hoff[cl], P<uint32_t>(c->prop) + 4 * (size_t)cstart[cl],
+                                 cnt * 16, hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipMemcpyAsync(c->h_adj + 4 * hoff[cl], P<uint32_t>(c->fout) + 4 * (size_t)cstart[cl],
+                                 cnt * 16, hipMemcpyDeviceToHost, c->stream));
+      }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
   // failed_verts per cloud: the vertex opposite each non-Delaunay real-only face
   std::vector<std::vector<uint32_t>> failed(C);
   for (auto& pr : byc) {
     uint32_t cl = pr.first, nc = a.off[cl + 1] - a.off[cl];
-    const uint32_t* T = c->h_tets + 4 * (size_t)cstart[cl];
-    const uint32_t* A = c->h_adj + 4 * (size_t)cstart[cl];
+    const uint32_t* T = c->h_tets + 4 * (size_t)hoff[cl];
+    const uint32_t* A = c->h_adj + 4 * (size_t)hoff[cl];
     uint32_t t = pr.second >> 2, f = pr.second & 3u;
     uint32_t e = A[4 * (size_t)t + f];
     if (e == kInvalid) continue;
@@ -826,28 +859,50 @@ void batch_impl(void* a_) {
     ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
     HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
-    size_t i = 0;
-    while (i < byc.size()) {
-      uint32_t cl = byc[i].first;
+    // clouds are independent: repair them on several host threads
+    struct Job {
+      uint32_t cl;
       std::vector<uint32_t> viol;
-      for (; i < byc.size() && byc[i].first == cl; ++i) viol.push_back(byc[i].second);
-      size_t cnt = cstart[cl + 1] - cstart[cl];
-      size_t cap = cnt + cnt / 8 + 4096;
-      CloudRows& R = rows[cl];
-      R.tets.assign(c->h_tets + 4 * (size_t)cstart[cl], c->h_tets + 4 * (size_t)(cstart[cl] + cnt));
-      R.adj.assign(c->h_adj + 4 * (size_t)cstart[cl], c->h_adj + 4 * (size_t)(cstart[cl] + cnt));
-      R.tets.resize(4 * cap);
-      R.adj.resize(4 * cap);
-      MeshBuf mb{R.tets.data(), R.adj.data(), cnt, cap, {}};
+      wtet_status st = WTET_OK;
       RepairReport rr;
-      wtet_status rs = repair_delaunay(c->h_xyzn + 3 * (size_t)a.off[cl], a.off[cl + 1] - a.off[cl],
-                                       mb, viol, a.cfg.predicate_mode, &rr);
-      if (rs != WTET_OK)
-        throw Fail{rs, "host repair of cloud " + std::to_string(cl) + " failed: " + rr.message};
-      R.tets.resize(4 * mb.count);
-      R.adj.resize(4 * mb.count);
-      R.repaired = true;
-      c->stats.repair_regions += rr.regions;
+    };
+    std::vector<Job> jobs;
+    for (size_t i = 0; i < byc.size();) {
+      Job j;
+      j.cl = byc[i].first;
+      for (; i < byc.size() && byc[i].first == j.cl; ++i) j.viol.push_back(byc[i].second);
+      jobs.push_back(std::move(j));
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (size_t k; (k = next.fetch_add(1)) < jobs.size();) {
+        Job& jb = jobs[k];
+        uint32_t cl = jb.cl;
+        size_t cnt = cstart[cl + 1] - cstart[cl];
+        size_t cap = cnt + cnt / 8 + 4096;
+        CloudRows& R = rows[cl];
+        R.tets.assign(c->h_tets + 4 * (size_t)hoff[cl], c->h_tets + 4 * (size_t)(hoff[cl] + cnt));
+        R.adj.assign(c->h_adj + 4 * (size_t)hoff[cl], c->h_adj + 4 * (size_t)(hoff[cl] + cnt));
+        R.tets.resize(4 * cap);
+        R.adj.resize(4 * cap);
+        MeshBuf mb{R.tets.data(), R.adj.data(), cnt, cap, {}};
+        jb.st = repair_delaunay(c->h_xyzn + 3 * (size_t)a.off[cl], a.off[cl + 1] - a.off[cl], mb,
+                                jb.viol, a.cfg.predicate_mode, &jb.rr);
+        R.tets.resize(4 * mb.count);
+        R.adj.resize(4 * mb.count);
+        R.repaired = true;
+      }
+    };
+    unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    size_t nth = std::min<size_t>(std::min<size_t>(hw, 16), jobs.size());
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nth; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    for (const Job& jb : jobs) {
+      if (jb.st != WTET_OK)
+        throw Fail{jb.st, "host repair of cloud " + std::to_string(jb.cl) + " failed: " + jb.rr.message};
+      c->stats.repair_regions += jb.rr.regions;
     }
   }
   auto t2 = std::chrono::steady_clock::now();

@@ -201,8 +201,11 @@ WT_HD BI<D> to_fixed(float v, int E) {
   int s = p.e + tz - E;  // >= 0
   int limb = s >> 5, bit = s & 31;
   uint64_t x = (uint64_t)(p.m >> tz) << bit;
-  if (limb < D) r.w[limb] = (uint32_t)x;
-  if (limb + 1 < D) r.w[limb + 1] = (uint32_t)(x >> 32);
+  // unrolled selects instead of r.w[limb]: a run-time index into a local
+  // array would put it in scratch memory
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+    r.w[i] = i == limb ? (uint32_t)x : (i == limb + 1 ? (uint32_t)(x >> 32) : 0u);
   return p.neg ? bi_neg(r) : r;
 }
 
