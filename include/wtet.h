@@ -83,6 +83,8 @@ typedef struct wtet_stats {
   uint64_t tet_capacity;   /* tet slots allocated when the loop started        */
   float presize_pred[3];   /* learned presizing: predicted log(slots / n),
                               log(peak active / n), log(1 + stuck faces)        */
+  uint64_t repair_rows_staged;   /* rows copied to the host for the repair     */
+  uint32_t repair_full_downloads; /* repairs that needed every row            */
 } wtet_stats;
 
 typedef struct wtet_ctx wtet_ctx;

@@ -240,3 +240,26 @@ def test_autotune_picks_a_candidate(oracle, ctx):
         ctx.free(d)
         ctx.set_flip_items(i0)
     _check(oracle, ctx, p)
+
+
+@pytest.mark.parametrize("kind", ["sphere", "uniform"])
+def test_resident_repair_on_neighbourhood(oracle, ctx, kind):
+    """A resident result is repaired on the downloaded neighbourhood of the
+    stuck faces only; the mesh equals the host-result call's."""
+    p = W.sphere(200000, 71) if kind == "sphere" else W.uniform(300000, 72)
+    d = ctx.upload(p)
+    try:
+        m, dt, da = ctx.delaunay_resident(d, len(p))
+        st = ctx.stats()
+        tets = ctx.download(dt, (m, 4), np.uint32)
+        adj = ctx.download(da, (m, 4), np.uint32)
+    finally:
+        ctx.free(d)
+    assert st["violations_gpu"] > 0, st
+    assert 0 < st["repair_rows_staged"] < m or st["repair_full_downloads"] == 1, st
+    host = ctx.delaunay(p)
+    assert np.array_equal(oracle.canonical_real(tets, len(p)), oracle.canonical_real(host.tets, len(p)))
+    back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
+    assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))
+    v, _ = wtet_amd.check_quality(p, wtet_amd.DelaunayResult(tets, adj, np.zeros(0, np.uint32)))
+    assert v == 0

@@ -116,6 +116,8 @@ class _Stats(ctypes.Structure):
         ("presize_slots", ctypes.c_uint64),
         ("tet_capacity", ctypes.c_uint64),
         ("presize_pred", ctypes.c_float * 3),
+        ("repair_rows_staged", ctypes.c_uint64),
+        ("repair_full_downloads", ctypes.c_uint32),
     ]
 
 

@@ -58,6 +58,8 @@ struct wtet_ctx {
   // device-driven flip iterations: sums (checked tets, flips, iterations),
   // persistent grid, and the captured graph of one round's iterations
   DevBuf isums, slow_t, slow_p;
+  // neighbourhood of the host repair (sparse download)
+  DevBuf nbr_mark, nbr_list, nbr_misc, nbr_pack;
   uint32_t items = 2;  // list entries per thread the persistent grids are sized for
   bool use_graphs = true;
   struct GraphEntry {
@@ -97,7 +99,8 @@ struct wtet_ctx {
                    &fit,    &fround, &tmsg,   &tfwd,  &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
                    &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
-                   &presize_params, &feat_acc, &feat, &pred, &isums, &slow_t, &slow_p};
+                   &presize_params, &feat_acc, &feat, &pred, &isums, &slow_t, &slow_p,
+                   &nbr_mark, &nbr_list, &nbr_misc, &nbr_pack};
     all.assign(std::begin(l), std::end(l));
   }
 };
@@ -592,9 +595,9 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   mout = Mout;
   nviol = c->h_ctr[C_VIOL];
   if (cstart[C] != Mout) throw Fail{WTET_ERR_INTERNAL, "output row count mismatch"};
-  // the host needs the rows: for its result, or for the repair of one cloud
-  // (a batch downloads only the clouds with violations)
-  if (!resident || (nviol > 0 && C == 1)) {
+  // the host needs every row only for a host result; a resident result's
+  // repair downloads the neighbourhood it works on
+  if (!resident) {
     // room for the repair to append (it rarely adds more than a few tets)
     size_t cap = (size_t)Mout + Mout / 32 + 65536;
     size_t c1 = c->h_cap, c2 = c->h_cap;
@@ -653,6 +656,62 @@ struct DelaunayArgs {
   wtet_device_result* dout;   // device-resident result
 };
 
+// Rows the host repair will read, for a resident result: those within a
+// few face-steps of the recorded faces and the tail rows that fill holes,
+// with their neighbours.  They land in the pinned staging at their own row;
+// present[] marks them.  rec (rows) is on the device in c->rec.
+constexpr int kRepairRings = 6;
+constexpr uint32_t kRepairTail = 4096;
+
+void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M, std::vector<uint8_t>& present) {
+  hipStream_t s = c->stream;
+  size_t cap = (size_t)M + M / 32 + 65536;
+  size_t c1 = c->h_cap, c2 = c->h_cap;
+  ensure_pinned(c->h_tets, c1, cap * 4);
+  ensure_pinned(c->h_adj, c2, cap * 4);
+  c->h_cap = std::min(c1, c2);
+  uint32_t tail = std::min(M, kRepairTail);
+  ensure(c, c->nbr_mark, (size_t)M * 4 + 16);
+  ensure(c, c->nbr_list, (size_t)M * 4 + 16);
+  ensure(c, c->nbr_misc, 4 * (kRepairRings + 4));
+  uint32_t* cnt = P<uint32_t>(c->nbr_misc);
+  launch_neighbourhood(P<uint32_t>(c->rec), nrec, M, tail, kRepairRings, P<uint32_t>(c->fout),
+                       P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), cnt, cnt + 1, s);
+  HIP_CHECK(hipGetLastError());
+  uint32_t n = 0;
+  HIP_CHECK(hipMemcpyAsync(&n, cnt, 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  if (n > M) throw Fail{WTET_ERR_INTERNAL, "neighbourhood larger than the mesh"};
+  ensure(c, c->nbr_pack, (size_t)n * 36 + 16);
+  launch_nbr_gather(P<uint32_t>(c->nbr_list), n, P<uint32_t>(c->prop), P<uint32_t>(c->fout),
+                    P<uint32_t>(c->nbr_pack), s);
+  HIP_CHECK(hipGetLastError());
+  ensure_pinned(c->h_patch, c->h_patch_cap, (size_t)n * 9 + 9);
+  HIP_CHECK(hipMemcpyAsync(c->h_patch, c->nbr_pack.p, (size_t)n * 36, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  present.assign(M, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t* q = c->h_patch + 9 * (size_t)i;
+    uint32_t r = q[0];
+    std::memcpy(c->h_tets + 4 * (size_t)r, q + 1, 16);
+    std::memcpy(c->h_adj + 4 * (size_t)r, q + 5, 16);
+    present[r] = 1;
+  }
+  c->stats.repair_rows_staged = n;
+}
+
+void download_rows(wtet_ctx* c, uint32_t M) {
+  size_t cap = (size_t)M + M / 32 + 65536;
+  size_t c1 = c->h_cap, c2 = c->h_cap;
+  ensure_pinned(c->h_tets, c1, cap * 4);
+  ensure_pinned(c->h_adj, c2, cap * 4);
+  c->h_cap = std::min(c1, c2);
+  HIP_CHECK(hipMemcpyAsync(c->h_tets, c->prop.p, (size_t)M * 16, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->h_adj, c->fout.p, (size_t)M * 16, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->stats.repair_rows_staged = M;
+}
+
 void delaunay_impl(void* a_) {
   DelaunayArgs& a = *static_cast<DelaunayArgs*>(a_);
   wtet_ctx* c = a.c;
@@ -671,7 +730,17 @@ void delaunay_impl(void* a_) {
   bool resident = a.dout != nullptr;
   run_device(c, d_xyz, {0u, a.n}, a.cfg, K, rec, nviol, resident, mout, cstart);
   c->stats.violations_gpu = nviol;
+  auto t1 = std::chrono::steady_clock::now();
+  std::vector<uint8_t> present;
+  if (resident && nviol > 0) {
+    if (rec.size() < nviol) download_rows(c, mout);  // records overflowed: repair needs all
+    else stage_neighbourhood(c, (uint32_t)rec.size(), mout, present);
+  }
   MeshBuf mb{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
+  if (!present.empty()) {
+    mb.present = present.data();
+    mb.known = mout;
+  }
   bool on_host = !resident || nviol > 0;
   // failed_verts: the vertex opposite each non-Delaunay real-only face
   std::vector<uint32_t> failed;
@@ -688,7 +757,6 @@ void delaunay_impl(void* a_) {
     }
   std::sort(failed.begin(), failed.end());
   failed.erase(std::unique(failed.begin(), failed.end()), failed.end());
-  auto t1 = std::chrono::steady_clock::now();
   bool repaired = false;
   if (nviol > 0 && a.cfg.enable_splaying) {
     // the repair needs the normalized coordinates of the input ids
@@ -697,6 +765,14 @@ void delaunay_impl(void* a_) {
     HIP_CHECK(hipStreamSynchronize(c->stream));
     RepairReport rr;
     wtet_status rs = repair_delaunay(c->h_xyzn, a.n, mb, rec, a.cfg.predicate_mode, &rr);
+    if (rs != WTET_OK && rr.needs_full) {
+      // the neighbourhood was too small: every row, and again from the start
+      download_rows(c, mout);
+      mb = MeshBuf{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
+      rr = RepairReport{};
+      rs = repair_delaunay(c->h_xyzn, a.n, mb, rec, a.cfg.predicate_mode, &rr);
+      c->stats.repair_full_downloads++;
+    }
     if (rs != WTET_OK) throw Fail{rs, "host repair failed: " + rr.message};
     c->stats.repair_regions = rr.regions;
     repaired = true;

@@ -730,6 +730,73 @@ void launch_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx, hip
   if (n) k_map_records<<<nblocks(n), kBlock, 0, s>>>(rec, n, slot2idx);
 }
 
+// ---------------------------------------------------------------------------
+// neighbourhood of the rows the host repair will touch: the rows of the
+// recorded faces and the last `tail` rows (moved into holes), grown ring by
+// ring over the output adjacency.  list[0, cnt) holds each marked row once;
+// ring k reads list[bounds[k], bounds[k+1]).
+// ---------------------------------------------------------------------------
+__global__ void k_nbr_seed(const uint32_t* rec, uint32_t nrec, uint32_t M, uint32_t tail,
+                           uint32_t* mark, uint32_t* list, uint32_t* cnt) {
+  uint32_t i = gid();
+  uint32_t row = kInvalid;
+  if (i < nrec) row = rec[i] >> 2;
+  else if (i < nrec + tail) row = M - tail + (i - nrec);
+  bool add = row < M && atomicOr(&mark[row], 1u) == 0u;
+  if (add) list[append_slot(cnt)] = row;
+}
+
+__global__ void k_nbr_snap(const uint32_t* cnt, uint32_t* bounds, int k) {
+  if (threadIdx.x == 0) bounds[k] = cnt[0];
+}
+
+__global__ void k_nbr_ring(const uint32_t* adj, uint32_t M, uint32_t* mark, uint32_t* list,
+                           uint32_t* cnt, const uint32_t* bounds, int k) {
+  uint32_t b = bounds[k], e = bounds[k + 1];
+  uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = b + gid(); i < e; i += stride) {
+    uint32_t r = list[i];
+    for (int f = 0; f < 4; ++f) {
+      uint32_t a = adj[4 * (size_t)r + f];
+      if (a == kInvalid) continue;
+      uint32_t nb = a >> 5;
+      if (nb < M && atomicOr(&mark[nb], 1u) == 0u) list[append_slot(cnt)] = nb;
+    }
+  }
+}
+
+// rows list[0, n) of the output arrays -> packed (row, tet[4], adj[4])
+__global__ void k_nbr_gather(const uint32_t* list, uint32_t n, const uint32_t* tets,
+                             const uint32_t* adj, uint32_t* out) {
+  uint32_t i = gid();
+  if (i >= n) return;
+  uint32_t r = list[i];
+  uint32_t* q = out + 9 * (size_t)i;
+  q[0] = r;
+  uint4 t = reinterpret_cast<const uint4*>(tets)[r];
+  uint4 a = reinterpret_cast<const uint4*>(adj)[r];
+  q[1] = t.x; q[2] = t.y; q[3] = t.z; q[4] = t.w;
+  q[5] = a.x; q[6] = a.y; q[7] = a.z; q[8] = a.w;
+}
+
+void launch_neighbourhood(const uint32_t* rec, uint32_t nrec, uint32_t M, uint32_t tail, int rings,
+                          const uint32_t* adj, uint32_t* mark, uint32_t* list, uint32_t* cnt,
+                          uint32_t* bounds, hipStream_t s) {
+  k_fill_u32<<<nblocks(M), kBlock, 0, s>>>(mark, 0u, M);
+  k_fill_u32<<<1, kBlock, 0, s>>>(cnt, 0u, 1);
+  k_nbr_seed<<<nblocks((uint64_t)nrec + tail), kBlock, 0, s>>>(rec, nrec, M, tail, mark, list, cnt);
+  k_fill_u32<<<1, kBlock, 0, s>>>(bounds, 0u, 1);
+  for (int k = 0; k < rings; ++k) {
+    k_nbr_snap<<<1, 64, 0, s>>>(cnt, bounds, k + 1);
+    k_nbr_ring<<<256, kBlock, 0, s>>>(adj, M, mark, list, cnt, bounds, k);
+  }
+}
+
+void launch_nbr_gather(const uint32_t* list, uint32_t n, const uint32_t* tets, const uint32_t* adj,
+                       uint32_t* out, hipStream_t s) {
+  if (n) k_nbr_gather<<<nblocks(n), kBlock, 0, s>>>(list, n, tets, adj, out);
+}
+
 __global__ void k_apply_patch(const uint32_t* patch, uint32_t n, uint32_t* tets, uint32_t* adj) {
   uint32_t i = gid();
   if (i >= n) return;

@@ -240,19 +240,40 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
 // ---------------------------------------------------------------------------
 // in-place repair on the result arrays (ids, adjacency (t << 5) | f)
 // ---------------------------------------------------------------------------
+struct NotPresent {};
+
 struct Mesh32 {
   MeshBuf& m;
   uint32_t inf;
+  void need(uint32_t t) const {
+    if (m.present && t < m.known && !m.present[t]) throw NotPresent{};
+  }
   size_t count() const { return m.count; }
-  bool dead(uint32_t t) const { return m.tets[4 * (size_t)t] == kInvalid; }
+  bool dead(uint32_t t) const {
+    need(t);
+    return m.tets[4 * (size_t)t] == kInvalid;
+  }
   bool has_inf(uint32_t t) const {
+    need(t);
     const uint32_t* v = &m.tets[4 * (size_t)t];
     return v[0] == inf || v[1] == inf || v[2] == inf || v[3] == inf;
   }
-  uint32_t* tv(uint32_t t) { return &m.tets[4 * (size_t)t]; }
-  const uint32_t* tv(uint32_t t) const { return &m.tets[4 * (size_t)t]; }
-  uint32_t& ta(uint32_t t, int f) { return m.adj[4 * (size_t)t + f]; }
-  uint32_t ta(uint32_t t, int f) const { return m.adj[4 * (size_t)t + f]; }
+  uint32_t* tv(uint32_t t) {
+    need(t);
+    return &m.tets[4 * (size_t)t];
+  }
+  const uint32_t* tv(uint32_t t) const {
+    need(t);
+    return &m.tets[4 * (size_t)t];
+  }
+  uint32_t& ta(uint32_t t, int f) {
+    need(t);
+    return m.adj[4 * (size_t)t + f];
+  }
+  uint32_t ta(uint32_t t, int f) const {
+    need(t);
+    return m.adj[4 * (size_t)t + f];
+  }
 };
 
 bool face_bad(const PointSet& S, const Mesh32& W, uint32_t t, int f, int mode) {
@@ -266,8 +287,24 @@ bool face_bad(const PointSet& S, const Mesh32& W, uint32_t t, int f, int mode) {
 
 }  // namespace
 
+static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
+                               const std::vector<uint32_t>& viol, int mode, RepairReport* rep);
+
 wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
                             const std::vector<uint32_t>& viol, int mode, RepairReport* rep) {
+  try {
+    return repair_impl(xyzn, n, mesh, viol, mode, rep);
+  } catch (const NotPresent&) {
+    if (rep) {
+      rep->needs_full = true;
+      rep->message = "the repair reached rows outside the downloaded neighbourhood";
+    }
+    return WTET_ERR_NOT_CONVERGED;
+  }
+}
+
+static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
+                               const std::vector<uint32_t>& viol, int mode, RepairReport* rep) {
   PointSet S;
   S.xyzn = xyzn;
   S.n = n;

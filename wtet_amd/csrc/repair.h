@@ -14,6 +14,7 @@
 struct RepairReport {
   uint32_t regions = 0;      // regions re-triangulated
   uint32_t max_region = 0;   // largest region (tets)
+  bool needs_full = false;   // the repair reached a row the sparse view lacks
   std::string message;
 };
 
@@ -28,6 +29,11 @@ struct MeshBuf {
   size_t count;
   size_t cap;
   std::vector<uint32_t> changed;
+  // sparse view: when set, only rows r < known with present[r] != 0 hold
+  // data (rows >= known are the repair's own); reaching another row stops
+  // the repair with needs_full
+  const uint8_t* present = nullptr;
+  size_t known = 0;
 };
 
 // xyzn: n normalized points (input id order).  viol: (t << 2) | f of
