@@ -263,3 +263,27 @@ def test_resident_repair_on_neighbourhood(oracle, ctx, kind):
     assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))
     v, _ = wtet_amd.check_quality(p, wtet_amd.DelaunayResult(tets, adj, np.zeros(0, np.uint32)))
     assert v == 0
+
+
+@pytest.mark.parametrize("name", ["lattice_128", "sphere_1000000_seed2"])
+def test_degenerate_full_size_properties(ctx, name):
+    """Degenerate inputs at full size: the 128^3 lattice (2,097,152 points,
+    co-spherical everywhere: exact stage and SoS) and 1M points on the unit
+    sphere.  Too large for the oracle in test time: the reference's own
+    audit (check_quality, src/lib.rs:25-78) must find no violation, every
+    point must be a vertex, and the adjacency must be closed."""
+    p = W.lattice(128) if name == "lattice_128" else W.sphere(1_000_000, 2)
+    assert W.sha256(p) == GOLD[name]
+    res = ctx.delaunay(p, wtet_amd.Config(collect_stats=True))
+    st = res.stats
+    v, interior = wtet_amd.check_quality(p, res)
+    assert v == 0, (v, st)
+    assert interior > len(p)
+    ids = np.unique(res.tets)
+    assert len(ids) == len(p) + 5 and ids[0] == 0
+    adj = res.adjacency
+    m = len(adj)
+    back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
+    assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))
+    if name == "lattice_128":
+        assert st["pred"][7] > 0  # SoS decided some insphere calls
