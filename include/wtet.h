@@ -286,6 +286,18 @@ wtet_status wtet_kernel_times(const wtet_ctx *ctx, uint32_t kind, const char **n
 wtet_status wtet_repair(const float *xyzn, uint32_t n, uint32_t **tets, uint32_t **adjacency,
                         uint32_t *num_tets, int mode, uint32_t *regions);
 
+/* Insertion order of enable_hilbert_sorting (host copies of the device
+ * code): the reference's first stride (calculate_initial_stride,
+ * src/gpu/mod.rs:533-548), the position of each of n Hilbert-sorted points
+ * in its stratified order (apply_strided_sampling_indexed, :573-600;
+ * stride 0: the first stride for n), and the 3D Hilbert index of 8-bit
+ * cells (xyz: n triples in 0..255).  The Hilbert curve is this library's
+ * own (Skilling's method); the reference's comes from the hilbert_index
+ * crate, which is not in the tree. */
+uint32_t wtet_initial_stride(uint32_t n);
+wtet_status wtet_stratified_ranks(uint32_t n, uint32_t stride, uint32_t *rank);
+wtet_status wtet_hilbert_index(const uint32_t *xyz, uint32_t n, uint32_t *out);
+
 /* Predicate batches (orient3d/insphere staging and SoS, src/predicates.rs).
  * kind: 0 orient3d_sos (4 pts, 4 ids), 1 orient4d (5 pts, 5 ids),
  *       2 orient3d raw sign (4 pts), 3 insphere raw sign (5 pts).

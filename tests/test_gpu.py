@@ -287,3 +287,31 @@ def test_degenerate_full_size_properties(ctx, name):
     assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))
     if name == "lattice_128":
         assert st["pred"][7] > 0  # SoS decided some insphere calls
+
+
+@pytest.mark.parametrize("case", ["uniform", "lattice", "duplicates", "clusters"])
+def test_hilbert_insertion_order(oracle, ctx, case):
+    """enable_hilbert_sorting: the reference's 8-bit Hilbert keys and
+    stratified insertion order (src/gpu/mod.rs:473-600).  The order cannot
+    change the triangulation: same set as the oracle."""
+    if case == "uniform":
+        p = W.uniform(30000, 81)
+    elif case == "lattice":
+        p = W.lattice(12)
+    elif case == "duplicates":
+        u = W.uniform(5000, 82)
+        p = np.concatenate([u, u[:2500], u[:100]])
+    else:
+        p = W.clusters(20000, 83)
+    r = _check(oracle, ctx, p, enable_hilbert_sorting=True)
+    if case == "duplicates":
+        assert r.stats["duplicates"] == 2600
+
+
+def test_hilbert_order_in_a_batch(oracle, ctx):
+    clouds = [W.uniform(3000, 84), W.lattice(6), W.clusters(2000, 85), W.uniform(5, 86)]
+    res = ctx.delaunay_batch(clouds, config=wtet_amd.Config(enable_hilbert_sorting=True))
+    for c, p in enumerate(clouds):
+        pn, _, _ = oracle.normalize_points(p)
+        ref = oracle.canonical_real(oracle.delaunay(pn), len(p))
+        assert np.array_equal(oracle.canonical_real(res[c].tets, len(p)), ref), c

@@ -186,3 +186,54 @@ def test_mixed_batch_workload_shape():
     assert len(off) == 65 and off[0] == 0 and int(off[-1]) == len(pts)
     assert sizes.min() >= 64 and sizes.max() <= 50000
     assert pts.dtype == np.float32 and pts.min() >= 0 and pts.max() <= 1
+
+
+def test_insertion_order_matches_reference_known_answers(oracle):
+    """The library's stride and stratified order (the code the device runs)
+    against the reference's known answers (src/gpu/mod.rs:606-696, as data
+    in tests/golden) and the oracle's restatement of the same functions."""
+    import json
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "reference_known_answers.json")))
+    for c in gold["initial_stride"]:
+        assert _core.initial_stride(c["n"]) == c["stride"], c["src"]
+    for c in gold["stratified_order"]:
+        rank = _core.stratified_ranks(c["n"], c["stride"])
+        order = np.empty(c["n"], np.int64)
+        order[rank] = np.arange(c["n"])
+        if "order" in c:
+            assert order.tolist() == c["order"], c["src"]
+        else:
+            assert order[: len(c["prefix"])].tolist() == c["prefix"], c["src"]
+    for n in (1, 2, 3, 17, 100, 1000, 4097, 65536, 100001, 250000):
+        s = oracle.initial_stride(n)
+        assert _core.initial_stride(n) == s
+        rank = _core.stratified_ranks(n, s)
+        order = np.empty(n, np.int64)
+        order[rank] = np.arange(n)
+        assert np.array_equal(order, oracle.stratified_order(n, s)), n
+
+
+def test_hilbert_index_is_a_hilbert_curve():
+    """The defining properties on the whole 256^3 grid: a bijection onto
+    0..2^24-1 whose consecutive indices are face-neighbour cells, starting at
+    the origin; plus the reference's own checks (unique corners, locality;
+    src/hilbert.rs:45-120)."""
+    g = np.arange(256, dtype=np.uint32)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    cells = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    h = _core.hilbert_index(cells)
+    inv = np.empty(1 << 24, np.int64)
+    inv[h] = np.arange(1 << 24)
+    assert np.array_equal(np.sort(h), np.arange(1 << 24, dtype=np.uint32))
+    walk = cells[inv].astype(np.int64)
+    assert np.all(np.abs(np.diff(walk, axis=0)).sum(1) == 1)
+    assert walk[0].tolist() == [0, 0, 0]
+    corners = np.array([[a, b, c] for a in (0, 255) for b in (0, 255) for c in (0, 255)])
+    assert len(set(_core.hilbert_index(corners).tolist())) == 8
+    q = lambda v: np.rint(np.array(v, np.float32) * np.float32(255)).astype(np.uint32)  # noqa: E731
+    pts = [q([.5, .5, .5]), q([.51, .5, .5]), q([.5, .51, .5]), q([.5, .5, .51]), q([.1, .1, .1]), q([.9, .9, .9])]
+    hh = _core.hilbert_index(np.array(pts)).astype(np.int64)
+    near = np.abs(hh[1:4] - hh[0]).mean()
+    far = np.abs(hh[4:6] - hh[0]).mean()
+    assert len(set(hh.tolist())) == 6 and near < far

@@ -180,6 +180,10 @@ def lib():
             L.wtet_device_copy.argtypes = [P, P, P, ctypes.c_size_t]
             L.wtet_kernel_times.argtypes = [P, u32, ctypes.POINTER(ctypes.c_char_p),
                                             ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)]
+            L.wtet_initial_stride.argtypes = [u32]
+            L.wtet_initial_stride.restype = u32
+            L.wtet_stratified_ranks.argtypes = [u32, u32, P]
+            L.wtet_hilbert_index.argtypes = [P, u32, P]
             L.wtet_predicates_host.argtypes = [ctypes.c_int, P, P, u32, ctypes.c_int, P]
             L.wtet_predicates_device.argtypes = [P, ctypes.c_int, P, P, u32, ctypes.c_int, P, P]
             L.wtet_repair.argtypes = [P, u32, ctypes.POINTER(ctypes.POINTER(u32)),
@@ -478,6 +482,27 @@ class Context:
         if s != 0:
             raise RuntimeError(f"device predicates failed: {self.error()}")
         return out, st8
+
+
+def initial_stride(n: int) -> int:
+    """The reference's first stratified stride for n points."""
+    return int(lib().wtet_initial_stride(int(n)))
+
+
+def stratified_ranks(n: int, stride: int = 0) -> np.ndarray:
+    """Position of each of n sorted points in the stratified insertion order."""
+    out = np.zeros(max(n, 1), np.uint32)
+    if lib().wtet_stratified_ranks(int(n), int(stride), _p(out)) != 0:
+        raise ValueError(f"invalid n {n} or stride {stride}")
+    return out[:n]
+
+
+def hilbert_index(cells) -> np.ndarray:
+    """3D Hilbert index of (k, 3) integer cells in 0..255."""
+    c = np.ascontiguousarray(cells, np.uint32).reshape(-1, 3)
+    out = np.zeros(max(len(c), 1), np.uint32)
+    lib().wtet_hilbert_index(_p(c), len(c), _p(out))
+    return out[:len(c)]
 
 
 def predicates_host(kind: int, pts: np.ndarray, ids: np.ndarray | None, mode: int = 0):

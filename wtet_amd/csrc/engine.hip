@@ -353,16 +353,15 @@ void grow_for(wtet_ctx* c, uint64_t need) {
 }
 
 // Morton bits per axis and the bit where the cloud index starts in a key
-void key_layout(uint32_t C, int* kbits, int* end_bit) {
-  if (C <= 1) {
-    *kbits = 21;
-    *end_bit = 63;
-    return;
-  }
+// Key layout, low to high: Morton code (kbits per axis), then in Hilbert
+// mode the 24-bit Hilbert index, then the cloud index.
+void key_layout(uint32_t C, bool hilbert, int* kbits, int* cshift, int* end_bit) {
   int cb = 0;
   while ((1ull << cb) < C) ++cb;
-  *kbits = std::min(21, (63 - cb) / 3);
-  *end_bit = 3 * *kbits + cb;
+  int room = 63 - cb - (hilbert ? 24 : 0);
+  *kbits = std::min(hilbert ? 13 : 21, room / 3);
+  *cshift = 3 * *kbits + (hilbert ? 24 : 0);
+  *end_bit = *cshift + cb;
 }
 
 // The device phase for clouds off[c]..off[c+1] of d_xyz (one cloud: {0, n}).
@@ -377,15 +376,16 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   const uint32_t C = (uint32_t)off.size() - 1;
   const uint32_t n = off[C];
   const uint32_t V = n + 4 * C;  // infinity vertex
-  int kbits = 21, end_bit = 63;
-  key_layout(C, &kbits, &end_bit);
+  const bool hilbert = cfg.enable_hilbert_sorting != 0;
+  int kbits = 21, cshift = 63, end_bit = 63;
+  key_layout(C, hilbert, &kbits, &cshift, &end_bit);
 
   ensure(c, c->xyzn, (size_t)n * 12);
   ensure(c, c->keys, (size_t)n * 8);
   ensure(c, c->ids, (size_t)n * 4);
   ensure(c, c->keys2, (size_t)n * 8);
   ensure(c, c->ids2, (size_t)n * 4);
-  ensure(c, c->sort_tmp, std::max<size_t>(sort_temp_bytes(n, end_bit), 16));
+  ensure(c, c->sort_tmp, std::max<size_t>(std::max(sort_temp_bytes(n, end_bit), sort_temp_bytes(n, 64)), 16));
   ensure(c, c->mm6, (size_t)C * 24);
   ensure(c, c->doff, ((size_t)C + 1) * 4);
   ensure(c, c->cstart, ((size_t)C + 1) * 4);
@@ -417,7 +417,8 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   CL.C = C;
   CL.V = V;
   CL.kbits = kbits;
-  CL.cshift = 3 * kbits;
+  CL.cshift = cshift;
+  CL.hilbert = hilbert ? 1 : 0;
 
   HIP_CHECK(hipEventRecord(c->ev0, s));
   launch_fill_u32(P<uint32_t>(c->ctr), 0u, C_COUNT, s);
@@ -450,6 +451,13 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   read_ctr(c);
   uint32_t U = c->h_ctr[C_NPTS];
   c->stats.duplicates = n - U;
+  if (hilbert) {
+    timed(c, K_PREP, [&] {
+      launch_stratify(P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), U, CL, P<uint64_t>(c->keys),
+                      P<uint64_t>(c->keys2), P<uint32_t>(c->ids2), c->sort_tmp.p, c->sort_tmp.bytes, s);
+    });
+    HIP_CHECK(hipGetLastError());
+  }
   // tet slots for the whole call: fixed 7 per point, the learned
   // prediction, or a small start that the grow-and-retry path extends
   uint64_t slots;
@@ -1373,10 +1381,11 @@ static void features_impl(void* a_) {
   CL.off = P<uint32_t>(c->doff);
   CL.C = C;
   CL.V = n + 4 * C;
-  int kbits, end_bit;
-  key_layout(C, &kbits, &end_bit);
+  int kbits, cshift, end_bit;
+  key_layout(C, false, &kbits, &cshift, &end_bit);
   CL.kbits = kbits;
-  CL.cshift = 3 * kbits;
+  CL.cshift = cshift;
+  CL.hilbert = 0;
   launch_prepare(a.d_xyz, n, CL, P<uint32_t>(c->mm6), P<float>(c->xyzn), P<uint64_t>(c->keys),
                  P<uint32_t>(c->ids), s);
   launch_features(a.d_xyz, n, CL.off, C, P<uint32_t>(c->mm6), P<double>(c->feat_acc),
@@ -1686,6 +1695,22 @@ wtet_status wtet_autotune(wtet_ctx* c, const float* d_xyz, const uint64_t* offse
   c->items = best;
   if (best_items) *best_items = best;
   if (best_ms) *best_ms = bms;
+  return WTET_OK;
+}
+
+uint32_t wtet_initial_stride(uint32_t n) { return initial_stride(n); }
+
+wtet_status wtet_stratified_ranks(uint32_t n, uint32_t stride, uint32_t* rank) {
+  if (!rank || n == 0) return WTET_ERR_INVALID_ARGUMENT;
+  if (stride == 0) stride = initial_stride(n);
+  if (stride & (stride - 1)) return WTET_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; ++i) rank[i] = stratified_rank(i, n, stride);
+  return WTET_OK;
+}
+
+wtet_status wtet_hilbert_index(const uint32_t* xyz, uint32_t n, uint32_t* out) {
+  if (!xyz || !out) return WTET_ERR_INVALID_ARGUMENT;
+  for (uint32_t i = 0; i < n; ++i) out[i] = hilbert3_8(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
   return WTET_OK;
 }
 

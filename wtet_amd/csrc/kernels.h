@@ -37,6 +37,12 @@ void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n,
                             uint32_t* blk, uint32_t* uvert, uint32_t* utet, uint32_t* ctr,
                             hipStream_t s);
 
+// Hilbert mode: reorder the uninserted list into the reference's
+// stratified order, per cloud (scratch: key, key_tmp, val_tmp of U entries)
+void launch_stratify(uint32_t* uvert, uint32_t* utet, uint32_t U, const Clouds& K, uint64_t* key,
+                     uint64_t* key_tmp, uint32_t* val_tmp, void* temp, size_t temp_bytes,
+                     hipStream_t s);
+
 // rounds
 void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                  hipStream_t s);

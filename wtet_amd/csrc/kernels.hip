@@ -285,6 +285,39 @@ void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n,
   k_uninit_scatter<<<nb, kBlock, 0, s>>>(pts, keys, n, K, blk, uvert, utet);
 }
 
+// Reference insertion order of the Hilbert mode: within each cloud the
+// uninserted list follows the stratified order of the sorted points
+// (src/gpu/mod.rs:473-600).  key = (cloud, rank among all the cloud's points);
+// after the sort each entry's start tet is its cloud's super tet again.
+__global__ void k_stratify_keys(const uint32_t* uvert, uint32_t U, Clouds K, uint64_t* key) {
+  uint32_t r = gid();
+  if (r >= U) return;
+  uint32_t v = uvert[r];
+  uint32_t c = K.C > 1 ? cloud_of_vertex(K.off, K.C, v) : 0u;
+  uint32_t nc = K.off[c + 1] - K.off[c];
+  uint32_t jl = v - 4u * c - K.off[c];
+  key[r] = ((uint64_t)c << 32) | stratified_rank(jl, nc, initial_stride(nc));
+}
+
+__global__ void k_list_from_sorted(const uint32_t* sv, uint32_t U, Clouds K, uint32_t* uvert,
+                                   uint32_t* utet) {
+  uint32_t r = gid();
+  if (r >= U) return;
+  uint32_t v = sv[r];
+  uvert[r] = v;
+  utet[r] = 5u * (K.C > 1 ? cloud_of_vertex(K.off, K.C, v) : 0u);
+}
+
+void launch_stratify(uint32_t* uvert, uint32_t* utet, uint32_t U, const Clouds& K, uint64_t* key,
+                     uint64_t* key_tmp, uint32_t* val_tmp, void* temp, size_t temp_bytes,
+                     hipStream_t s) {
+  if (!U) return;
+  k_stratify_keys<<<nblocks(U), kBlock, 0, s>>>(uvert, U, K, key);
+  int bits = 32 + (K.C > 1 ? cloud_key_bits(K.C) : 0);
+  (void)rocprim::radix_sort_pairs(temp, temp_bytes, key, key_tmp, uvert, val_tmp, U, 0, bits, s);
+  k_list_from_sorted<<<nblocks(U), kBlock, 0, s>>>(val_tmp, U, K, uvert, utet);
+}
+
 // ---------------------------------------------------------------------------
 // insertion rounds
 // ---------------------------------------------------------------------------

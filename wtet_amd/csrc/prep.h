@@ -22,7 +22,8 @@ struct Clouds {
   uint32_t C;
   uint32_t V;           // infinity vertex id
   int kbits;            // Morton bits per axis
-  int cshift;           // key bit where the cloud index starts (3 * kbits)
+  int cshift;           // key bit where the cloud index starts
+  int hilbert;          // 1: keys lead with the reference's 8-bit Hilbert index
 };
 
 // largest c < C with off[c] <= i (input index)
@@ -72,6 +73,67 @@ WT_HD uint64_t expand21(uint64_t v) {
   return v;
 }
 
+// 3D Hilbert index of an 8-bit-per-axis cell (J. Skilling, "Programming the
+// Hilbert curve", AIP Conf. Proc. 707, 2004: axes to transposed index, then
+// the transposed bits interleaved, x first).  Consecutive indices are
+// face-neighbour cells.
+WT_HD uint32_t hilbert3_8(uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t X[3] = {x & 255u, y & 255u, z & 255u};
+  // undo excess work
+  for (uint32_t Q = 128u; Q > 1u; Q >>= 1) {
+    uint32_t P = Q - 1u;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  // Gray encode
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0u;
+  for (uint32_t Q = 128u; Q > 1u; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1u;
+  X[0] ^= t;
+  X[1] ^= t;
+  X[2] ^= t;
+  uint32_t h = 0u;
+  for (int b = 7; b >= 0; --b)
+    for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+  return h;
+}
+
+// The reference's first insertion stride for n points (power of two in
+// [64, 16384] from sqrt(n), doubled above 1e5 points;
+// src/gpu/mod.rs:533-548), from integer arithmetic: the smallest power of
+// two >= the clamped scale.
+WT_HD uint32_t initial_stride(uint32_t n) {
+  // scale = sqrt(n) (n <= 1e5) or 2 sqrt(n); stride = 2^ceil(log2(clamp(scale, 64, 16384)))
+  // 2^k >= c*sqrt(n)  <=>  4^k >= c^2 n
+  uint64_t c2n = (n > 100000u ? 4ull : 1ull) * (uint64_t)n;
+  uint32_t s = 64u;
+  while (s < 16384u && (uint64_t)s * s < c2n) s <<= 1;
+  return s;
+}
+
+// Position of sorted element i among n in the reference's stratified order
+// with first stride S (src/gpu/mod.rs:573-600): passes of strides S, S/2,
+// .., 1 take the not yet taken multiples of the stride, in index order.
+WT_HD uint32_t stratified_rank(uint32_t i, uint32_t n, uint32_t S) {
+  if (i % S == 0u) return i / S;
+  uint32_t before = (n + S - 1u) / S;  // pass 0
+  for (uint32_t s = S >> 1; s >= 1u; s >>= 1) {
+    // pass of stride s: multiples of s that are not multiples of 2s
+    if (i % s == 0u) return before + (i / s - 1u) / 2u;
+    before += (n + s - 1u) / s - (n + 2u * s - 1u) / (2u * s);
+  }
+  return before;  // unreachable: every i is a multiple of 1
+}
+
 // Normalize input point i of its cloud (f32 sub then div, scale floor
 // 1e-10: src/lib.rs:313-337) and build its sort key (cloud, Morton code).
 // mm holds per cloud the ordered-u32 min (3) and max (3) of each axis.
@@ -97,9 +159,25 @@ WT_HD void normalize_body(const float* xyz, uint32_t i, const Clouds& K, const u
     float cl = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
     q[k] = (uint32_t)(cl * qmax);
   }
-  // the Morton order is for memory locality only (the insertion order
-  // cannot change the triangulation)
-  uint64_t key = (expand21(q[2]) << 2) | (expand21(q[1]) << 1) | expand21(q[0]);
+  uint64_t key;
+  if (K.hilbert) {
+    // the reference's Hilbert keys: each axis over its own extent, rounded
+    // to 0..255 (src/hilbert.rs:20-38)
+    uint32_t h[3];
+    for (int k = 0; k < 3; ++k) {
+      float u = ext[k] > 0.0f ? (xyz[3 * (size_t)i + k] - mn[k]) / ext[k] : 0.0f;
+      u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+      h[k] = (uint32_t)rintf(u * 255.0f);
+    }
+    // a fine Morton code below the 24-bit index keeps runs of equal keys
+    // down to (nearly) equal coordinates, as the duplicate scan needs
+    key = ((uint64_t)hilbert3_8(h[0], h[1], h[2]) << (3 * K.kbits)) |
+          (expand21(q[2]) << 2) | (expand21(q[1]) << 1) | expand21(q[0]);
+  } else {
+    // the Morton order is for memory locality only (the insertion order
+    // cannot change the triangulation)
+    key = (expand21(q[2]) << 2) | (expand21(q[1]) << 1) | expand21(q[0]);
+  }
   if (K.C > 1) key |= (uint64_t)c << K.cshift;
   keys[i] = key;
   ids[i] = i;
