@@ -275,6 +275,72 @@ wtet_status wtet_set_profiling(wtet_ctx *ctx, int on);
 wtet_status wtet_kernel_times(const wtet_ctx *ctx, uint32_t kind, const char **name,
                               uint32_t *launches, double *total_ms);
 
+/* Stage-level entry points: the loop of phase1::run (src/phase1.rs:16-1431)
+ * driven one dispatch group at a time by the caller, as the reference's own
+ * tests drive phase1::run and GpuState::dispatch_* (src/gpu/dispatch.rs).
+ * One session per context, one cloud; any other triangulation call on the
+ * context ends it.  The order within a round is vote, split, split_points,
+ * then any number of flip iterations, then relocate before the next vote
+ * (a wrong order returns WTET_ERR_INVALID_ARGUMENT).  Each call fills *info
+ * (may be null) with the state after it. */
+typedef struct wtet_stage_info {
+  uint32_t round;      /* insertion rounds begun                             */
+  uint32_t uninserted; /* points not yet inserted                            */
+  uint32_t winners;    /* insertions of the current round                   */
+  uint32_t tet_slots;  /* tet slots in use (alive or dead)                  */
+  uint32_t active;     /* tets queued for the next flip check               */
+  uint32_t checked;    /* tets checked by the last flip iteration           */
+  uint32_t flips;      /* flips executed by the last flip iteration         */
+  uint32_t violations; /* non-Delaunay faces found by the last gather       */
+} wtet_stage_info;
+
+/* GpuState::new + sort + dispatch_init (src/gpu/mod.rs:48-204, :422-600,
+ * src/gpu/dispatch.rs:33, init.wgsl:87-148): upload and normalize the
+ * points, order them, five start tets, every point uninserted. */
+wtet_status wtet_stage_init(wtet_ctx *ctx, const float *xyz, uint32_t n, const wtet_config *cfg,
+                            wtet_stage_info *info);
+/* dispatch_reset_votes + dispatch_vote + dispatch_pick_winner +
+ * dispatch_build_insert_list (src/phase1.rs:113-166) and the compaction of
+ * the uninserted list (dispatch_compact_vertex_arrays*, :1343-1352):
+ * info->winners points are chosen, the rest stay uninserted. */
+wtet_status wtet_stage_vote(wtet_ctx *ctx, wtet_stage_info *info);
+/* dispatch_split (src/phase1.rs:305, split.wgsl:104-327): 1 -> 4 splits of
+ * the winners' tets; the children are queued for the flip check. */
+wtet_status wtet_stage_split(wtet_ctx *ctx, wtet_stage_info *info);
+/* dispatch_split_points (src/phase1.rs:318, split_points.wgsl:222-335):
+ * every uninserted point of a split tet moves into the child holding it. */
+wtet_status wtet_stage_split_points(wtet_ctx *ctx, wtet_stage_info *info);
+/* One flip iteration (src/phase1.rs:584-674 fast, :767-916 exact):
+ * check the queued tets, vote, execute the accepted 2-3 / 3-2 flips,
+ * rewire adjacency, queue the new tets.  predicate_mode -1: the session's;
+ * 0 staged FP32 -> FP64 -> exact; 1 exact only (the decisions are the same). */
+wtet_status wtet_stage_flip(wtet_ctx *ctx, int predicate_mode, wtet_stage_info *info);
+/* dispatch_init_tet_to_flip + dispatch_update_flip_trace +
+ * dispatch_relocate_points_fast (src/phase1.rs:1086-1104): uninserted
+ * points of flipped tets to the tets now holding them. */
+wtet_status wtet_stage_relocate(wtet_ctx *ctx, wtet_stage_info *info);
+/* dispatch_gather (src/phase1.rs:1402, gather.wgsl:68-151): exact audit of
+ * every finite face of the current mesh; info->violations. */
+wtet_status wtet_stage_gather(wtet_ctx *ctx, wtet_stage_info *info);
+/* GpuState::readback (src/gpu/mod.rs:207-332) of the current mesh: the
+ * live tets in slot order as wtet_result rows (input ids, n..n+3 super,
+ * n+4 infinity, (row << 5) | face adjacency); failed_verts from the last
+ * gather if the mesh has not changed since.  No repair.  Free with
+ * wtet_result_free. */
+wtet_status wtet_stage_readback(wtet_ctx *ctx, wtet_result *out);
+/* Where each uninserted point is: point_ids[i] (input id) lies in row
+ * rows[i] of a readback of the same mesh state (rows follow slot order).
+ * Valid after split_points or relocate; *count receives the number of
+ * points (at most cap are written). */
+wtet_status wtet_stage_locations(wtet_ctx *ctx, uint32_t cap, uint32_t *point_ids, uint32_t *rows,
+                                 uint32_t *count);
+/* The end of delaunay_3d (src/lib.rs:178-310) once every point is in:
+ * remaining flip iterations, gather, readback, host repair; *out equals the
+ * wtet_delaunay result for the same input.  Ends the session. */
+wtet_status wtet_stage_finish(wtet_ctx *ctx, wtet_result *out);
+/* Ends the session (no-op without one). */
+wtet_status wtet_stage_end(wtet_ctx *ctx);
+
 /* Host phase 2 alone (src/lib.rs:227-307: cpu_flip_violations ->
  * full_bw_reconstruction -> splay, src/phase2.rs): turns a valid
  * triangulation of the n normalized points (ids 0..n-1, super vertices

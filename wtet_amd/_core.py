@@ -121,6 +121,11 @@ class _Stats(ctypes.Structure):
     ]
 
 
+class _StageInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in ("round", "uninserted", "winners", "tet_slots", "active",
+                                              "checked", "flips", "violations")]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -189,6 +194,15 @@ def lib():
             L.wtet_repair.argtypes = [P, u32, ctypes.POINTER(ctypes.POINTER(u32)),
                                       ctypes.POINTER(ctypes.POINTER(u32)), ctypes.POINTER(u32),
                                       ctypes.c_int, P]
+            SI = ctypes.POINTER(_StageInfo)
+            L.wtet_stage_init.argtypes = [P, P, u32, ctypes.POINTER(_Config), SI]
+            for name in ("vote", "split", "split_points", "relocate", "gather"):
+                getattr(L, "wtet_stage_" + name).argtypes = [P, SI]
+            L.wtet_stage_flip.argtypes = [P, ctypes.c_int, SI]
+            L.wtet_stage_readback.argtypes = [P, ctypes.POINTER(_Result)]
+            L.wtet_stage_locations.argtypes = [P, u32, P, P, ctypes.POINTER(u32)]
+            L.wtet_stage_finish.argtypes = [P, ctypes.POINTER(_Result)]
+            L.wtet_stage_end.argtypes = [P]
             _lib = L
         return _lib
 
@@ -372,6 +386,10 @@ class Context:
             if st in (1, 2):
                 raise ValueError(msg)
             raise RuntimeError(f"wtet_delaunay failed ({_STATUS.get(st, st)}): {msg}")
+        return self._take(res)
+
+    def _take(self, res) -> "DelaunayResult":
+        """Copies a wtet_result into numpy arrays and frees it."""
         try:
             m = res.num_tets
             tets = np.ctypeslib.as_array(res.tets, shape=(m * 4,)).copy().reshape(m, 4) if m else \
@@ -384,6 +402,79 @@ class Context:
         finally:
             lib().wtet_result_free(ctypes.byref(res))
         return DelaunayResult(tets, adj, failed, self.stats())
+
+    # -- stage-level calls (one step of the device loop at a time) ---------------
+
+    def _stage(self, name, fn, *args) -> dict:
+        info = _StageInfo()
+        st = fn(self._h, *args, ctypes.byref(info))
+        if st != 0:
+            msg = self.error()
+            if st in (1, 2):
+                raise ValueError(msg)
+            raise RuntimeError(f"wtet_stage_{name} failed ({_STATUS.get(st, st)}): {msg}")
+        return {k: getattr(info, k) for k, _ in _StageInfo._fields_}
+
+    def stage_init(self, points, config: "Config | None" = None) -> dict:
+        """Starts a stage session: points uploaded, normalized and ordered, five
+        start tets, every point uninserted (GpuState::new + dispatch_init)."""
+        pts = _validate(points)
+        self._prepare(config)
+        c = config._c() if config is not None else Config()._c()
+        return self._stage("init", lib().wtet_stage_init, _p(pts), len(pts), ctypes.byref(c))
+
+    def stage_vote(self) -> dict:
+        return self._stage("vote", lib().wtet_stage_vote)
+
+    def stage_split(self) -> dict:
+        return self._stage("split", lib().wtet_stage_split)
+
+    def stage_split_points(self) -> dict:
+        return self._stage("split_points", lib().wtet_stage_split_points)
+
+    def stage_flip(self, predicate_mode: int = -1) -> dict:
+        """One flip iteration; predicate_mode -1 the session's, 0 staged, 1 exact."""
+        return self._stage("flip", lib().wtet_stage_flip, int(predicate_mode))
+
+    def stage_relocate(self) -> dict:
+        return self._stage("relocate", lib().wtet_stage_relocate)
+
+    def stage_gather(self) -> dict:
+        return self._stage("gather", lib().wtet_stage_gather)
+
+    def stage_readback(self) -> "DelaunayResult":
+        """The current mesh (live tets in slot order, no repair)."""
+        res = _Result()
+        st = lib().wtet_stage_readback(self._h, ctypes.byref(res))
+        if st != 0:
+            raise (ValueError if st in (1, 2) else RuntimeError)(self.error())
+        return self._take(res)
+
+    def stage_locations(self):
+        """(point_ids, rows): uninserted input ids and the readback rows of
+        the tets holding them."""
+        cnt = ctypes.c_uint32(0)
+        st = lib().wtet_stage_locations(self._h, 0, None, None, ctypes.byref(cnt))
+        if st != 0:
+            raise (ValueError if st in (1, 2) else RuntimeError)(self.error())
+        ids = np.zeros(cnt.value, np.uint32)
+        rows = np.zeros(cnt.value, np.uint32)
+        if cnt.value:
+            st = lib().wtet_stage_locations(self._h, cnt.value, _p(ids), _p(rows), ctypes.byref(cnt))
+            if st != 0:
+                raise (ValueError if st in (1, 2) else RuntimeError)(self.error())
+        return ids, rows
+
+    def stage_finish(self) -> "DelaunayResult":
+        """Remaining flips, gather, readback and host repair; ends the session."""
+        res = _Result()
+        st = lib().wtet_stage_finish(self._h, ctypes.byref(res))
+        if st != 0:
+            raise (ValueError if st in (1, 2) else RuntimeError)(self.error())
+        return self._take(res)
+
+    def stage_end(self):
+        lib().wtet_stage_end(self._h)
 
     def delaunay(self, points: np.ndarray, config: "Config | None" = None) -> "DelaunayResult":
         pts = _validate(points)

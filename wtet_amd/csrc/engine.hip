@@ -36,6 +36,8 @@ struct DevBuf {
 
 }  // namespace
 
+struct wtet_stage_session;
+
 struct wtet_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -68,6 +70,8 @@ struct wtet_ctx {
   };
   std::vector<GraphEntry> graphs;  // one per (buffers, grid, iterations)
   uint32_t graph_captures = 0;
+  // stage-level session (wtet_stage_*), null when none
+  wtet_stage_session* stage = nullptr;
   // counters
   DevBuf ctr, gstats, patch;
   uint32_t* h_ctr = nullptr;  // pinned mirror of the counters
@@ -364,28 +368,58 @@ void key_layout(uint32_t C, bool hilbert, int* kbits, int* cshift, int* end_bit)
   *end_bit = *cshift + cb;
 }
 
-// The device phase for clouds off[c]..off[c+1] of d_xyz (one cloud: {0, n}).
-// Leaves the rows in prop/fout (cloud c from cstart[c]), the violation
-// records as rows, and -- unless resident with no violation -- a copy of the
-// rows in the pinned staging buffers.
-void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& off,
-                const wtet_config& cfg, Counters& K, std::vector<uint32_t>& viol_rec,
-                uint32_t& nviol, bool resident, uint32_t& mout, std::vector<uint32_t>& cstart) {
-  hipStream_t s = c->stream;
-  int mode = cfg.predicate_mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
-  const uint32_t C = (uint32_t)off.size() - 1;
-  const uint32_t n = off[C];
-  const uint32_t V = n + 4 * C;  // infinity vertex
-  const bool hilbert = cfg.enable_hilbert_sorting != 0;
-  int kbits = 21, cshift = 63, end_bit = 63;
-  key_layout(C, hilbert, &kbits, &cshift, &end_bit);
+void end_stage(wtet_ctx* c);
 
+// State of one device phase over clouds off[c]..off[c+1] (one cloud: {0, n}):
+// run_device drives it start to end; the stage entry points drive it one
+// step at a time.
+struct Run {
+  wtet_config cfg{};
+  std::vector<uint32_t> off;
+  uint32_t C = 0, n = 0, V = 0;  // clouds, points, infinity vertex
+  int mode = PRED_FILTERED;
+  bool hilbert = false, learned = false, flipping = true;
+  int kbits = 21, cshift = 63, end_bit = 63;
+  Clouds CL{};
+  uint64_t* st = nullptr;  // predicate stage counters (collect_stats)
+  uint32_t cur = 0;        // uninserted list buffer in use
+  uint32_t round = 0;
+  uint32_t U = 0;          // uninserted points
+  uint32_t W = 0;          // insertions of the current round
+  uint64_t dsum_prev = 0;  // deferred-check total at the previous round
+  int max_fi = 10;
+  Counters K;
+};
+
+void run_setup(wtet_ctx* c, Run& R, const std::vector<uint32_t>& off, const wtet_config& cfg) {
+  R = Run{};
+  R.cfg = cfg;
+  R.off = off;
+  R.mode = cfg.predicate_mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
+  R.C = (uint32_t)off.size() - 1;
+  R.n = off[R.C];
+  R.V = R.n + 4 * R.C;
+  R.hilbert = cfg.enable_hilbert_sorting != 0;
+  R.learned = cfg.presize_mode == 1;
+  R.flipping = cfg.enable_flipping != 0;
+  R.max_fi = (int)std::min<uint32_t>(std::max<uint32_t>(cfg.max_flip_iterations, 1u), 64u);
+  key_layout(R.C, R.hilbert, &R.kbits, &R.cshift, &R.end_bit);
+  R.st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
+}
+
+// Buffers, normalization, keys, sort, vertex layout, uninserted list,
+// presizing and the start tets (src/gpu/mod.rs:48-204, src/lib.rs:313-337,
+// init.wgsl:87-148).
+void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
+  hipStream_t s = c->stream;
+  const uint32_t C = R.C, n = R.n, V = R.V;
+  const std::vector<uint32_t>& off = R.off;
   ensure(c, c->xyzn, (size_t)n * 12);
   ensure(c, c->keys, (size_t)n * 8);
   ensure(c, c->ids, (size_t)n * 4);
   ensure(c, c->keys2, (size_t)n * 8);
   ensure(c, c->ids2, (size_t)n * 4);
-  ensure(c, c->sort_tmp, std::max<size_t>(std::max(sort_temp_bytes(n, end_bit), sort_temp_bytes(n, 64)), 16));
+  ensure(c, c->sort_tmp, std::max<size_t>(std::max(sort_temp_bytes(n, R.end_bit), sort_temp_bytes(n, 64)), 16));
   ensure(c, c->mm6, (size_t)C * 24);
   ensure(c, c->doff, ((size_t)C + 1) * 4);
   ensure(c, c->cstart, ((size_t)C + 1) * 4);
@@ -401,24 +435,23 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   ensure(c, c->isums, 64);
   ensure(c, c->rec, (1u << 20) * 4);
   ensure(c, c->blk, ((size_t)nblocks(n) + 2) * 4);
-  const bool learned = cfg.presize_mode == 1;
-  if (learned) {
+  if (R.learned) {
     if (!c->presize_loaded) throw Fail{WTET_ERR_INVALID_ARGUMENT, "presize_mode 1 needs a model (wtet_set_presize_model)"};
     ensure(c, c->feat_acc, (size_t)C * (kHistBins / 2 + 9) * 8 + 64);
     ensure(c, c->feat, (size_t)C * kFeat * 4);
     ensure(c, c->pred, (size_t)C * kOut * 4);
     c->h_pred.resize((size_t)C * kOut);
   }
-  uint64_t* st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
+  R.st = R.cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
 
   HIP_CHECK(hipMemcpyAsync(c->doff.p, off.data(), ((size_t)C + 1) * 4, hipMemcpyHostToDevice, s));
-  Clouds CL;
+  Clouds& CL = R.CL;
   CL.off = P<uint32_t>(c->doff);
   CL.C = C;
   CL.V = V;
-  CL.kbits = kbits;
-  CL.cshift = cshift;
-  CL.hilbert = hilbert ? 1 : 0;
+  CL.kbits = R.kbits;
+  CL.cshift = R.cshift;
+  CL.hilbert = R.hilbert ? 1 : 0;
 
   HIP_CHECK(hipEventRecord(c->ev0, s));
   launch_fill_u32(P<uint32_t>(c->ctr), 0u, C_COUNT, s);
@@ -429,7 +462,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
     launch_prepare(d_xyz, n, CL, P<uint32_t>(c->mm6), P<float>(c->xyzn), P<uint64_t>(c->keys),
                    P<uint32_t>(c->ids), s);
     sort_pairs(P<uint64_t>(c->keys), P<uint32_t>(c->ids), P<uint64_t>(c->keys2),
-               P<uint32_t>(c->ids2), n, end_bit, c->sort_tmp.p, c->sort_tmp.bytes, s);
+               P<uint32_t>(c->ids2), n, R.end_bit, c->sort_tmp.p, c->sort_tmp.bytes, s);
     launch_gather_points(P<float>(c->xyzn), P<uint32_t>(c->ids2), P<uint64_t>(c->keys2), n, CL,
                          P<float4>(c->pts), s);
     // uninserted list: every point (minus redundant duplicates) in its
@@ -437,7 +470,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
     launch_init_uninserted(P<float4>(c->pts), P<uint64_t>(c->keys2), n, CL, P<uint32_t>(c->blk),
                            P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), P<uint32_t>(c->ctr),
                            s);
-    if (learned) {
+    if (R.learned) {
       // features from the raw points and this call's per-cloud boxes, then
       // the network's predictions (log space, margin included)
       launch_features(d_xyz, n, CL.off, C, P<uint32_t>(c->mm6), P<double>(c->feat_acc),
@@ -449,11 +482,11 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   });
   HIP_CHECK(hipGetLastError());
   read_ctr(c);
-  uint32_t U = c->h_ctr[C_NPTS];
-  c->stats.duplicates = n - U;
-  if (hilbert) {
+  R.U = c->h_ctr[C_NPTS];
+  c->stats.duplicates = n - R.U;
+  if (R.hilbert) {
     timed(c, K_PREP, [&] {
-      launch_stratify(P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), U, CL, P<uint64_t>(c->keys),
+      launch_stratify(P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), R.U, CL, P<uint64_t>(c->keys),
                       P<uint64_t>(c->keys2), P<uint32_t>(c->ids2), c->sort_tmp.p, c->sort_tmp.bytes, s);
     });
     HIP_CHECK(hipGetLastError());
@@ -461,7 +494,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   // tet slots for the whole call: fixed 7 per point, the learned
   // prediction, or a small start that the grow-and-retry path extends
   uint64_t slots;
-  if (learned) {
+  if (R.learned) {
     double sum = 0.0;
     for (uint32_t cl = 0; cl < C; ++cl) {
       double lp = std::min<double>(c->h_pred[(size_t)cl * kOut], 4.0);  // <= e^4 slots per point
@@ -469,7 +502,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
     }
     for (int k = 0; k < 3; ++k) c->stats.presize_pred[k] = C == 1 ? c->h_pred[k] : 0.0f;
     slots = (uint64_t)std::ceil(sum) + 5ull * C + 1024;
-  } else if (cfg.presize_mode == 2) {
+  } else if (R.cfg.presize_mode == 2) {
     slots = 2ull * n + 4096 + 5ull * C;
   } else {
     // the reference's fixed formula per cloud (src/gpu/mod.rs:101-108)
@@ -482,112 +515,154 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   c->stats.presize_slots = slots;
   ensure_tets(c, slots, false);
   c->stats.tet_capacity = c->cap_tets;
-  launch_start_tets(mesh_of(c, V, mode), CL, s);
+  launch_start_tets(mesh_of(c, V, R.mode), CL, s);
   launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u * C, -1, 0u, s);
   HIP_CHECK(hipGetLastError());
-
-  uint32_t cur = 0, round = 0;
-  uint64_t dsum_prev = 0;
-  uint32_t max_rounds = std::max<uint32_t>(cfg.max_insert_iterations, 1u);
-  int max_fi = (int)std::min<uint32_t>(std::max<uint32_t>(cfg.max_flip_iterations, 1u), 64u);
-  bool flipping = cfg.enable_flipping != 0;
   launch_fill_u64(P<uint64_t>(c->isums), 0ull, 4, s);
   launch_set_ctr(P<uint32_t>(c->ctr), C_CAP, (uint32_t)c->cap_tets, -1, 0u, s);
-  while (U > 0) {
-    ++round;
-    if (round > std::max<uint32_t>(max_rounds, 4096u))
-      throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
-    Mesh M = mesh_of(c, V, mode);
-    launch_set_ctr(M.ctr, C_ULEN, U, C_ROUND, round, s);
-    timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
-    timed(c, K_PICK, [&] {
-      launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round,
-                  P<uint2>(c->wlist), P<uint32_t>(c->uvert[cur ^ 1]),
-                  P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), c->ring, s);
-    });
-    HIP_CHECK(hipGetLastError());
-    // the one host read of a round: winners, reserved slots, and whether the
-    // previous round's flips stopped for capacity
-    read_ctr(c);
-    uint32_t W = c->h_ctr[C_WIN];
-    if (c->h_ctr[C_BADTET]) throw Fail{WTET_ERR_INTERNAL, "a point was located outside the super tet"};
-    if (W == 0) throw Fail{WTET_ERR_INTERNAL, "no insertion possible in a round"};
-    uint64_t need = (uint64_t)c->h_ctr[C_TETS] + 64 + (c->h_ctr[C_STOP] ? c->h_ctr[C_A] : 0u);
-    if (need > c->cap_tets || c->h_ctr[C_STOP]) {
-      grow_for(c, need);
-      M = mesh_of(c, V, mode);
-    }
-    K.sumU += U;
-    K.rounds++;
-    timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), W, round, lists_of(c), s); });
-    U -= W;
-    cur ^= 1;
-    timed(c, K_SPLIT_POINTS, [&] {
-      launch_split_points(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st,
-                          P<uint32_t>(c->slow_p), s);
-    });
-    HIP_CHECK(hipGetLastError());
-    if (flipping) {
-      // the first list holds the entries carried over plus four per split;
-      // the deferred count per iteration follows the previous round's
-      uint64_t dsum = c->h_ctr[C_SLOW_SUM];
-      uint64_t dper = (2 * (dsum - dsum_prev) + max_fi - 1) / (uint64_t)max_fi;
-      dsum_prev = dsum;
-      run_flip_batch(c, V, mode, st, max_fi, (uint64_t)c->h_ctr[C_A] + 4ull * W, dper);
-      if (U > 0) {
-        timed(c, K_RELOCATE, [&] {
-          launch_relocate(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round, st,
-                          P<uint32_t>(c->slow_p), s);
-        });
-        HIP_CHECK(hipGetLastError());
-      }
-    } else {
-      launch_set_ctr(M.ctr, C_A, 0u, -1, 0u, s);
-    }
-  }
-  // finish flipping: batches of iterations until the active list is empty
-  if (flipping) {
-    for (uint32_t guard = 0;; ++guard) {
-      if (guard > 20000) throw Fail{WTET_ERR_NOT_CONVERGED, "flipping did not terminate"};
-      read_ctr(c);
-      if (c->h_ctr[C_STOP]) grow_for(c, (uint64_t)c->h_ctr[C_TETS] + c->h_ctr[C_A] + 64);
-      else if (c->h_ctr[C_A] == 0) break;
-      run_flip_batch(c, V, mode, st, 16, c->h_ctr[C_A], c->h_ctr[C_A] / 64);
-    }
-  }
-  uint64_t sums[4] = {0, 0, 0, 0};
-  HIP_CHECK(hipMemcpyAsync(sums, c->isums.p, 32, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  K.sumA = sums[0];
-  K.sumF = sums[1];
-  K.iters = (uint32_t)sums[2];
-  // audit every finite face
+}
+
+// One insertion round's vote, winner pick and compaction of the survivors
+// (vote.wgsl:89-344, compact_vertex_arrays_*.wgsl): returns the round's
+// insertions; the survivors become the uninserted list.
+uint32_t dev_vote(wtet_ctx* c, Run& R) {
+  hipStream_t s = c->stream;
+  ++R.round;
+  if (R.round > std::max<uint32_t>(R.cfg.max_insert_iterations, 4096u))
+    throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
+  Mesh M = mesh_of(c, R.V, R.mode);
+  const uint32_t U = R.U, cur = R.cur, round = R.round;
+  launch_set_ctr(M.ctr, C_ULEN, U, C_ROUND, round, s);
+  timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
+  timed(c, K_PICK, [&] {
+    launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round,
+                P<uint2>(c->wlist), P<uint32_t>(c->uvert[cur ^ 1]),
+                P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), c->ring, s);
+  });
+  HIP_CHECK(hipGetLastError());
+  // the one host read of a round: winners, reserved slots, and whether the
+  // previous round's flips stopped for capacity
   read_ctr(c);
-  uint32_t T = c->h_ctr[C_TETS];
-  Mesh M = mesh_of(c, V, mode);
-  launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, s);
-  const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
-  timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, st, s); });
-  // compaction, grouped by cloud
+  uint32_t W = c->h_ctr[C_WIN];
+  if (c->h_ctr[C_BADTET]) throw Fail{WTET_ERR_INTERNAL, "a point was located outside the super tet"};
+  if (W == 0) throw Fail{WTET_ERR_INTERNAL, "no insertion possible in a round"};
+  uint64_t need = (uint64_t)c->h_ctr[C_TETS] + 64 + (c->h_ctr[C_STOP] ? c->h_ctr[C_A] : 0u);
+  if (need > c->cap_tets || c->h_ctr[C_STOP]) grow_for(c, need);
+  R.K.sumU += U;
+  R.K.rounds++;
+  R.W = W;
+  R.U = U - W;
+  R.cur = cur ^ 1;
+  return W;
+}
+
+// 1 -> 4 splits of the round's winners (split.wgsl:104-327); the children
+// join the active list.
+void dev_split(wtet_ctx* c, Run& R) {
+  Mesh M = mesh_of(c, R.V, R.mode);
+  timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), R.W, R.round, lists_of(c), c->stream); });
+  HIP_CHECK(hipGetLastError());
+}
+
+// each uninserted point of a split tet into its child (split_points.wgsl:222-335)
+void dev_split_points(wtet_ctx* c, Run& R) {
+  Mesh M = mesh_of(c, R.V, R.mode);
+  timed(c, K_SPLIT_POINTS, [&] {
+    launch_split_points(M, P<uint32_t>(c->uvert[R.cur]), P<uint32_t>(c->utet[R.cur]), R.U, R.round,
+                        R.st, P<uint32_t>(c->slow_p), c->stream);
+  });
+  HIP_CHECK(hipGetLastError());
+}
+
+// the round's flip iterations (src/phase1.rs, flip loop), from device counters
+void dev_round_flips(wtet_ctx* c, Run& R) {
+  // the first list holds the entries carried over plus four per split;
+  // the deferred count per iteration follows the previous round's
+  uint64_t dsum = c->h_ctr[C_SLOW_SUM];
+  uint64_t dper = (2 * (dsum - R.dsum_prev) + R.max_fi - 1) / (uint64_t)R.max_fi;
+  R.dsum_prev = dsum;
+  run_flip_batch(c, R.V, R.mode, R.st, R.max_fi, (uint64_t)c->h_ctr[C_A] + 4ull * R.W, dper);
+}
+
+// uninserted points of flipped tets to their new tets (relocate_points_fast.wgsl)
+void dev_relocate(wtet_ctx* c, Run& R) {
+  if (R.U == 0) return;
+  Mesh M = mesh_of(c, R.V, R.mode);
+  timed(c, K_RELOCATE, [&] {
+    launch_relocate(M, P<uint32_t>(c->uvert[R.cur]), P<uint32_t>(c->utet[R.cur]), R.U, R.round, R.st,
+                    P<uint32_t>(c->slow_p), c->stream);
+  });
+  HIP_CHECK(hipGetLastError());
+}
+
+// batches of flip iterations until the active list is empty
+void dev_finish_flips(wtet_ctx* c, Run& R) {
+  for (uint32_t guard = 0;; ++guard) {
+    if (guard > 20000) throw Fail{WTET_ERR_NOT_CONVERGED, "flipping did not terminate"};
+    read_ctr(c);
+    if (c->h_ctr[C_STOP]) grow_for(c, (uint64_t)c->h_ctr[C_TETS] + c->h_ctr[C_A] + 64);
+    else if (c->h_ctr[C_A] == 0) break;
+    run_flip_batch(c, R.V, R.mode, R.st, 16, c->h_ctr[C_A], c->h_ctr[C_A] / 64);
+  }
+}
+
+// compaction of the live tets into rows grouped by cloud (src/gpu/mod.rs:207-332);
+// scratch: ord[0], ord[1] and, for many clouds, ord_tmp plus act[0], act[1]
+// (the flip lists, empty once flipping has finished).  out=false computes
+// only slot2idx and cstart.
+void dev_compact(wtet_ctx* c, Run& R, uint32_t T, bool out, bool keep_lists) {
   OutputBufs OB = {};
   OB.blk = P<uint32_t>(c->blk);
-  OB.olist = P<uint32_t>(c->act[0]);
-  OB.okey = P<uint32_t>(c->act[1]);
   OB.slot2pos = P<uint32_t>(c->slot2idx);
   OB.cstart = P<uint32_t>(c->cstart);
-  if (C > 1) {
+  if (keep_lists) {
+    if (R.C > 1) throw Fail{WTET_ERR_INVALID_ARGUMENT, "a mid-run readback covers one cloud"};
+    ensure(c, c->ord[0], (size_t)c->cap_tets * 4);
+    OB.olist = P<uint32_t>(c->ord[0]);
+  } else {
+    OB.olist = P<uint32_t>(c->act[0]);
+    OB.okey = P<uint32_t>(c->act[1]);
+  }
+  if (R.C > 1) {
     ensure(c, c->ord[0], (size_t)c->cap_tets * 4);
     ensure(c, c->ord[1], (size_t)c->cap_tets * 4);
-    ensure(c, c->ord_tmp, std::max<size_t>(sort_temp_bytes_u32(T, cloud_key_bits(C)), 16));
+    ensure(c, c->ord_tmp, std::max<size_t>(sort_temp_bytes_u32(T, cloud_key_bits(R.C)), 16));
     OB.olist2 = P<uint32_t>(c->ord[0]);
     OB.okey2 = P<uint32_t>(c->ord[1]);
     OB.temp = c->ord_tmp.p;
     OB.temp_bytes = c->ord_tmp.bytes;
   }
-  OB.out_tets = P<uint32_t>(c->prop);
-  OB.out_adj = P<uint32_t>(c->fout);
-  timed(c, K_OUTPUT, [&] { launch_output(M, T, CL, OB, s); });
+  if (out) {
+    OB.out_tets = P<uint32_t>(c->prop);
+    OB.out_adj = P<uint32_t>(c->fout);
+  }
+  Mesh M = mesh_of(c, R.V, R.mode);
+  timed(c, K_OUTPUT, [&] { launch_output(M, T, R.CL, OB, c->stream); });
   HIP_CHECK(hipGetLastError());
+}
+
+// Audit of every finite face (gather.wgsl:68-151), compaction into rows,
+// and the copies the host needs.  Leaves the rows in prop/fout (cloud c
+// from cstart[c]), the violation records as rows, and -- unless resident --
+// a copy of the rows in the pinned staging buffers.
+void dev_output(wtet_ctx* c, Run& R, std::vector<uint32_t>& viol_rec, uint32_t& nviol, bool resident,
+                uint32_t& mout, std::vector<uint32_t>& cstart) {
+  hipStream_t s = c->stream;
+  const uint32_t C = R.C;
+  uint64_t sums[4] = {0, 0, 0, 0};
+  HIP_CHECK(hipMemcpyAsync(sums, c->isums.p, 32, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  R.K.sumA = sums[0];
+  R.K.sumF = sums[1];
+  R.K.iters = (uint32_t)sums[2];
+  // audit every finite face
+  read_ctr(c);
+  uint32_t T = c->h_ctr[C_TETS];
+  Mesh M = mesh_of(c, R.V, R.mode);
+  launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, s);
+  const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
+  timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, R.st, s); });
+  dev_compact(c, R, T, true, false);
   HIP_CHECK(hipEventRecord(c->ev1, s));
   cstart.resize((size_t)C + 1);
   HIP_CHECK(hipMemcpyAsync(cstart.data(), c->cstart.p, ((size_t)C + 1) * 4, hipMemcpyDeviceToHost, s));
@@ -597,7 +672,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->stats.ms_device = ms;
   c->stats.walk_failures = c->h_ctr[C_WALK_FAIL];
-  K.peakA = c->h_ctr[C_PEAKA];
+  R.K.peakA = c->h_ctr[C_PEAKA];
   if (c->h_ctr[C_WALK_FAIL]) throw Fail{WTET_ERR_INTERNAL, "point relocation walk did not terminate"};
   uint32_t Mout = c->h_ctr[C_ALIVE];
   mout = Mout;
@@ -622,10 +697,57 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
     launch_map_records(P<uint32_t>(c->rec), nrec, P<uint32_t>(c->slot2idx), s);
     HIP_CHECK(hipMemcpyAsync(viol_rec.data(), c->rec.p, (size_t)nrec * 4, hipMemcpyDeviceToHost, s));
   }
-  if (cfg.collect_stats)
+  if (R.cfg.collect_stats)
     HIP_CHECK(hipMemcpyAsync(c->stats.pred, c->gstats.p, 64, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   c->stats.tet_slots = T;
+}
+
+// The device phase for clouds off[c]..off[c+1] of d_xyz (one cloud: {0, n}),
+// the loop of src/phase1.rs:16-1431 with one host read per round.
+void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& off,
+                const wtet_config& cfg, Counters& K, std::vector<uint32_t>& viol_rec,
+                uint32_t& nviol, bool resident, uint32_t& mout, std::vector<uint32_t>& cstart) {
+  end_stage(c);  // the call reuses the buffers a stage session holds
+  Run R;
+  run_setup(c, R, off, cfg);
+  dev_prepare(c, R, d_xyz);
+  while (R.U > 0) {
+    dev_vote(c, R);
+    dev_split(c, R);
+    dev_split_points(c, R);
+    if (R.flipping) {
+      dev_round_flips(c, R);
+      dev_relocate(c, R);
+    } else {
+      launch_set_ctr(P<uint32_t>(c->ctr), C_A, 0u, -1, 0u, c->stream);
+    }
+  }
+  if (R.flipping) dev_finish_flips(c, R);
+  dev_output(c, R, viol_rec, nviol, resident, mout, cstart);
+  K = R.K;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// stage-level session (wtet_stage_*): the state of run_device between calls
+// ---------------------------------------------------------------------------
+struct wtet_stage_session {
+  Run R;
+  // 1: between rounds (vote, flip, relocate, gather, readback, finish),
+  // 2: voted (split next), 3: split (split_points next)
+  int phase = 0;
+  bool stale = false;        // flips since the last relocation
+  bool audit_valid = false;  // the last gather saw the current mesh
+  uint32_t checked = 0, flips = 0, violations = 0, nrec = 0;
+};
+
+namespace {
+
+void end_stage(wtet_ctx* c) {
+  delete c->stage;
+  c->stage = nullptr;
 }
 
 }  // namespace
@@ -720,23 +842,32 @@ void download_rows(wtet_ctx* c, uint32_t M) {
   c->stats.repair_rows_staged = M;
 }
 
-void delaunay_impl(void* a_) {
-  DelaunayArgs& a = *static_cast<DelaunayArgs*>(a_);
-  wtet_ctx* c = a.c;
-  auto t0 = std::chrono::steady_clock::now();
-  HIP_CHECK(hipSetDevice(c->device));
-  std::memset(&c->stats, 0, sizeof(c->stats));
-  const float* d_xyz = a.xyz;
-  if (!a.on_device) {
-    ensure(c, c->xyz_in, (size_t)a.n * 12);
-    HIP_CHECK(hipMemcpyAsync(c->xyz_in.p, a.xyz, (size_t)a.n * 12, hipMemcpyHostToDevice, c->stream));
-    d_xyz = P<float>(c->xyz_in);
+// failed_verts (gather.wgsl:68-151): the vertex opposite each non-Delaunay
+// face between two real-only tets, from violation records (row << 2 | face)
+std::vector<uint32_t> failed_vertices(const uint32_t* tets, const uint32_t* adj,
+                                      const std::vector<uint32_t>& rec, uint32_t n) {
+  std::vector<uint32_t> failed;
+  for (uint32_t r : rec) {
+    uint32_t t = r >> 2, f = r & 3u;
+    uint32_t e = adj[4 * (size_t)t + f];
+    if (e == kInvalid) continue;
+    uint32_t u = e >> 5, g = e & 3u;
+    bool real = true;
+    for (int k = 0; k < 4; ++k) real = real && tets[4 * (size_t)t + k] < n && tets[4 * (size_t)u + k] < n;
+    if (real) failed.push_back(tets[4 * (size_t)u + g]);
   }
-  Counters K;
-  std::vector<uint32_t> rec, cstart;
-  uint32_t nviol = 0, mout = 0;
-  bool resident = a.dout != nullptr;
-  run_device(c, d_xyz, {0u, a.n}, a.cfg, K, rec, nviol, resident, mout, cstart);
+  std::sort(failed.begin(), failed.end());
+  failed.erase(std::unique(failed.begin(), failed.end()), failed.end());
+  return failed;
+}
+
+// Host side after the device phase of one cloud: failed_verts, the host
+// repair (src/lib.rs:227-307) and the result, as host arrays (out) or as
+// device rows (dout).
+void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counters& K,
+                   std::vector<uint32_t>& rec, uint32_t nviol, uint32_t mout, wtet_result* out,
+                   wtet_device_result* dout, std::chrono::steady_clock::time_point t0) {
+  const bool resident = dout != nullptr;
   c->stats.violations_gpu = nviol;
   auto t1 = std::chrono::steady_clock::now();
   std::vector<uint8_t> present;
@@ -750,35 +881,22 @@ void delaunay_impl(void* a_) {
     mb.known = mout;
   }
   bool on_host = !resident || nviol > 0;
-  // failed_verts: the vertex opposite each non-Delaunay real-only face
   std::vector<uint32_t> failed;
-  if (on_host)
-    for (uint32_t r : rec) {
-      uint32_t t = r >> 2, f = r & 3u;
-      uint32_t e = mb.adj[4 * (size_t)t + f];
-      if (e == kInvalid) continue;
-      uint32_t u = e >> 5, g = e & 3u;
-      bool real = true;
-      for (int k = 0; k < 4; ++k)
-        real = real && mb.tets[4 * (size_t)t + k] < a.n && mb.tets[4 * (size_t)u + k] < a.n;
-      if (real) failed.push_back(mb.tets[4 * (size_t)u + g]);
-    }
-  std::sort(failed.begin(), failed.end());
-  failed.erase(std::unique(failed.begin(), failed.end()), failed.end());
+  if (on_host) failed = failed_vertices(mb.tets, mb.adj, rec, n);
   bool repaired = false;
-  if (nviol > 0 && a.cfg.enable_splaying) {
+  if (nviol > 0 && cfg.enable_splaying) {
     // the repair needs the normalized coordinates of the input ids
-    ensure_pinned(c->h_xyzn, c->h_npts, (size_t)a.n * 3);
-    HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)a.n * 12, hipMemcpyDeviceToHost, c->stream));
+    ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
+    HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     RepairReport rr;
-    wtet_status rs = repair_delaunay(c->h_xyzn, a.n, mb, rec, a.cfg.predicate_mode, &rr);
+    wtet_status rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr);
     if (rs != WTET_OK && rr.needs_full) {
       // the neighbourhood was too small: every row, and again from the start
       download_rows(c, mout);
       mb = MeshBuf{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
       rr = RepairReport{};
-      rs = repair_delaunay(c->h_xyzn, a.n, mb, rec, a.cfg.predicate_mode, &rr);
+      rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr);
       c->stats.repair_full_downloads++;
     }
     if (rs != WTET_OK) throw Fail{rs, "host repair failed: " + rr.message};
@@ -793,7 +911,7 @@ void delaunay_impl(void* a_) {
   c->stats.rounds = K.rounds;
   c->stats.flip_iterations = K.iters;
   c->stats.peak_active = K.peakA;
-  c->stats.num_points = a.n;
+  c->stats.num_points = n;
   if (resident) {
     if (repaired) {
       // upload only the slots the repair wrote: (slot, tet[4], adj[4])
@@ -818,16 +936,15 @@ void delaunay_impl(void* a_) {
       }
       mout = (uint32_t)mb.count;
     }
-    a.dout->num_tets = mout;
-    a.dout->d_tets = P<uint32_t>(c->prop);
-    a.dout->d_adjacency = P<uint32_t>(c->fout);
-    a.dout->num_failed = (uint32_t)failed.size();
+    dout->num_tets = mout;
+    dout->d_tets = P<uint32_t>(c->prop);
+    dout->d_adjacency = P<uint32_t>(c->fout);
+    dout->num_failed = (uint32_t)failed.size();
     c->stats.num_tets_out = mout;
     c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return;
   }
   std::vector<uint32_t> tets(mb.tets, mb.tets + 4 * mb.count), adj(mb.adj, mb.adj + 4 * mb.count);
-  wtet_result* out = a.out;
   out->num_tets = (uint32_t)(tets.size() / 4);
   out->tets = (uint32_t*)std::malloc(std::max<size_t>(tets.size(), 1) * 4);
   out->adjacency = (uint32_t*)std::malloc(std::max<size_t>(adj.size(), 1) * 4);
@@ -839,6 +956,26 @@ void delaunay_impl(void* a_) {
   if (!failed.empty()) std::memcpy(out->failed_verts, failed.data(), failed.size() * 4);
   c->stats.num_tets_out = out->num_tets;
   c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void delaunay_impl(void* a_) {
+  DelaunayArgs& a = *static_cast<DelaunayArgs*>(a_);
+  wtet_ctx* c = a.c;
+  auto t0 = std::chrono::steady_clock::now();
+  HIP_CHECK(hipSetDevice(c->device));
+  std::memset(&c->stats, 0, sizeof(c->stats));
+  const float* d_xyz = a.xyz;
+  if (!a.on_device) {
+    ensure(c, c->xyz_in, (size_t)a.n * 12);
+    HIP_CHECK(hipMemcpyAsync(c->xyz_in.p, a.xyz, (size_t)a.n * 12, hipMemcpyHostToDevice, c->stream));
+    d_xyz = P<float>(c->xyz_in);
+  }
+  Counters K;
+  std::vector<uint32_t> rec, cstart;
+  uint32_t nviol = 0, mout = 0;
+  bool resident = a.dout != nullptr;
+  run_device(c, d_xyz, {0u, a.n}, a.cfg, K, rec, nviol, resident, mout, cstart);
+  finish_single(c, a.n, a.cfg, K, rec, nviol, mout, a.out, a.dout, t0);
 }
 
 
@@ -1106,6 +1243,7 @@ wtet_status wtet_create(int device, wtet_ctx** out) {
 
 void wtet_destroy(wtet_ctx* c) {
   if (!c) return;
+  end_stage(c);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
@@ -1821,3 +1959,285 @@ wtet_status wtet_predicates_device(wtet_ctx* c, int kind, const float* pts, cons
 }
 
 }  // extern "C"
+namespace {
+
+template <typename Fn>
+wtet_status guarded(wtet_ctx* c, Fn&& fn) {
+  struct A {
+    Fn* f;
+  } a{&fn};
+  return run_guarded(c, [](void* p) { (*static_cast<A*>(p)->f)(); }, &a);
+}
+
+wtet_stage_session* session_of(wtet_ctx* c, int phase, const char* what) {
+  wtet_stage_session* S = c->stage;
+  HIP_CHECK(hipSetDevice(c->device));
+  if (!S) throw Fail{WTET_ERR_INVALID_ARGUMENT, std::string(what) + ": no stage session (wtet_stage_init)"};
+  if (phase && S->phase != phase) {
+    static const char* next[] = {"", "vote", "split", "split_points"};
+    throw Fail{WTET_ERR_INVALID_ARGUMENT,
+               std::string(what) + " out of order: " + next[S->phase] + " comes next"};
+  }
+  return S;
+}
+
+void stage_info(wtet_ctx* c, const wtet_stage_session* S, wtet_stage_info* info) {
+  if (!info) return;
+  read_ctr(c);
+  info->round = S->R.round;
+  info->uninserted = S->R.U;
+  info->winners = S->R.W;
+  info->tet_slots = c->h_ctr[C_TETS];
+  info->active = c->h_ctr[C_A];
+  info->checked = S->checked;
+  info->flips = S->flips;
+  info->violations = S->violations;
+}
+
+// slots holding tets: between a vote and its split the slots the vote
+// reserved (from ctr[C_BASE]) are not written yet
+uint32_t written_slots(wtet_ctx* c, const wtet_stage_session* S) {
+  read_ctr(c);
+  return S->phase == 2 ? c->h_ctr[C_BASE] : c->h_ctr[C_TETS];
+}
+
+}  // namespace
+
+wtet_status wtet_stage_init(wtet_ctx* c, const float* xyz, uint32_t n, const wtet_config* cfg,
+                            wtet_stage_info* info) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  end_stage(c);
+  if (!xyz) {
+    c->err = "points pointer is null";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (n < 4) {
+    c->err = "Need at least 4 points for 3D Delaunay, got " + std::to_string(n);
+    return WTET_ERR_TOO_FEW_POINTS;
+  }
+  if (n > (1u << 26)) {
+    c->err = "too many points for one call";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  wtet_config kc;
+  if (cfg) kc = *cfg;
+  else wtet_config_default(&kc);
+  if (kc.insertion_rule != 0 && kc.insertion_rule != 1) {
+    c->err = "Invalid insertion_rule. Must be 'circumcenter' or 'centroid'.";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (kc.presize_mode < 0 || kc.presize_mode > 2 || kc.predicate_mode < 0 || kc.predicate_mode > 1) {
+    c->err = "presize_mode must be 0, 1 or 2 and predicate_mode 0 or 1";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  wtet_status st = guarded(c, [&] {
+    HIP_CHECK(hipSetDevice(c->device));
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    ensure(c, c->xyz_in, (size_t)n * 12);
+    HIP_CHECK(hipMemcpyAsync(c->xyz_in.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+    auto* S = new wtet_stage_session();
+    c->stage = S;
+    run_setup(c, S->R, {0u, n}, kc);
+    dev_prepare(c, S->R, P<float>(c->xyz_in));
+    S->phase = 1;
+    stage_info(c, S, info);
+  });
+  if (st != WTET_OK) end_stage(c);
+  return st;
+}
+
+wtet_status wtet_stage_vote(wtet_ctx* c, wtet_stage_info* info) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  return guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 1, "vote");
+    if (S->stale) throw Fail{WTET_ERR_INVALID_ARGUMENT, "vote out of order: relocate after flipping"};
+    S->checked = S->flips = 0;
+    if (S->R.U == 0) {
+      S->R.W = 0;  // every point is in: nothing to vote on
+    } else {
+      dev_vote(c, S->R);
+      S->phase = 2;
+    }
+    stage_info(c, S, info);
+  });
+}
+
+wtet_status wtet_stage_split(wtet_ctx* c, wtet_stage_info* info) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  return guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 2, "split");
+    dev_split(c, S->R);
+    S->phase = 3;
+    S->audit_valid = false;
+    stage_info(c, S, info);
+  });
+}
+
+wtet_status wtet_stage_split_points(wtet_ctx* c, wtet_stage_info* info) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  return guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 3, "split_points");
+    dev_split_points(c, S->R);
+    S->phase = 1;
+    stage_info(c, S, info);
+  });
+}
+
+wtet_status wtet_stage_flip(wtet_ctx* c, int predicate_mode, wtet_stage_info* info) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  if (predicate_mode < -1 || predicate_mode > 1) {
+    c->err = "predicate_mode must be -1, 0 or 1";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  return guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 1, "flip");
+    Run& R = S->R;
+    int mode = predicate_mode < 0 ? R.mode : (predicate_mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED);
+    read_ctr(c);
+    if (c->h_ctr[C_STOP]) grow_for(c, (uint64_t)c->h_ctr[C_TETS] + c->h_ctr[C_A] + 64);
+    uint32_t a = c->h_ctr[C_A];
+    uint64_t before[4] = {0, 0, 0, 0}, after[4] = {0, 0, 0, 0};
+    HIP_CHECK(hipMemcpyAsync(before, c->isums.p, 32, hipMemcpyDeviceToHost, c->stream));
+    if (a > 0) run_flip_batch(c, R.V, mode, R.st, 1, a, a / 8 + 1);
+    HIP_CHECK(hipMemcpyAsync(after, c->isums.p, 32, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    S->checked = (uint32_t)(after[0] - before[0]);
+    S->flips = (uint32_t)(after[1] - before[1]);
+    if (S->flips) {
+      S->audit_valid = false;
+      if (R.U > 0) S->stale = true;
+    }
+    stage_info(c, S, info);
+  });
+}
+
+wtet_status wtet_stage_relocate(wtet_ctx* c, wtet_stage_info* info) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  return guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 1, "relocate");
+    dev_relocate(c, S->R);
+    read_ctr(c);
+    if (c->h_ctr[C_WALK_FAIL]) throw Fail{WTET_ERR_INTERNAL, "point relocation walk did not terminate"};
+    S->stale = false;
+    stage_info(c, S, info);
+  });
+}
+
+wtet_status wtet_stage_gather(wtet_ctx* c, wtet_stage_info* info) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  return guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 0, "gather");
+    uint32_t T = written_slots(c, S);
+    Mesh M = mesh_of(c, S->R.V, S->R.mode);
+    launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, c->stream);
+    const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
+    timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, S->R.st, c->stream); });
+    HIP_CHECK(hipGetLastError());
+    read_ctr(c);
+    S->violations = c->h_ctr[C_VIOL];
+    S->nrec = std::min(c->h_ctr[C_FAILREC], rec_cap);
+    S->audit_valid = true;
+    stage_info(c, S, info);
+  });
+}
+
+wtet_status wtet_stage_readback(wtet_ctx* c, wtet_result* out) {
+  if (!c || !out) return WTET_ERR_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  wtet_status st = guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 0, "readback");
+    hipStream_t s = c->stream;
+    uint32_t T = written_slots(c, S);
+    // the flip lists stay as they are: the rows go through ord[0]
+    dev_compact(c, S->R, T, true, true);
+    read_ctr(c);
+    uint32_t m = c->h_ctr[C_ALIVE];
+    std::vector<uint32_t> tets((size_t)m * 4), adj((size_t)m * 4), rec;
+    HIP_CHECK(hipMemcpyAsync(tets.data(), c->prop.p, (size_t)m * 16, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(adj.data(), c->fout.p, (size_t)m * 16, hipMemcpyDeviceToHost, s));
+    if (S->audit_valid && S->nrec) {
+      // records are slot-based; map a copy of them to rows
+      std::vector<uint32_t> slot2row(T);
+      rec.resize(S->nrec);
+      HIP_CHECK(hipMemcpyAsync(slot2row.data(), c->slot2idx.p, (size_t)T * 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipMemcpyAsync(rec.data(), c->rec.p, (size_t)S->nrec * 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      for (uint32_t& r : rec) r = (slot2row[r >> 2] << 2) | (r & 3u);
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> failed = failed_vertices(tets.data(), adj.data(), rec, S->R.n);
+    out->num_tets = m;
+    out->tets = (uint32_t*)std::malloc(std::max<size_t>(tets.size(), 1) * 4);
+    out->adjacency = (uint32_t*)std::malloc(std::max<size_t>(adj.size(), 1) * 4);
+    out->num_failed = (uint32_t)failed.size();
+    out->failed_verts = (uint32_t*)std::malloc(std::max<size_t>(failed.size(), 1) * 4);
+    if (!out->tets || !out->adjacency || !out->failed_verts) throw std::bad_alloc();
+    if (m) {
+      std::memcpy(out->tets, tets.data(), tets.size() * 4);
+      std::memcpy(out->adjacency, adj.data(), adj.size() * 4);
+    }
+    if (!failed.empty()) std::memcpy(out->failed_verts, failed.data(), failed.size() * 4);
+  });
+  if (st != WTET_OK) wtet_result_free(out);
+  return st;
+}
+
+wtet_status wtet_stage_locations(wtet_ctx* c, uint32_t cap, uint32_t* point_ids, uint32_t* rows,
+                                 uint32_t* count) {
+  if (!c || !count || (cap && (!point_ids || !rows))) return WTET_ERR_INVALID_ARGUMENT;
+  return guarded(c, [&] {
+    wtet_stage_session* S = session_of(c, 1, "locations");
+    if (S->stale) throw Fail{WTET_ERR_INVALID_ARGUMENT, "locations out of order: relocate after flipping"};
+    Run& R = S->R;
+    hipStream_t s = c->stream;
+    *count = R.U;
+    uint32_t k = std::min(cap, R.U);
+    if (k == 0) return;
+    uint32_t T = written_slots(c, S);
+    dev_compact(c, R, T, false, true);  // slot -> row, as a readback numbers them
+    std::vector<uint32_t> uv(k), ut(k), slot2row(T);
+    std::vector<float4> pts((size_t)R.V + 1);
+    HIP_CHECK(hipMemcpyAsync(uv.data(), c->uvert[R.cur].p, (size_t)k * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(ut.data(), c->utet[R.cur].p, (size_t)k * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(slot2row.data(), c->slot2idx.p, (size_t)T * 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(pts.data(), c->pts.p, pts.size() * 16, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < k; ++i) {
+      if (uv[i] >= R.V || ut[i] >= T) throw Fail{WTET_ERR_INTERNAL, "uninserted list entry out of range"};
+      point_ids[i] = pt_rank(pts[uv[i]]);
+      rows[i] = slot2row[ut[i]];
+    }
+  });
+}
+
+wtet_status wtet_stage_finish(wtet_ctx* c, wtet_result* out) {
+  if (!c || !out) return WTET_ERR_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  wtet_status st = guarded(c, [&] {
+    auto t0 = std::chrono::steady_clock::now();
+    wtet_stage_session* S = session_of(c, 1, "finish");
+    Run& R = S->R;
+    if (R.U > 0)
+      throw Fail{WTET_ERR_INVALID_ARGUMENT, "finish: " + std::to_string(R.U) + " points are not inserted yet"};
+    if (R.flipping) dev_finish_flips(c, R);
+    std::vector<uint32_t> rec, cstart;
+    uint32_t nviol = 0, mout = 0;
+    dev_output(c, R, rec, nviol, false, mout, cstart);
+    Counters K = R.K;
+    uint32_t n = R.n;
+    wtet_config cfg = R.cfg;
+    end_stage(c);
+    finish_single(c, n, cfg, K, rec, nviol, mout, out, nullptr, t0);
+  });
+  if (st != WTET_OK) {
+    wtet_result_free(out);
+    if (st != WTET_ERR_INVALID_ARGUMENT) end_stage(c);  // a misuse keeps the session
+  }
+  return st;
+}
+
+wtet_status wtet_stage_end(wtet_ctx* c) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  end_stage(c);
+  return WTET_OK;
+}
