@@ -85,6 +85,9 @@ typedef struct wtet_stats {
                               log(peak active / n), log(1 + stuck faces)        */
   uint64_t repair_rows_staged;   /* rows copied to the host for the repair     */
   uint32_t repair_full_downloads; /* repairs that needed every row            */
+  double ms_repair_staging;       /* of ms_repair: rows and points to the host */
+  double ms_repair_compute;       /* of ms_repair: the host re-triangulation   */
+  uint32_t repair_fetches;        /* rows fetched on demand during the repair */
 } wtet_stats;
 
 typedef struct wtet_ctx wtet_ctx;
@@ -207,6 +210,17 @@ wtet_status wtet_device_copy(wtet_ctx *ctx, void *d_dst, const void *d_src, size
  * The triangulation does not depend on it.  No counterpart in the
  * reference. */
 wtet_status wtet_set_insertion_ring(wtet_ctx *ctx, int ring);
+/* The first `rounds` insertion rounds use ring 0 whatever the ring: while
+ * the mesh is small a round costs about the same however few points it
+ * inserts, and ring 0 inserts several times as many.  0xFFFFFFFF (the
+ * default): round(log4(largest cloud)) - 4, clamped to 0..6.  The
+ * triangulation does not depend on it.  No counterpart in the reference. */
+wtet_status wtet_set_insertion_warmup(wtet_ctx *ctx, uint32_t rounds);
+/* A resident result's host repair works on the rows within `rings`
+ * face-steps (1..16, default 6) of each stuck face, copied to the host; a
+ * repair that needs more falls back to every row.  No counterpart in the
+ * reference. */
+wtet_status wtet_set_repair_rings(wtet_ctx *ctx, int rings);
 
 /* The flip iterations of a round run on the device without host reads
  * (counts, iteration tag and list parity live in device counters); they

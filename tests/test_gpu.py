@@ -86,10 +86,10 @@ def test_device_predicates_match_oracle(oracle, ctx, mode):
 # -- triangulations ----------------------------------------------------------------
 
 def test_config1_uniform_10k_exact_path(oracle, ctx):
-    """10,000 U[0,1)^3 f32 seed 0, exact predicates only."""
+    """10,000 U[0,1)^3 f32 seed 0, Hilbert insertion order, exact predicates only."""
     p = W.uniform(10000, 0)
     assert W.sha256(p) == GOLD["uniform_10000_seed0"]
-    res = _check(oracle, ctx, p, mode="exact")
+    res = _check(oracle, ctx, p, mode="exact", enable_hilbert_sorting=True)
     can = oracle.canonical_real(res.tets, len(p))
     g = GOLD["oracle_canonical_uniform_10000_seed0"]
     assert len(can) == g["count"]
@@ -150,7 +150,7 @@ def test_uniform_200k_matches_oracle(oracle, ctx):
 
 
 def test_uniform_1m_properties(ctx):
-    """configs[1] shape (1,000,000 U[0,1)^3 f32 seed 1): no face fails the
+    """1,000,000 U[0,1)^3 f32 points, seed 1: no face fails the
     insphere test and the counts are consistent (size-independent checks)."""
     p = W.uniform(1000000, 1)
     assert W.sha256(p) == GOLD["uniform_1000000_seed1"]
@@ -315,3 +315,26 @@ def test_hilbert_order_in_a_batch(oracle, ctx):
         pn, _, _ = oracle.normalize_points(p)
         ref = oracle.canonical_real(oracle.delaunay(pn), len(p))
         assert np.array_equal(oracle.canonical_real(res[c].tets, len(p)), ref), c
+
+
+@pytest.mark.parametrize("rings", [1, 2])
+def test_resident_repair_fetches_missing_rows(oracle, ctx, rings):
+    """With a thin staged neighbourhood the repair reaches rows the host
+    lacks; it fetches the rows around them and goes on (after a few fetches
+    it takes every row instead).  The mesh equals the host-result call's."""
+    p = W.uniform(300000, 72)
+    d = ctx.upload(p)
+    ctx.set_repair_rings(rings)
+    try:
+        m, dt, da = ctx.delaunay_resident(d, len(p))
+        st = ctx.stats()
+        tets = ctx.download(dt, (m, 4), np.uint32)
+        adj = ctx.download(da, (m, 4), np.uint32)
+    finally:
+        ctx.set_repair_rings(6)
+        ctx.free(d)
+    assert st["violations_gpu"] > 0 and st["repair_fetches"] > 0, st
+    host = ctx.delaunay(p)
+    assert np.array_equal(oracle.canonical_real(tets, len(p)), oracle.canonical_real(host.tets, len(p)))
+    back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
+    assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))

@@ -118,6 +118,9 @@ class _Stats(ctypes.Structure):
         ("presize_pred", ctypes.c_float * 3),
         ("repair_rows_staged", ctypes.c_uint64),
         ("repair_full_downloads", ctypes.c_uint32),
+        ("ms_repair_staging", ctypes.c_double),
+        ("ms_repair_compute", ctypes.c_double),
+        ("repair_fetches", ctypes.c_uint32),
     ]
 
 
@@ -175,6 +178,8 @@ def lib():
             L.wtet_device_info.argtypes = [P, ctypes.c_char_p, u32]
             L.wtet_set_profiling.argtypes = [P, ctypes.c_int]
             L.wtet_set_insertion_ring.argtypes = [P, ctypes.c_int]
+            L.wtet_set_insertion_warmup.argtypes = [P, u32]
+            L.wtet_set_repair_rings.argtypes = [P, ctypes.c_int]
             L.wtet_set_flip_graphs.argtypes = [P, ctypes.c_int]
             L.wtet_set_flip_items.argtypes = [P, u32]
             L.wtet_get_flip_items.argtypes = [P, P, P]
@@ -320,6 +325,18 @@ class Context:
         """Insertion independence radius in face-steps (0, 1 or 2)."""
         if lib().wtet_set_insertion_ring(self._h, int(ring)) != 0:
             raise ValueError("ring must be 0, 1 or 2")
+
+    def set_insertion_warmup(self, rounds: "int | None"):
+        """The first `rounds` rounds insert one point per tet (ring 0); None:
+        chosen from the largest cloud (the default)."""
+        v = 0xFFFFFFFF if rounds is None else int(rounds)
+        if lib().wtet_set_insertion_warmup(self._h, v) != 0:
+            raise ValueError(self.error())
+
+    def set_repair_rings(self, rings: int):
+        """Rows within `rings` face-steps of each stuck face go to the host repair."""
+        if lib().wtet_set_repair_rings(self._h, int(rings)) != 0:
+            raise ValueError("rings must be in 1..16")
 
     def set_flip_graphs(self, on: bool):
         """Replay the captured graph of a round's flip iterations (default) or

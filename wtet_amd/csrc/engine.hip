@@ -91,6 +91,10 @@ struct wtet_ctx {
   // (src/profiler.rs GpuProfiler scopes, here actually recorded)
   bool profiling = false;
   int ring = 1;  // insertion independence radius (face-steps), see is_winner
+  // the first rounds insert one point per tet (ring 0); kAutoWarmup: from
+  // the largest cloud (warmup_rounds)
+  uint32_t ring0_rounds = 0xFFFFFFFFu;
+  int repair_rings = 6;       // face-steps of rows staged around each stuck face
   std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
   std::vector<int> evkind;
   size_t evused = 0;
@@ -162,6 +166,9 @@ void ensure_pinned(T*& p, size_t& cap, size_t need) {
     throw Fail{WTET_ERR_OUT_OF_MEMORY, "pinned host allocation failed"};
   }
   cap = nc;
+  // touch every page once: the host maps pinned pages on first write, and
+  // the repair's scattered row writes would otherwise fault page by page
+  std::memset(static_cast<void*>(p), 0, nc * sizeof(T));
 }
 
 // per-tet-slot arrays sized for `cap` slots; keep=true preserves contents
@@ -388,8 +395,21 @@ struct Run {
   uint32_t W = 0;          // insertions of the current round
   uint64_t dsum_prev = 0;  // deferred-check total at the previous round
   int max_fi = 10;
+  uint32_t warmup = 0;     // rounds that insert one point per tet
   Counters K;
 };
+
+// Rounds that insert one point per tet before the ring applies.  While a
+// cloud is small a round costs about the same whatever it inserts, and ring
+// 0 inserts several times as many points as ring 1 there; later, adjacent
+// insertions leave faces the flips cannot fix.  Measured best: 6 rounds for
+// clouds of 1M-2M points, 4 for clouds of at most 50,000 (about log4(n) - 4).
+uint32_t warmup_rounds(const std::vector<uint32_t>& off) {
+  uint32_t big = 0;
+  for (size_t k = 0; k + 1 < off.size(); ++k) big = std::max(big, off[k + 1] - off[k]);
+  int l4 = (int)std::lround(std::log((double)std::max(big, 1u)) / std::log(4.0));
+  return (uint32_t)std::min(6, std::max(0, l4 - 4));
+}
 
 void run_setup(wtet_ctx* c, Run& R, const std::vector<uint32_t>& off, const wtet_config& cfg) {
   R = Run{};
@@ -403,6 +423,7 @@ void run_setup(wtet_ctx* c, Run& R, const std::vector<uint32_t>& off, const wtet
   R.learned = cfg.presize_mode == 1;
   R.flipping = cfg.enable_flipping != 0;
   R.max_fi = (int)std::min<uint32_t>(std::max<uint32_t>(cfg.max_flip_iterations, 1u), 64u);
+  R.warmup = c->ring0_rounds == 0xFFFFFFFFu ? warmup_rounds(off) : c->ring0_rounds;
   key_layout(R.C, R.hilbert, &R.kbits, &R.cshift, &R.end_bit);
   R.st = cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
 }
@@ -537,7 +558,7 @@ uint32_t dev_vote(wtet_ctx* c, Run& R) {
   timed(c, K_PICK, [&] {
     launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round,
                 P<uint2>(c->wlist), P<uint32_t>(c->uvert[cur ^ 1]),
-                P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), c->ring, s);
+                P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), round <= R.warmup ? 0 : c->ring, s);
   });
   HIP_CHECK(hipGetLastError());
   // the one host read of a round: winners, reserved slots, and whether the
@@ -790,8 +811,39 @@ struct DelaunayArgs {
 // few face-steps of the recorded faces and the tail rows that fill holes,
 // with their neighbours.  They land in the pinned staging at their own row;
 // present[] marks them.  rec (rows) is on the device in c->rec.
-constexpr int kRepairRings = 6;
+constexpr int kRepairRingsMax = 16;
 constexpr uint32_t kRepairTail = 4096;
+
+// packed rows list[from, to) of the neighbourhood into the host mirror
+void scatter_rows(wtet_ctx* c, uint32_t from, uint32_t to, std::vector<uint8_t>& present) {
+  hipStream_t s = c->stream;
+  uint32_t n = to - from;
+  if (n == 0) return;
+  ensure(c, c->nbr_pack, (size_t)n * 36 + 16);
+  launch_nbr_gather(P<uint32_t>(c->nbr_list) + from, n, P<uint32_t>(c->prop), P<uint32_t>(c->fout),
+                    P<uint32_t>(c->nbr_pack), s);
+  HIP_CHECK(hipGetLastError());
+  ensure_pinned(c->h_patch, c->h_patch_cap, (size_t)n * 9 + 9);
+  HIP_CHECK(hipMemcpyAsync(c->h_patch, c->nbr_pack.p, (size_t)n * 36, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t* q = c->h_patch + 9 * (size_t)i;
+    uint32_t r = q[0];
+    std::memcpy(c->h_tets + 4 * (size_t)r, q + 1, 16);
+    std::memcpy(c->h_adj + 4 * (size_t)r, q + 5, 16);
+    present[r] = 1;
+  }
+  c->stats.repair_rows_staged += n;
+}
+
+// rows listed so far (device counter), checked against the mesh size
+uint32_t listed_rows(wtet_ctx* c, uint32_t M) {
+  uint32_t n = 0;
+  HIP_CHECK(hipMemcpyAsync(&n, c->nbr_misc.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (n > M) throw Fail{WTET_ERR_INTERNAL, "neighbourhood larger than the mesh"};
+  return n;
+}
 
 void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M, std::vector<uint8_t>& present) {
   hipStream_t s = c->stream;
@@ -803,31 +855,38 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M, std::vector<uin
   uint32_t tail = std::min(M, kRepairTail);
   ensure(c, c->nbr_mark, (size_t)M * 4 + 16);
   ensure(c, c->nbr_list, (size_t)M * 4 + 16);
-  ensure(c, c->nbr_misc, 4 * (kRepairRings + 4));
+  ensure(c, c->nbr_misc, 4 * (kRepairRingsMax + 4));
   uint32_t* cnt = P<uint32_t>(c->nbr_misc);
-  launch_neighbourhood(P<uint32_t>(c->rec), nrec, M, tail, kRepairRings, P<uint32_t>(c->fout),
-                       P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), cnt, cnt + 1, s);
+  launch_neighbourhood(P<uint32_t>(c->rec), nrec, M, tail, c->repair_rings, P<uint32_t>(c->fout),
+                       P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), cnt, cnt + 1, false, s);
   HIP_CHECK(hipGetLastError());
-  uint32_t n = 0;
-  HIP_CHECK(hipMemcpyAsync(&n, cnt, 4, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  if (n > M) throw Fail{WTET_ERR_INTERNAL, "neighbourhood larger than the mesh"};
-  ensure(c, c->nbr_pack, (size_t)n * 36 + 16);
-  launch_nbr_gather(P<uint32_t>(c->nbr_list), n, P<uint32_t>(c->prop), P<uint32_t>(c->fout),
-                    P<uint32_t>(c->nbr_pack), s);
-  HIP_CHECK(hipGetLastError());
-  ensure_pinned(c->h_patch, c->h_patch_cap, (size_t)n * 9 + 9);
-  HIP_CHECK(hipMemcpyAsync(c->h_patch, c->nbr_pack.p, (size_t)n * 36, hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  uint32_t n = listed_rows(c, M);
   present.assign(M, 0);
-  for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t* q = c->h_patch + 9 * (size_t)i;
-    uint32_t r = q[0];
-    std::memcpy(c->h_tets + 4 * (size_t)r, q + 1, 16);
-    std::memcpy(c->h_adj + 4 * (size_t)r, q + 5, 16);
-    present[r] = 1;
-  }
-  c->stats.repair_rows_staged = n;
+  c->stats.repair_rows_staged = 0;
+  scatter_rows(c, 0, n, present);
+}
+
+// the repair reached `row`, which the host lacks: the rows within the
+// staging rings of it that are not on the host yet.  Each fetch restarts a
+// region, so after kMaxFetches of them, or once a quarter of the mesh is on
+// the host, it returns false and the repair takes every row instead.
+constexpr uint32_t kMaxFetches = 4;
+
+bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, std::vector<uint8_t>& present,
+                uint32_t& fetches) {
+  if (row >= M || present[row] || ++fetches > kMaxFetches) return false;
+  uint32_t before = listed_rows(c, M);
+  if (before > M / 4) return false;
+  uint32_t* misc = P<uint32_t>(c->nbr_misc);
+  uint32_t* seed = misc + kRepairRingsMax + 3;  // after the count and the ring bounds
+  uint32_t key = row << 2;
+  HIP_CHECK(hipMemcpyAsync(seed, &key, 4, hipMemcpyHostToDevice, c->stream));
+  launch_neighbourhood(seed, 1, M, 0, c->repair_rings, P<uint32_t>(c->fout), P<uint32_t>(c->nbr_mark),
+                       P<uint32_t>(c->nbr_list), misc, misc + 1, true, c->stream);
+  HIP_CHECK(hipGetLastError());
+  uint32_t after = listed_rows(c, M);
+  scatter_rows(c, before, after, present);
+  return present[row] != 0;
 }
 
 void download_rows(wtet_ctx* c, uint32_t M) {
@@ -876,9 +935,13 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     else stage_neighbourhood(c, (uint32_t)rec.size(), mout, present);
   }
   MeshBuf mb{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
+  uint32_t fetches = 0;
   if (!present.empty()) {
     mb.present = present.data();
     mb.known = mout;
+    mb.fetch = [c, mout, &present, &fetches](uint32_t row) {
+      return fetch_rows(c, row, mout, present, fetches);
+    };
   }
   bool on_host = !resident || nviol > 0;
   std::vector<uint32_t> failed;
@@ -889,6 +952,8 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
     HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+    auto tc = std::chrono::steady_clock::now();
+    c->stats.ms_repair_staging = std::chrono::duration<double, std::milli>(tc - t1).count();
     RepairReport rr;
     wtet_status rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr);
     if (rs != WTET_OK && rr.needs_full) {
@@ -901,6 +966,9 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     }
     if (rs != WTET_OK) throw Fail{rs, "host repair failed: " + rr.message};
     c->stats.repair_regions = rr.regions;
+    c->stats.repair_fetches = std::min(fetches, kMaxFetches);
+    c->stats.ms_repair_compute =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
     repaired = true;
   }
   auto t2 = std::chrono::steady_clock::now();
@@ -1849,6 +1917,18 @@ wtet_status wtet_stratified_ranks(uint32_t n, uint32_t stride, uint32_t* rank) {
 wtet_status wtet_hilbert_index(const uint32_t* xyz, uint32_t n, uint32_t* out) {
   if (!xyz || !out) return WTET_ERR_INVALID_ARGUMENT;
   for (uint32_t i = 0; i < n; ++i) out[i] = hilbert3_8(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  return WTET_OK;
+}
+
+wtet_status wtet_set_repair_rings(wtet_ctx* c, int rings) {
+  if (!c || rings < 1 || rings > kRepairRingsMax) return WTET_ERR_INVALID_ARGUMENT;
+  c->repair_rings = rings;
+  return WTET_OK;
+}
+
+wtet_status wtet_set_insertion_warmup(wtet_ctx* c, uint32_t rounds) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  c->ring0_rounds = rounds;
   return WTET_OK;
 }
 

@@ -108,10 +108,12 @@ void launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs&
 // violation records (slot << 2 | face) -> output indices
 void launch_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx, hipStream_t s);
 // rows within `rings` face-steps of the recorded faces' rows and of the last
-// `tail` rows: list[0, *cnt) (mark: M words of scratch, bounds: rings + 2)
+// `tail` rows: list[0, *cnt) (mark: M words of scratch, bounds: rings + 2).
+// more=true keeps the rows already listed and appends those around the new
+// seeds that are not (list[old *cnt, new *cnt)).
 void launch_neighbourhood(const uint32_t* rec, uint32_t nrec, uint32_t M, uint32_t tail, int rings,
                           const uint32_t* adj, uint32_t* mark, uint32_t* list, uint32_t* cnt,
-                          uint32_t* bounds, hipStream_t s);
+                          uint32_t* bounds, bool more, hipStream_t s);
 // packed (row, tet[4], adj[4]) records of the listed rows
 void launch_nbr_gather(const uint32_t* list, uint32_t n, const uint32_t* tets, const uint32_t* adj,
                        uint32_t* out, hipStream_t s);

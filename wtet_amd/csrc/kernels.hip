@@ -814,11 +814,14 @@ __global__ void k_nbr_gather(const uint32_t* list, uint32_t n, const uint32_t* t
 
 void launch_neighbourhood(const uint32_t* rec, uint32_t nrec, uint32_t M, uint32_t tail, int rings,
                           const uint32_t* adj, uint32_t* mark, uint32_t* list, uint32_t* cnt,
-                          uint32_t* bounds, hipStream_t s) {
-  k_fill_u32<<<nblocks(M), kBlock, 0, s>>>(mark, 0u, M);
-  k_fill_u32<<<1, kBlock, 0, s>>>(cnt, 0u, 1);
-  k_nbr_seed<<<nblocks((uint64_t)nrec + tail), kBlock, 0, s>>>(rec, nrec, M, tail, mark, list, cnt);
-  k_fill_u32<<<1, kBlock, 0, s>>>(bounds, 0u, 1);
+                          uint32_t* bounds, bool more, hipStream_t s) {
+  if (!more) {
+    k_fill_u32<<<nblocks(M), kBlock, 0, s>>>(mark, 0u, M);
+    k_fill_u32<<<1, kBlock, 0, s>>>(cnt, 0u, 1);
+  }
+  k_nbr_snap<<<1, 64, 0, s>>>(cnt, bounds, 0);
+  if (nrec + tail)
+    k_nbr_seed<<<nblocks((uint64_t)nrec + tail), kBlock, 0, s>>>(rec, nrec, M, tail, mark, list, cnt);
   for (int k = 0; k < rings; ++k) {
     k_nbr_snap<<<1, 64, 0, s>>>(cnt, bounds, k + 1);
     k_nbr_ring<<<256, kBlock, 0, s>>>(adj, M, mark, list, cnt, bounds, k);

@@ -240,13 +240,15 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
 // ---------------------------------------------------------------------------
 // in-place repair on the result arrays (ids, adjacency (t << 5) | f)
 // ---------------------------------------------------------------------------
-struct NotPresent {};
+struct NotPresent {
+  uint32_t row;
+};
 
 struct Mesh32 {
   MeshBuf& m;
   uint32_t inf;
   void need(uint32_t t) const {
-    if (m.present && t < m.known && !m.present[t]) throw NotPresent{};
+    if (m.present && t < m.known && !m.present[t]) throw NotPresent{t};
   }
   size_t count() const { return m.count; }
   bool dead(uint32_t t) const {
@@ -321,10 +323,11 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
   std::unordered_set<uint32_t> inR;
   std::vector<uint32_t> R, ring;
 
-  while (!queue.empty()) {
-    auto [t, f] = queue.front();
-    queue.pop_front();
-    if (t >= W.count() || W.dead(t) || !face_bad(S, W, t, f, mode)) continue;
+  wtet_status fail = WTET_OK;
+  // one queued face: every row it reads is read before the first write, so
+  // a missing row can be fetched and the face processed again
+  auto process = [&](uint32_t t, int f) -> void {
+    if (t >= W.count() || W.dead(t) || !face_bad(S, W, t, f, mode)) return;
     R.clear();
     inR.clear();
     auto add = [&](uint32_t x) {
@@ -353,7 +356,8 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
       std::string err;
       if (!local_delaunay(S, real, mode, L, err)) {
         if (rep) rep->message = err;
-        return WTET_ERR_INTERNAL;
+        fail = WTET_ERR_INTERNAL;
+        return;
       }
       std::unordered_map<FaceKey, std::array<int64_t, 2>, FaceHash> fmap;
       fmap.reserve(4 * L.alive.size());
@@ -376,6 +380,7 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
         for (int h = 0; h < 4; ++h) {
           uint32_t nb = W.ta(x, h);
           if (nb != kInvalid && inR.count(nb >> 5)) continue;
+          if (nb != kInvalid) (void)W.dead(nb >> 5);  // written below: must be present
           const uint32_t* xv = W.tv(x);
           uint32_t fa = xv[face_vi(h, 0)], fb = xv[face_vi(h, 1)], fc = xv[face_vi(h, 2)];
           FaceKey k = face_key(fa, fb, fc);
@@ -413,7 +418,8 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
           int64_t nb = L.a[4 * d + h];
           if (nb < 0) {
             if (rep) rep->message = "region flood reached the hull";
-            return WTET_ERR_INTERNAL;
+            fail = WTET_ERR_INTERNAL;
+            return;
           }
           if (inner.emplace(nb >> 2, kInvalid).second) {
             st.push_back(nb >> 2);
@@ -430,7 +436,8 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
         } else {
           if (mesh.count >= mesh.cap) {
             if (rep) rep->message = "repair needs more room than the staging buffer has";
-            return WTET_ERR_OUT_OF_MEMORY;
+            fail = WTET_ERR_OUT_OF_MEMORY;
+            return;
           }
           slot = (uint32_t)mesh.count++;
           for (int h = 0; h < 4; ++h) W.ta(slot, h) = kInvalid;
@@ -472,6 +479,20 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
       }
       done = true;
     }
+  };
+  while (!queue.empty()) {
+    auto [t, f] = queue.front();
+    for (;;) {
+      try {
+        process(t, f);
+        break;
+      } catch (const NotPresent& np) {
+        if (!mesh.fetch || !mesh.fetch(np.row)) throw;
+        if (rep) rep->fetches++;
+      }
+    }
+    if (fail != WTET_OK) return fail;
+    queue.pop_front();
   }
   // fill holes with tets moved from the end
   std::sort(holes.begin(), holes.end());

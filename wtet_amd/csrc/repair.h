@@ -6,6 +6,7 @@
 
 #include <stdint.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,7 @@ struct RepairReport {
   uint32_t regions = 0;      // regions re-triangulated
   uint32_t max_region = 0;   // largest region (tets)
   bool needs_full = false;   // the repair reached a row the sparse view lacks
+  uint32_t fetches = 0;      // rows fetched into the sparse view on demand
   std::string message;
 };
 
@@ -30,10 +32,14 @@ struct MeshBuf {
   size_t cap;
   std::vector<uint32_t> changed;
   // sparse view: when set, only rows r < known with present[r] != 0 hold
-  // data (rows >= known are the repair's own); reaching another row stops
-  // the repair with needs_full
-  const uint8_t* present = nullptr;
+  // data (rows >= known are the repair's own).  Reaching another row calls
+  // fetch(row), which copies rows around it in and marks them present, and
+  // retries; without fetch, or when it returns false, the repair stops with
+  // needs_full.  Rows the repair has written are never fetched again: every
+  // neighbour of a written row is read before the write.
+  uint8_t* present = nullptr;
   size_t known = 0;
+  std::function<bool(uint32_t)> fetch;
 };
 
 // xyzn: n normalized points (input id order).  viol: (t << 2) | f of
