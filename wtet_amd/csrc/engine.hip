@@ -48,7 +48,7 @@ struct wtet_ctx {
   // point preparation
   DevBuf xyz_in, xyzn, keys, ids, keys2, ids2, sort_tmp, mm6, pts, doff;
   // mesh (per tet slot)
-  DevBuf tet, opp, tinfo, tvote, fvote, tsplit, fit, fround, tmsg, tfwd;
+  DevBuf tet, adj_t, tinfo, tfwd, tvote, fvote, tsplit, tmsg, fit, fround;
   // lists
   DevBuf uvert[2], utet[2], wlist, act[2], prop, fout, blk, slot2idx, rec;
   // output grouping for many clouds
@@ -103,8 +103,8 @@ struct wtet_ctx {
 
   wtet_ctx() {
     DevBuf* l[] = {&xyz_in, &xyzn,   &keys,   &ids,   &keys2,  &ids2,     &sort_tmp, &mm6,
-                   &pts,    &doff,   &tet,    &opp,   &tinfo,  &tvote,    &fvote,    &tsplit,
-                   &fit,    &fround, &tmsg,   &tfwd,  &uvert[0], &uvert[1], &utet[0], &utet[1],
+                   &pts,    &doff,   &tet,    &adj_t,  &tinfo,  &tfwd,  &tvote,    &fvote,    &tsplit,
+                   &tmsg,   &fit,    &fround, &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
                    &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
                    &presize_params, &feat_acc, &feat, &pred, &isums, &slow_t, &slow_p,
@@ -176,20 +176,20 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
   if (cap <= c->cap_tets) return;
   if (cap >= (1ull << 30)) throw Fail{WTET_ERR_OUT_OF_MEMORY, "more than 2^30 tet slots required"};
   ensure(c, c->tet, cap * 16, keep);
-  ensure(c, c->opp, cap * 16, keep);
+  ensure(c, c->adj_t, cap * sizeof(AdjRec), keep);
   ensure(c, c->tinfo, cap * 4, keep);
+  ensure(c, c->tfwd, cap * 4, keep);
   ensure(c, c->tvote, cap * 8, keep);
   ensure(c, c->fvote, cap * 8, keep);
-  ensure(c, c->tsplit, cap * 8, keep);
+  ensure(c, c->tsplit, cap * 16, keep);
+  ensure(c, c->tmsg, cap * sizeof(FlipMsg), keep);
   ensure(c, c->fit, cap * 4, keep);
   ensure(c, c->fround, cap * 4, keep);
-  ensure(c, c->tmsg, cap * 16, keep);
-  ensure(c, c->tfwd, cap * 4, keep);
   ensure(c, c->act[0], cap * 4, keep);
   ensure(c, c->act[1], cap * 4, keep);
   ensure(c, c->slow_t, cap * 4, false);
   ensure(c, c->prop, cap * 16, false);
-  ensure(c, c->fout, cap * 16, false);
+  ensure(c, c->fout, cap * 32, false);  // two uint4 per flip; also the output adjacency
   ensure(c, c->slot2idx, cap * 4, false);
   ensure(c, c->blk, (cap / kBlock + 2) * 4, false);
   if (keep) c->stats.grow_events++;
@@ -242,15 +242,15 @@ Mesh mesh_of(wtet_ctx* c, uint32_t inf, int mode) {
   Mesh M;
   M.pts = P<float4>(c->pts);
   M.tet = P<uint4>(c->tet);
-  M.opp = P<uint4>(c->opp);
+  M.adj = P<AdjRec>(c->adj_t);
   M.tinfo = P<uint32_t>(c->tinfo);
+  M.tfwd = P<uint32_t>(c->tfwd);
   M.tvote = P<uint64_t>(c->tvote);
   M.fvote = P<uint64_t>(c->fvote);
-  M.tsplit = P<uint2>(c->tsplit);
+  M.tsplit = P<uint4>(c->tsplit);
+  M.tmsg = P<FlipMsg>(c->tmsg);
   M.fit = P<uint32_t>(c->fit);
   M.fround = P<uint32_t>(c->fround);
-  M.tmsg = P<uint4>(c->tmsg);
-  M.tfwd = P<uint32_t>(c->tfwd);
   M.ctr = P<uint32_t>(c->ctr);
   M.inf = inf;
   M.mode = mode;
@@ -323,10 +323,10 @@ void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, ui
     return;
   }
   std::vector<uint64_t> key = {
-      (uint64_t)(uintptr_t)M.pts,   (uint64_t)(uintptr_t)M.tet,    (uint64_t)(uintptr_t)M.opp,
-      (uint64_t)(uintptr_t)M.tinfo, (uint64_t)(uintptr_t)M.tvote,  (uint64_t)(uintptr_t)M.fvote,
-      (uint64_t)(uintptr_t)M.tsplit, (uint64_t)(uintptr_t)M.fit,   (uint64_t)(uintptr_t)M.fround,
-      (uint64_t)(uintptr_t)M.tmsg,  (uint64_t)(uintptr_t)M.tfwd,   (uint64_t)(uintptr_t)M.ctr,
+      (uint64_t)(uintptr_t)M.pts,   (uint64_t)(uintptr_t)M.tet,    (uint64_t)(uintptr_t)M.tvote,
+      (uint64_t)(uintptr_t)M.adj,   (uint64_t)(uintptr_t)M.tinfo,  (uint64_t)(uintptr_t)M.tfwd,
+      (uint64_t)(uintptr_t)M.fvote, (uint64_t)(uintptr_t)M.tsplit, (uint64_t)(uintptr_t)M.tmsg,
+      (uint64_t)(uintptr_t)M.ctr,   (uint64_t)(uintptr_t)M.fit,    (uint64_t)(uintptr_t)M.fround,
       (uint64_t)M.inf,              (uint64_t)M.mode,              (uint64_t)(uintptr_t)L.act[0],
       (uint64_t)(uintptr_t)L.act[1], (uint64_t)(uintptr_t)L.prop,  (uint64_t)(uintptr_t)L.out,
       (uint64_t)(uintptr_t)L.slow,  (uint64_t)(uintptr_t)st,       (uint64_t)(uintptr_t)c->isums.p,

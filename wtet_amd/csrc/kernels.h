@@ -55,7 +55,7 @@ void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uin
 struct FlipLists {
   uint32_t* act[2];
   uint4* prop;
-  uint4* out;
+  uint4* out;    // two uint4 per executed flip (flip_exec_body)
   uint32_t* slow;  // entries the FP32-only check deferred (ctr[C_SLOW_T])
 };
 // appends the children to the current active list after its ctr[C_A]

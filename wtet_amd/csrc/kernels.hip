@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(kBlock) k_pick_scatter(Mesh M, const uint32_t*
   uint32_t t = utet[i], v = uvert[i];
   if (w) {
     wlist[r] = make_uint2(t, v);
-    M.tsplit[t] = make_uint2(round, M.ctr[C_BASE] + 3u * r);
+    M.tsplit[t] = make_uint4(round, M.ctr[C_BASE] + 3u * r, v, 0u);
   } else {
     uvert2[i - r] = v;
     utet2[i - r] = t;
@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
     for (uint32_t k = 0; k < nq; ++k) act_next[q + k] = qid[k];
     if (win) {
       uint32_t sl = nslot ? base[1] + rs : kInvalid;
-      flip_exec_body(M, pr, it, round, sl, &L.out[base[0] + rf]);
+      flip_exec_body(M, pr, it, round, sl, &L.out[2 * (size_t)(base[0] + rf)]);
       if (nslot) act_next[q + nq] = sl;
     }
     // the next tile's prefix sums synchronize before base[] is rewritten
@@ -680,7 +680,7 @@ void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64
 // ---------------------------------------------------------------------------
 __global__ void k_alive_count(Mesh M, uint32_t T, uint32_t* blk) {
   uint32_t t = gid();
-  bool a = t < T && (M.tinfo[t] & kAlive);
+  bool a = t < T && (INFO(M, t) & kAlive);
   uint32_t tot;
   block_rank(a, &tot);
   if (threadIdx.x == 0) blk[blockIdx.x] = tot;
@@ -690,7 +690,7 @@ __global__ void k_alive_count(Mesh M, uint32_t T, uint32_t* blk) {
 __global__ void k_alive_scatter(Mesh M, uint32_t T, const uint32_t* blk, uint32_t* olist,
                                 uint32_t* slot2pos) {
   uint32_t t = gid();
-  bool a = t < T && (M.tinfo[t] & kAlive);
+  bool a = t < T && (INFO(M, t) & kAlive);
   uint32_t tot;
   uint32_t r = blk[blockIdx.x] + block_rank(a, &tot);
   if (t < T) {
@@ -703,7 +703,7 @@ __global__ void k_alive_scatter(Mesh M, uint32_t T, const uint32_t* blk, uint32_
 __global__ void k_slot_keys(Mesh M, uint32_t T, Clouds K, uint32_t* okey, uint32_t* oslot) {
   uint32_t t = gid();
   if (t >= T) return;
-  okey[t] = (M.tinfo[t] & kAlive) ? cloud_of_tet(M, t, K) : K.C;
+  okey[t] = (INFO(M, t) & kAlive) ? cloud_of_tet(M, t, K) : K.C;
   oslot[t] = t;
 }
 

@@ -43,22 +43,68 @@ enum Ctr : int {
   C_COUNT = 32
 };
 
+// Adjacency of a tet slot in one 32-byte record: per face f, w[2f] = the
+// neighbour across it ((tet << 2) | face) and w[2f + 1] = that neighbour's
+// vertex opposite the face.  The flip check reads its own slot only, and
+// the adjacency update rewrites a face with one 8-byte store.
+struct __attribute__((aligned(32))) AdjRec {
+  uint4 lo;  // faces 0, 1
+  uint4 hi;  // faces 2, 3
+};
+
+// Where the faces of a tet destroyed by a flip went: for each external face
+// f, w[2f] = (new tet << 2) | face and w[2f + 1] = the new tet's vertex
+// opposite it (one 8-byte load per face in flip_update_body).
+struct __attribute__((aligned(32))) FlipMsg {
+  uint4 lo;  // faces 0, 1
+  uint4 hi;  // faces 2, 3
+};
+
+// message of a destroyed tet: new neighbour pointers e (kInvalid where
+// unused), all opposite the vertex v
+WT_HD void set_msg(FlipMsg* m, uint4 e, uint32_t v) {
+  m->lo = make_uint4(e.x, v, e.y, v);
+  m->hi = make_uint4(e.z, v, e.w, v);
+}
+
+static_assert(sizeof(AdjRec) == 32, "adjacency record layout");
+static_assert(sizeof(FlipMsg) == 32, "flip message layout");
+
 struct Mesh {
   const float4* pts;  // n + 5 points (n real, 4 super, 1 infinity)
   uint4* tet;         // vertex ids
-  uint4* opp;         // neighbours: (tet << 2) | face
+  AdjRec* adj;        // neighbours and the vertices across the faces
   uint32_t* tinfo;    // kAlive | kQueued | kDead
+  uint32_t* tfwd;     // live neighbour of a slot freed by a 3-2 flip
   uint64_t* tvote;    // insertion vote (min key)
   uint64_t* fvote;    // flip vote (min key, iteration-tagged)
-  uint2* tsplit;      // (round, first new slot) for tets split this round
+  uint4* tsplit;      // (round, first new slot, inserted point) for tets split this round
+  FlipMsg* tmsg;      // per slot destroyed by a flip this iteration
   uint32_t* fit;      // flip iteration that destroyed the tet held in the slot
   uint32_t* fround;   // round of the last flip that touched the slot
-  uint4* tmsg;        // where each face of a destroyed tet went
-  uint32_t* tfwd;     // live neighbour of a slot freed by a 3-2 flip
   uint32_t* ctr;      // Ctr counters
   uint32_t inf;       // id of the infinity vertex
   int mode;           // PredMode
 };
+
+WT_HD uint4& TV(const Mesh& M, uint32_t t) { return M.tet[t]; }
+WT_HD uint4 TN(const Mesh& M, uint32_t t) {
+  uint4 a = M.adj[t].lo, b = M.adj[t].hi;
+  return make_uint4(a.x, a.z, b.x, b.z);
+}
+WT_HD uint4 TOV(const Mesh& M, uint32_t t) {
+  uint4 a = M.adj[t].lo, b = M.adj[t].hi;
+  return make_uint4(a.y, a.w, b.y, b.w);
+}
+// both halves of slot t's record: neighbours n and opposite vertices ov
+WT_HD void set_adj(const Mesh& M, uint32_t t, uint4 n, uint4 ov) {
+  M.adj[t].lo = make_uint4(n.x, ov.x, n.y, ov.y);
+  M.adj[t].hi = make_uint4(n.z, ov.z, n.w, ov.w);
+}
+WT_HD uint32_t& INFO(const Mesh& M, uint32_t t) { return M.tinfo[t]; }
+WT_HD uint32_t& FIT(const Mesh& M, uint32_t t) { return M.fit[t]; }
+WT_HD uint32_t& FROUND(const Mesh& M, uint32_t t) { return M.fround[t]; }
+WT_HD uint32_t& TFWD(const Mesh& M, uint32_t t) { return M.tfwd[t]; }
 
 WT_HD float3 P3(const Mesh& M, uint32_t v) {
   float4 p = M.pts[v];
@@ -115,7 +161,16 @@ WT_HD int o3v(const Mesh& M, uint32_t a, uint32_t b, uint32_t c, uint32_t d, Pre
 }
 
 WT_HD uint32_t* opp_word(const Mesh& M, uint32_t t, uint32_t f) {
-  return reinterpret_cast<uint32_t*>(&M.opp[t]) + f;
+  return reinterpret_cast<uint32_t*>(&M.adj[t]) + 2 * f;
+}
+
+WT_HD uint32_t* oppv_word(const Mesh& M, uint32_t t, uint32_t f) {
+  return reinterpret_cast<uint32_t*>(&M.adj[t]) + 2 * f + 1;
+}
+
+// face f of slot t: neighbour pointer and opposite vertex in one store
+WT_HD void set_face(const Mesh& M, uint32_t t, uint32_t f, uint32_t n, uint32_t ov) {
+  reinterpret_cast<uint2*>(&M.adj[t])[f] = make_uint2(n, ov);
 }
 
 // ---------------------------------------------------------------------------
@@ -127,7 +182,7 @@ WT_HD uint32_t* opp_word(const Mesh& M, uint32_t t, uint32_t f) {
 // returns kVoteNone when the point must not vote
 WT_HD uint64_t vote_key(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t i) {
   uint32_t t = utet[i];
-  uint4 tv = M.tet[t];
+  uint4 tv = TV(M, t);
   if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) {
     at_add_u32(&M.ctr[C_BADTET], 1u);
     return kVoteNone;
@@ -162,12 +217,12 @@ WT_HD bool is_winner(const Mesh& M, const uint32_t* utet, uint32_t i, int ring) 
   uint64_t kt = M.tvote[t];
   if ((uint32_t)(kt & 0xFFFFFFFFull) != i) return false;
   if (ring <= 0) return true;
-  uint4 to = M.opp[t];
+  uint4 to = TN(M, t);
   for (int f = 0; f < 4; ++f) {
     uint32_t nt = opp_tet(get4(to, f));
     if (M.tvote[nt] < kt) return false;
     if (ring > 1) {
-      uint4 no = M.opp[nt];
+      uint4 no = TN(M, nt);
       for (int g = 0; g < 4; ++g) {
         uint32_t n2 = opp_tet(get4(no, g));
         if (n2 != t && M.tvote[n2] < kt) return false;
@@ -190,36 +245,44 @@ WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t ro
                       uint32_t* act_next) {
   uint32_t t = wlist[k].x, p = wlist[k].y;
   uint32_t base = M.tsplit[t].y;
-  uint4 tv = M.tet[t];
-  uint4 to = M.opp[t];
+  uint4 tv = TV(M, t);
+  uint4 to = TN(M, t);
+  uint4 tov = TOV(M, t);
   uint32_t c[4] = {t, base, base + 1, base + 2};
   for (int j = 0; j < 4; ++j) {
     uint4 nv = tv;
     set4(nv, j, p);
-    uint4 no;
+    // child i (i != j) keeps t's vertex j opposite their shared face
+    uint4 no, nov;
     for (int i = 0; i < 4; ++i)
-      if (i != j) set4(no, i, enc_opp(c[i], (uint32_t)j));
+      if (i != j) {
+        set4(no, i, enc_opp(c[i], (uint32_t)j));
+        set4(nov, i, get4(tv, j));
+      }
     uint32_t e = get4(to, j);
     uint32_t nt = opp_tet(e), nf = opp_face(e);
-    uint2 ns = M.tsplit[nt];
+    uint4 ns = M.tsplit[nt];
     if (ns.x == round) {
+      // the neighbour splits too: its child nf holds its point opposite
       set4(no, j, enc_opp(child_slot(nt, ns.y, nf), nf));
+      set4(nov, j, ns.z);
     } else {
       set4(no, j, e);
-      *opp_word(M, nt, nf) = enc_opp(c[j], (uint32_t)j);
+      set4(nov, j, get4(tov, j));
+      set_face(M, nt, nf, enc_opp(c[j], (uint32_t)j), p);
     }
-    M.tet[c[j]] = nv;
-    M.opp[c[j]] = no;
+    TV(M, c[j]) = nv;
+    set_adj(M, c[j], no, nov);
     M.tvote[c[j]] = kVoteNone;
     uint32_t old = 0u;
     if (j == 0) {
-      old = at_or_u32(&M.tinfo[t], kQueued);
+      old = at_or_u32(&INFO(M, t), kQueued);
     } else {  // fresh slot: initialise every per-slot field
-      M.tinfo[c[j]] = kAlive | kQueued;
+      INFO(M, c[j]) = kAlive | kQueued;
       M.fvote[c[j]] = kVoteNone;
-      M.tsplit[c[j]] = make_uint2(0u, 0u);
-      M.fit[c[j]] = 0u;
-      M.fround[c[j]] = 0u;
+      M.tsplit[c[j]] = make_uint4(0u, 0u, 0u, 0u);
+      FIT(M, c[j]) = 0u;
+      FROUND(M, c[j]) = 0u;
     }
     if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = c[j];
   }
@@ -233,11 +296,11 @@ template <bool FAST>
 WT_HD bool split_points_body_t(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
                                uint32_t round, PredStats* st) {
   uint32_t t = utet[i];
-  uint2 sp = M.tsplit[t];
+  uint4 sp = M.tsplit[t];
   if (sp.x != round) return false;
   uint32_t base = sp.y;
-  uint4 c0 = M.tet[t];      // (p, v1, v2, v3)
-  uint32_t v0 = M.tet[base].x;  // child 1 = (v0, p, v2, v3)
+  uint4 c0 = TV(M, t);      // (p, v1, v2, v3)
+  uint32_t v0 = TV(M, base).x;  // child 1 = (v0, p, v2, v3)
   uint32_t p = c0.x, v1 = c0.y, v2 = c0.z, v3 = c0.w;
   uint32_t q = uvert[i];
   // children 0 | 1 across plane (p, v2, v3): +1 -> v0's side -> not child 0
@@ -279,20 +342,18 @@ WT_HD bool flip_check_body_t(const Mesh& M, const uint32_t* act, uint4* prop, ui
   uint32_t t = act[j];
   // each tet is on the list once and no other thread of this kernel writes
   // its info word: a plain read-modify-write clears the queued bit
-  uint32_t info = M.tinfo[t];
-  M.tinfo[t] = info & ~kQueued;
+  uint32_t info = INFO(M, t);
+  INFO(M, t) = info & ~kQueued;
   prop[j].x = kInvalid;
   if (!(info & kAlive)) return false;
-  uint4 tv = M.tet[t];
+  uint4 tv = TV(M, t);
   if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return false;
-  uint4 to = M.opp[t];
+  uint4 tov = TOV(M, t);
   float P[4][3];
   uint32_t R[4];
   for (int k = 0; k < 4; ++k) load_pt(M, get4(tv, k), P[k], &R[k]);
   for (int f = 0; f < 4; ++f) {
-    uint32_t e = get4(to, f);
-    uint32_t nt = opp_tet(e), nf = opp_face(e);
-    uint32_t o = get4(M.tet[nt], (int)nf);
+    uint32_t o = get4(tov, f);
     if (o == M.inf) continue;
     float O[3];
     uint32_t ro;
@@ -321,14 +382,13 @@ WT_HD bool flip_check_body_t(const Mesh& M, const uint32_t* act, uint4* prop, ui
       // 3-2 needs the edge to have degree 3: the tet across t's face
       // opposite the third vertex must have o as its apex
       int pthird = pos[(bad + 2) % 3];
-      uint32_t me = get4(to, pthird);
-      uint32_t m = opp_tet(me), mf = opp_face(me);
-      if (get4(M.tet[m], (int)mf) != o) continue;
+      if (get4(tov, pthird) != o) continue;
       kind = 1;
-      third = m;
+      third = opp_tet(*opp_word(M, t, (uint32_t)pthird));
     } else {
       continue;
     }
+    uint32_t nt = opp_tet(*opp_word(M, t, (uint32_t)f));
     uint64_t key = flip_key(it, t);
     at_min_u64(&M.fvote[t], key);
     at_min_u64(&M.fvote[nt], key);
@@ -351,14 +411,15 @@ WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint
 // them once every flip of the iteration is written.
 // s: the fresh slot of a 2-3 flip.  The caller queues the new tets (the
 // reused slots were claimed by flip_decide_body).
-// out: x,y,z = slots of the new tets (z unused for 3-2), w = kind
+// outk[0]: x,y,z = slots of the new tets (z unused for 3-2), w = kind;
+// outk[1]: the flip's vertices o, a and (3-2) z, for the adjacency update
 // ---------------------------------------------------------------------------
 WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t round,
                           uint32_t s, uint4* outk) {
   uint32_t t = pr.x, f = pr.y & 3u, kind = (pr.y >> 2) & 1u, edge = (pr.y >> 3) & 3u;
   uint32_t n = pr.z;
-  uint4 tv = M.tet[t], to = M.opp[t];
-  uint4 nv = M.tet[n], no = M.opp[n];
+  uint4 tv = TV(M, t), to = TN(M, t), tov = TOV(M, t);
+  uint4 nv = TV(M, n), no = TN(M, n), nov = TOV(M, n);
   uint32_t nf = opp_face(get4(to, (int)f));
   uint32_t a = get4(tv, (int)f), o = get4(nv, (int)nf);
   int pos[3] = {face_vi((int)f, 0), face_vi((int)f, 1), face_vi((int)f, 2)};
@@ -366,12 +427,17 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
     uint32_t x = get4(tv, pos[0]), y = get4(tv, pos[1]), z = get4(tv, pos[2]);
     int qx = find4(nv, x), qy = find4(nv, y), qz = find4(nv, z);
     uint32_t Txy = t, Tyz = n, Tzx = s;
-    M.tet[Txy] = make_uint4(x, y, o, a);
-    M.tet[Tyz] = make_uint4(y, z, o, a);
-    M.tet[Tzx] = make_uint4(z, x, o, a);
-    M.opp[Txy] = make_uint4(enc_opp(Tyz, 1), enc_opp(Tzx, 0), get4(to, pos[2]), get4(no, qz));
-    M.opp[Tyz] = make_uint4(enc_opp(Tzx, 1), enc_opp(Txy, 0), get4(to, pos[0]), get4(no, qx));
-    M.opp[Tzx] = make_uint4(enc_opp(Txy, 1), enc_opp(Tyz, 0), get4(to, pos[1]), get4(no, qy));
+    TV(M, Txy) = make_uint4(x, y, o, a);
+    TV(M, Tyz) = make_uint4(y, z, o, a);
+    TV(M, Tzx) = make_uint4(z, x, o, a);
+    // external faces keep the old tets' opposite vertices until
+    // flip_update_body sees a neighbour that flipped too
+    set_adj(M, Txy, make_uint4(enc_opp(Tyz, 1), enc_opp(Tzx, 0), get4(to, pos[2]), get4(no, qz)),
+            make_uint4(z, z, get4(tov, pos[2]), get4(nov, qz)));
+    set_adj(M, Tyz, make_uint4(enc_opp(Tzx, 1), enc_opp(Txy, 0), get4(to, pos[0]), get4(no, qx)),
+            make_uint4(x, x, get4(tov, pos[0]), get4(nov, qx)));
+    set_adj(M, Tzx, make_uint4(enc_opp(Txy, 1), enc_opp(Tyz, 0), get4(to, pos[1]), get4(no, qy)),
+            make_uint4(y, y, get4(tov, pos[1]), get4(nov, qy)));
     uint4 mt = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
     set4(mt, pos[2], enc_opp(Txy, 2));
     set4(mt, pos[0], enc_opp(Tyz, 2));
@@ -380,33 +446,36 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
     set4(mn, qz, enc_opp(Txy, 3));
     set4(mn, qx, enc_opp(Tyz, 3));
     set4(mn, qy, enc_opp(Tzx, 3));
-    M.tmsg[t] = mt;
-    M.tmsg[n] = mn;
-    M.fit[t] = it;
-    M.fit[n] = it;
-    M.fround[t] = round;
-    M.fround[n] = round;
+    set_msg(&M.tmsg[t], mt, o);
+    set_msg(&M.tmsg[n], mn, a);
+    FIT(M, t) = it;
+    FIT(M, n) = it;
+    FROUND(M, t) = round;
+    FROUND(M, n) = round;
     M.tvote[s] = kVoteNone;
     M.fvote[s] = kVoteNone;
-    M.tsplit[s] = make_uint2(0u, 0u);
-    M.fit[s] = 0u;
-    M.fround[s] = 0u;
-    M.tinfo[s] = kAlive | kQueued;  // fresh slot: queued by the caller
-    *outk = make_uint4(Txy, Tyz, Tzx, 0u);
+    M.tsplit[s] = make_uint4(0u, 0u, 0u, 0u);
+    FIT(M, s) = 0u;
+    FROUND(M, s) = 0u;
+    INFO(M, s) = kAlive | kQueued;  // fresh slot: queued by the caller
+    outk[0] = make_uint4(Txy, Tyz, Tzx, 0u);
+    outk[1] = make_uint4(o, a, 0u, 0u);
   } else {
     // rotate the face triple so the offending edge is (x, y), third z
     uint32_t x = get4(tv, pos[edge]), y = get4(tv, pos[(edge + 1) % 3]),
              z = get4(tv, pos[(edge + 2) % 3]);
     int px = pos[edge], py = pos[(edge + 1) % 3];
     uint32_t m = pr.w;
-    uint4 mv = M.tet[m], mo = M.opp[m];
+    uint4 mv = TV(M, m), mo = TN(M, m), mov = TOV(M, m);
     int qx = find4(nv, x), qy = find4(nv, y);
     int mx = find4(mv, x), my = find4(mv, y);
     uint32_t A = t, B = n;
-    M.tet[A] = make_uint4(a, z, o, x);
-    M.tet[B] = make_uint4(z, a, o, y);
-    M.opp[A] = make_uint4(get4(no, qy), get4(mo, my), get4(to, py), enc_opp(B, 3));
-    M.opp[B] = make_uint4(get4(mo, mx), get4(no, qx), get4(to, px), enc_opp(A, 3));
+    TV(M, A) = make_uint4(a, z, o, x);
+    TV(M, B) = make_uint4(z, a, o, y);
+    set_adj(M, A, make_uint4(get4(no, qy), get4(mo, my), get4(to, py), enc_opp(B, 3)),
+            make_uint4(get4(nov, qy), get4(mov, my), get4(tov, py), y));
+    set_adj(M, B, make_uint4(get4(mo, mx), get4(no, qx), get4(to, px), enc_opp(A, 3)),
+            make_uint4(get4(mov, mx), get4(nov, qx), get4(tov, px), x));
     uint4 mt = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
     set4(mt, py, enc_opp(A, 2));
     set4(mt, px, enc_opp(B, 2));
@@ -416,18 +485,19 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
     uint4 mm = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
     set4(mm, my, enc_opp(A, 1));
     set4(mm, mx, enc_opp(B, 0));
-    M.tmsg[t] = mt;
-    M.tmsg[n] = mn;
-    M.tmsg[m] = mm;
-    M.fit[t] = it;
-    M.fit[n] = it;
-    M.fit[m] = it;
-    M.fround[t] = round;
-    M.fround[n] = round;
-    M.fround[m] = round;
-    M.tinfo[m] = kDead;
-    M.tfwd[m] = A;
-    *outk = make_uint4(A, B, kInvalid, 1u);
+    set_msg(&M.tmsg[t], mt, o);
+    set_msg(&M.tmsg[n], mn, a);
+    set_msg(&M.tmsg[m], mm, z);
+    FIT(M, t) = it;
+    FIT(M, n) = it;
+    FIT(M, m) = it;
+    FROUND(M, t) = round;
+    FROUND(M, n) = round;
+    FROUND(M, m) = round;
+    INFO(M, m) = kDead;
+    TFWD(M, m) = A;
+    outk[0] = make_uint4(A, B, kInvalid, 1u);
+    outk[1] = make_uint4(o, a, z, 0u);
   }
 }
 
@@ -451,12 +521,12 @@ WT_HD bool flip_decide_body(const Mesh& M, const uint4& pr, uint32_t it, uint32_
   if (win) {
     uint32_t reuse[2] = {pr.x, pr.z};
     for (int k = 0; k < 2; ++k) {
-      uint32_t old = at_or_u32(&M.tinfo[reuse[k]], kAlive | kQueued);
+      uint32_t old = at_or_u32(&INFO(M, reuse[k]), kAlive | kQueued);
       if (!(old & kQueued)) qid[(*nq)++] = reuse[k];
     }
     if (((pr.y >> 2) & 1u) == 0u) *nslot = 1;
   } else {
-    uint32_t old = at_or_u32(&M.tinfo[pr.x], kQueued);
+    uint32_t old = at_or_u32(&INFO(M, pr.x), kQueued);
     if (!(old & kQueued)) qid[(*nq)++] = pr.x;
   }
   return win;
@@ -473,33 +543,43 @@ WT_HD void flip_decide_exec_body(const Mesh& M, const uint4* prop, uint32_t j, u
   if (!win) return;
   uint32_t s = nslot ? at_add_u32(&M.ctr[C_TETS], 1u) : kInvalid;
   uint32_t f = at_add_u32(&M.ctr[C_FLIPS], 1u);
-  flip_exec_body(M, pr, it, round, s, &out[f]);
+  flip_exec_body(M, pr, it, round, s, &out[2 * (size_t)f]);
   if (nslot) act_next[at_add_u32(&M.ctr[C_ACT_NEXT], 1u)] = s;
 }
 
-WT_HD void fix_face(const Mesh& M, uint32_t T, uint32_t e, uint32_t it) {
+// vT: T's vertex opposite face e
+WT_HD void fix_face(const Mesh& M, uint32_t T, uint32_t e, uint32_t it, uint32_t vT) {
   uint32_t ptr = *opp_word(M, T, e);
   uint32_t q = opp_tet(ptr), g = opp_face(ptr);
-  if (M.fit[q] == it) {
-    *opp_word(M, T, e) = get4(M.tmsg[q], (int)g);
+  if (FIT(M, q) == it) {
+    // the neighbour flipped too: the tet that now holds the face, and its
+    // vertex opposite (written by the other flip before this kernel)
+    uint2 w = reinterpret_cast<const uint2*>(&M.tmsg[q])[g];
+    set_face(M, T, e, w.x, w.y);
   } else {
-    *opp_word(M, q, g) = enc_opp(T, e);
+    set_face(M, q, g, enc_opp(T, e), vT);
   }
 }
 
 // src/shaders/update_opp.wgsl:73-220
 WT_HD void flip_update_body(const Mesh& M, const uint4* out, uint32_t k, uint32_t it) {
-  uint4 r = out[k];
+  uint4 r = out[2 * (size_t)k];
+  uint4 w = out[2 * (size_t)k + 1];  // o, a, z
   if (r.w == 0u) {
+    // new tets (x, y, o, a): external faces 2 (apex o) and 3 (apex a)
     uint32_t T[3] = {r.x, r.y, r.z};
     for (int i = 0; i < 3; ++i) {
-      fix_face(M, T[i], 2, it);
-      fix_face(M, T[i], 3, it);
+      fix_face(M, T[i], 2, it, w.x);
+      fix_face(M, T[i], 3, it, w.y);
     }
   } else {
-    uint32_t T[2] = {r.x, r.y};
-    for (int i = 0; i < 2; ++i)
-      for (uint32_t e = 0; e < 3; ++e) fix_face(M, T[i], e, it);
+    // A = (a, z, o, x), B = (z, a, o, y): external faces 0, 1, 2
+    fix_face(M, r.x, 0, it, w.y);
+    fix_face(M, r.x, 1, it, w.z);
+    fix_face(M, r.x, 2, it, w.x);
+    fix_face(M, r.y, 0, it, w.z);
+    fix_face(M, r.y, 1, it, w.y);
+    fix_face(M, r.y, 2, it, w.x);
   }
 }
 
@@ -518,8 +598,8 @@ WT_HD uint32_t walk_to_t(const Mesh& M, uint32_t t, uint32_t q, uint32_t seed, P
   uint32_t rq;
   load_pt(M, q, Q, &rq);
   for (uint32_t step = 0; step < kWalkLimit; ++step) {
-    uint4 tv = M.tet[t];
-    uint4 to = M.opp[t];
+    uint4 tv = TV(M, t);
+    uint4 to = TN(M, t);
     uint32_t r0 = hash32(seed + step * 0x9E3779B9u) & 3u;
     bool moved = false;
     for (uint32_t kk = 0; kk < 4; ++kk) {
@@ -554,8 +634,8 @@ template <bool FAST>
 WT_HD bool relocate_body_t(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
                            uint32_t round, PredStats* st) {
   uint32_t t = utet[i];
-  if (M.fround[t] != round) return false;
-  for (int guard = 0; guard < 64 && (M.tinfo[t] & kDead); ++guard) t = M.tfwd[t];
+  if (FROUND(M, t) != round) return false;
+  for (int guard = 0; guard < 64 && (INFO(M, t) & kDead); ++guard) t = TFWD(M, t);
   int state = 0;
   t = walk_to_t<FAST>(M, t, uvert[i], i * 0x85EBCA6Bu, st, &state);
   if (state == 2) return true;  // the full walk restarts from utet[i]
@@ -575,18 +655,18 @@ WT_HD void relocate_body(const Mesh& M, const uint32_t* uvert, uint32_t* utet, u
 // rec: (t << 2 | f) of violations, capacity cap
 // ---------------------------------------------------------------------------
 WT_HD void audit_body(const Mesh& M, uint32_t t, uint32_t* rec, uint32_t cap, PredStats* st) {
-  if (!(M.tinfo[t] & kAlive)) return;
-  uint4 tv = M.tet[t];
+  if (!(INFO(M, t) & kAlive)) return;
+  uint4 tv = TV(M, t);
   if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return;
-  uint4 to = M.opp[t];
+  uint4 to = TN(M, t);
+  uint4 tov = TOV(M, t);
   float P[4][3];
   uint32_t R[4];
   for (int k = 0; k < 4; ++k) load_pt(M, get4(tv, k), P[k], &R[k]);
   for (int f = 0; f < 4; ++f) {
-    uint32_t e = get4(to, f);
-    uint32_t nt = opp_tet(e), nf = opp_face(e);
+    uint32_t nt = opp_tet(get4(to, f));
     if (nt < t) continue;
-    uint32_t o = get4(M.tet[nt], (int)nf);
+    uint32_t o = get4(tov, f);
     if (o == M.inf) continue;
     float O[3];
     uint32_t ro;

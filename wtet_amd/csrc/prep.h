@@ -277,21 +277,23 @@ inline StartTets make_start_tets() {
 WT_HD void start_tet_body(const Mesh& M, const StartTets& S, uint32_t g, const Clouds& K) {
   uint32_t c = g / 5u, t = g - 5u * c;
   uint32_t sb = super_base(K.off, c);
-  uint4 tv, to;
+  uint4 tv, to, tov;
   for (int k = 0; k < 4; ++k) {
     uint32_t code = S.tv[4 * t + k];
     set4(tv, k, code == 4u ? K.V : sb + code);
     uint32_t e = S.to[4 * t + k];
     set4(to, k, enc_opp(5u * c + opp_tet(e), opp_face(e)));
+    uint32_t ocode = S.tv[4 * opp_tet(e) + opp_face(e)];  // the neighbour's vertex opposite
+    set4(tov, k, ocode == 4u ? K.V : sb + ocode);
   }
-  M.tet[g] = tv;
-  M.opp[g] = to;
-  M.tinfo[g] = kAlive;
+  TV(M, g) = tv;
+  set_adj(M, g, to, tov);
+  INFO(M, g) = kAlive;
   M.tvote[g] = kVoteNone;
   M.fvote[g] = kVoteNone;
-  M.tsplit[g] = make_uint2(0u, 0u);
-  M.fit[g] = 0u;
-  M.fround[g] = 0u;
+  M.tsplit[g] = make_uint4(0u, 0u, 0u, 0u);
+  FIT(M, g) = 0u;
+  FROUND(M, g) = 0u;
 }
 
 // Exact duplicates (equal normalized coordinates in one cloud) share a key,
@@ -322,7 +324,7 @@ WT_HD bool redundant_dup(const float4* cpts, const uint64_t* keys, uint32_t n, u
 // output: cloud of a live tet = cloud of its smallest vertex (never V)
 WT_HD uint32_t cloud_of_tet(const Mesh& M, uint32_t t, const Clouds& K) {
   if (K.C <= 1) return 0u;
-  uint4 v = M.tet[t];
+  uint4 v = TV(M, t);
   uint32_t m = v.x < v.y ? v.x : v.y;
   m = m < v.z ? m : v.z;
   m = m < v.w ? m : v.w;
@@ -339,7 +341,7 @@ WT_HD void output_body(const Mesh& M, uint32_t j, const uint32_t* olist, const u
   uint32_t c = K.C > 1 ? ocloud[j] : 0u;
   uint32_t base = cstart[c];
   uint32_t inf_id = K.off[c + 1] - K.off[c] + 4u;
-  uint4 tv = M.tet[t], to = M.opp[t];
+  uint4 tv = TV(M, t), to = TN(M, t);
   uint4 ov, oa;
   for (int k = 0; k < 4; ++k) {
     uint32_t v = get4(tv, k);
