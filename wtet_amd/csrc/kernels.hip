@@ -535,14 +535,17 @@ __global__ void __launch_bounds__(kBlock) k_flip_check(Mesh M, FlipLists L, uint
   flush_stats(ls, st);
 }
 
-// exclusive prefix of a small count v (< 4) within the block; *tot = block
-// total.  Every thread of the block must call it; w is LDS of kBlock/64.
-__device__ __forceinline__ uint32_t block_prefix_small(uint32_t v, uint32_t* tot, uint32_t* w) {
+// Exclusive prefixes of three small counts in one pass (a, b in {0, 1},
+// c in 0..3), packed 10 bits each (block totals stay below 1024).  Every
+// thread of the block must call it; w is LDS of kBlock/64.
+__device__ __forceinline__ uint32_t block_prefix3(uint32_t a, uint32_t b, uint32_t c, uint32_t* tot,
+                                                  uint32_t* w) {
   uint64_t lt = (1ull << __lane_id()) - 1ull;
-  uint64_t b0 = __ballot(v & 1u), b1 = __ballot(v & 2u);
-  uint32_t r = __popcll(b0 & lt) + 2u * __popcll(b1 & lt);
+  uint64_t ba = __ballot(a), bb = __ballot(b), c0 = __ballot(c & 1u), c1 = __ballot(c & 2u);
+  auto pack = [](uint32_t x, uint32_t y, uint32_t z) { return x | (y << 10) | (z << 20); };
+  uint32_t r = pack(__popcll(ba & lt), __popcll(bb & lt), __popcll(c0 & lt) + 2u * __popcll(c1 & lt));
   int wave = threadIdx.x >> 6;
-  if (__lane_id() == 0) w[wave] = __popcll(b0) + 2u * __popcll(b1);
+  if (__lane_id() == 0) w[wave] = pack(__popcll(ba), __popcll(bb), __popcll(c0) + 2u * __popcll(c1));
   __syncthreads();
   uint32_t off = 0, t = 0;
 #pragma unroll
@@ -558,10 +561,14 @@ __device__ __forceinline__ uint32_t block_prefix_small(uint32_t v, uint32_t* tot
 // Decide every proposal, then claim list space for the whole block with
 // one atomic per counter (flips, fresh slots, next active entries), then
 // execute the winners.  Counters share a cache line, so per-wave atomics
-// from every wave serialize; per-block claims keep them few.
+// from every wave serialize; per-block claims keep them few.  About a third
+// of the proposals win: the winners are packed through LDS so that the
+// flips run on dense waves.
 __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L) {
   __shared__ uint32_t wsum[kBlock / 64];
   __shared__ uint32_t base[3];
+  __shared__ uint4 wpr[kBlock];      // winning proposals of the tile, packed
+  __shared__ uint32_t wsl[kBlock];   // their fresh slots
   if (!M.ctr[C_RUN]) return;
   uint32_t A = M.ctr[C_A], it = M.ctr[C_IT], round = M.ctr[C_ROUND];
   uint32_t* act_next = L.act[(M.ctr[C_PAR] & 1u) ^ 1u];
@@ -572,10 +579,10 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
     uint32_t qid[2], nq = 0, nslot = 0;
     bool win = flip_decide_body(M, pr, it, qid, &nq, &nslot);
     uint32_t nqa = nq + nslot;  // the fresh slot is queued too
-    uint32_t tf, ts, tq;
-    uint32_t rf = block_prefix_small(win ? 1u : 0u, &tf, wsum);
-    uint32_t rs = block_prefix_small(nslot, &ts, wsum);
-    uint32_t rq = block_prefix_small(nqa, &tq, wsum);
+    uint32_t tot;
+    uint32_t r = block_prefix3(win ? 1u : 0u, nslot, nqa, &tot, wsum);
+    uint32_t rf = r & 1023u, rs = (r >> 10) & 1023u, rq = r >> 20;
+    uint32_t tf = tot & 1023u, ts = (tot >> 10) & 1023u, tq = tot >> 20;
     if (threadIdx.x == 0) {
       base[0] = tf ? atomicAdd(&M.ctr[C_FLIPS], tf) : 0u;
       base[1] = ts ? atomicAdd(&M.ctr[C_TETS], ts) : 0u;
@@ -586,10 +593,16 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
     for (uint32_t k = 0; k < nq; ++k) act_next[q + k] = qid[k];
     if (win) {
       uint32_t sl = nslot ? base[1] + rs : kInvalid;
-      flip_exec_body(M, pr, it, round, sl, &L.out[2 * (size_t)(base[0] + rf)]);
       if (nslot) act_next[q + nq] = sl;
+      wpr[rf] = pr;
+      wsl[rf] = sl;
     }
-    // the next tile's prefix sums synchronize before base[] is rewritten
+    __syncthreads();
+    uint32_t f0 = base[0];
+    for (uint32_t i = threadIdx.x; i < tf; i += kBlock)
+      flip_exec_body(M, wpr[i], it, round, wsl[i], &L.out[2 * (size_t)(f0 + i)]);
+    // the next tile's prefix sums synchronize before wpr, wsl and base are
+    // rewritten
   }
 }
 
