@@ -60,7 +60,8 @@ typedef struct wtet_result {
 
 typedef struct wtet_stats {
   uint64_t pred[8]; /* orient3d: f32, f64, exact nonzero, exact zero (SoS);
-                       insphere: f32, f64, exact nonzero, exact zero (SoS) */
+                       insphere: f32, f64 (filter or double-double adaptive
+                       stage), exact nonzero, exact zero (SoS) */
   uint64_t sum_uninserted; /* sum over rounds of uninserted points            */
   uint64_t sum_active;     /* sum over flip iterations of tets checked         */
   uint64_t sum_flips;      /* flips executed                                  */

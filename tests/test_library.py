@@ -237,3 +237,26 @@ def test_hilbert_index_is_a_hilbert_curve():
     near = np.abs(hh[1:4] - hh[0]).mean()
     far = np.abs(hh[4:6] - hh[0]).mean()
     assert len(set(hh.tolist())) == 6 and near < far
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_host_insphere_far_and_clustered(oracle, mode):
+    """insphere on sets mixing the far super vertices with points clustered
+    within 1e-7..1e-3 of each other: the FP64 filter's bound is dominated by
+    the far points, so these reach the adaptive (double-double) stage and
+    the exact stage; every sign equals the oracle's."""
+    rng = np.random.default_rng(77)
+    sup = np.array([[-100, -100, -100], [400, -100, -100], [-100, 400, -100], [-100, -100, 400]], np.float32)
+    qs = []
+    for k in range(6000):
+        base = rng.random(3).astype(np.float32)
+        spread = np.float32(10.0 ** rng.uniform(-7, -3))
+        q = (base + rng.standard_normal((5, 3)).astype(np.float32) * spread).astype(np.float32)
+        nsup = int(rng.integers(0, 4))
+        pick = rng.permutation(4)[:nsup]
+        q[rng.permutation(5)[:nsup]] = sup[pick]
+        qs.append(q)
+    q = np.stack(qs)
+    idx = np.array([rng.permutation(40)[:5] for _ in range(len(q))], np.uint32)
+    assert np.array_equal(_core.predicates_host(1, q, idx, mode), oracle.orient4d(q.astype(np.float64), idx))
+    assert np.array_equal(_core.predicates_host(3, q, None, mode), oracle.insphere_sign(q.astype(np.float64)))

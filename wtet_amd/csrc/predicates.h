@@ -426,13 +426,115 @@ WT_HD int insphere_f32(const float* a, const float* b, const float* c, const flo
   return 2;
 }
 
+// perm (optional): receives the permanent, for the adaptive stage
 WT_HD int insphere_f64(const float* a, const float* b, const float* c, const float* d,
-                       const float* e) {
+                       const float* e, double* perm = nullptr) {
   InsphereParts<double> r = insphere_eval<double>(a, b, c, d, e);
   const double kB = 1.7763568394003e-15;  // (16 + 224*2^-53) * 2^-53, rounded up
   double eb = kB * r.perm;
+  if (perm) *perm = r.perm;
   if (r.det > eb) return 1;
   if (-r.det > eb) return -1;
+  return 2;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 adaptive stage for insphere: the same determinant in double-double
+// arithmetic (error-free products by fused multiply-add, Knuth's two-sum;
+// T. J. Dekker, Numer. Math. 18, 1971; Y. Hida, X. S. Li, D. H. Bailey,
+// ARITH-15, 2001).  Each double-double operation errs by at most about
+// 4 * 2^-106 times the magnitudes it combines, so the whole evaluation errs
+// by less than 2^-100 times the permanent (the same expression over
+// absolute values); a sign is certified when it clears 2^-90 times the
+// permanent.  Requires every coordinate difference to be exact in f64
+// (checked), which also keeps the low words far above the subnormal range
+// (differences of f32 inputs are >= 2^-149 in magnitude, products of five
+// of them >= 2^-745).
+// ---------------------------------------------------------------------------
+struct DD {
+  double hi, lo;
+};
+
+WT_HD DD dd_two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+
+WT_HD DD dd_quick_two_sum(double a, double b) {
+  double s = a + b;
+  return {s, b - (s - a)};
+}
+
+WT_HD DD dd_two_prod(double a, double b) {
+  double p = a * b;
+  return {p, __builtin_fma(a, b, -p)};
+}
+
+WT_HD DD dd_add(DD a, DD b) {
+  DD s = dd_two_sum(a.hi, b.hi);
+  DD t = dd_two_sum(a.lo, b.lo);
+  s.lo += t.hi;
+  s = dd_quick_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return dd_quick_two_sum(s.hi, s.lo);
+}
+
+WT_HD DD dd_neg(DD a) { return {-a.hi, -a.lo}; }
+
+WT_HD DD dd_mul(DD a, DD b) {
+  DD p = dd_two_prod(a.hi, b.hi);
+  p.lo += a.hi * b.lo + a.lo * b.hi;
+  return dd_quick_two_sum(p.hi, p.lo);
+}
+
+WT_HD DD dd_mul_d(DD a, double b) {
+  DD p = dd_two_prod(a.hi, b);
+  p.lo += a.lo * b;
+  return dd_quick_two_sum(p.hi, p.lo);
+}
+
+// x_i y_j - x_j y_i, exact products
+WT_HD DD dd_minor(double xi, double yj, double xj, double yi) {
+  return dd_add(dd_two_prod(xi, yj), dd_neg(dd_two_prod(xj, yi)));
+}
+
+// perm: the FP64 filter's permanent of the same points
+WT_HD int insphere_dd(const float* a, const float* b, const float* c, const float* d,
+                      const float* e, double perm) {
+  const float* P[4] = {a, b, c, d};
+  double X[4], Y[4], Z[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    DD dx = dd_two_sum((double)P[r][0], -(double)e[0]);
+    DD dy = dd_two_sum((double)P[r][1], -(double)e[1]);
+    DD dz = dd_two_sum((double)P[r][2], -(double)e[2]);
+    if (dx.lo != 0.0 || dy.lo != 0.0 || dz.lo != 0.0) return 2;  // difference not exact
+    X[r] = dx.hi;
+    Y[r] = dy.hi;
+    Z[r] = dz.hi;
+  }
+  DD ab = dd_minor(X[0], Y[1], X[1], Y[0]);
+  DD bc = dd_minor(X[1], Y[2], X[2], Y[1]);
+  DD cd = dd_minor(X[2], Y[3], X[3], Y[2]);
+  DD da = dd_minor(X[3], Y[0], X[0], Y[3]);
+  DD ac = dd_minor(X[0], Y[2], X[2], Y[0]);
+  DD bd = dd_minor(X[1], Y[3], X[3], Y[1]);
+  DD abc = dd_add(dd_add(dd_mul_d(bc, Z[0]), dd_neg(dd_mul_d(ac, Z[1]))), dd_mul_d(ab, Z[2]));
+  DD bcd = dd_add(dd_add(dd_mul_d(cd, Z[1]), dd_neg(dd_mul_d(bd, Z[2]))), dd_mul_d(bc, Z[3]));
+  DD cda = dd_add(dd_add(dd_mul_d(da, Z[2]), dd_mul_d(ac, Z[3])), dd_mul_d(cd, Z[0]));
+  DD dab = dd_add(dd_add(dd_mul_d(ab, Z[3]), dd_mul_d(bd, Z[0])), dd_mul_d(da, Z[1]));
+  DD L[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    L[r] = dd_add(dd_add(dd_two_prod(X[r], X[r]), dd_two_prod(Y[r], Y[r])), dd_two_prod(Z[r], Z[r]));
+  DD det = dd_add(dd_add(dd_mul(L[3], abc), dd_neg(dd_mul(L[2], dab))),
+                  dd_add(dd_mul(L[1], cda), dd_neg(dd_mul(L[0], bcd))));
+  const double kBound = 8.077935669463161e-28;  // 2^-90
+  double eb = kBound * perm;
+  double lo = det.lo < 0 ? -det.lo : det.lo;
+  if (det.hi > 2.0 * eb + lo) return 1;
+  if (-det.hi > 2.0 * eb + lo) return -1;
   return 2;
 }
 
@@ -481,7 +583,9 @@ WT_HD int insphere_raw(const float* a, const float* b, const float* c, const flo
       stat_inc(st, 4);
       return s;
     }
-    s = insphere_f64(a, b, c, d, e);
+    double perm = 0.0;
+    s = insphere_f64(a, b, c, d, e, &perm);
+    if (s == 2) s = insphere_dd(a, b, c, d, e, perm);
     if (s != 2) {
       stat_inc(st, 5);
       return s;
