@@ -153,6 +153,22 @@ T* P(DevBuf& b) {
   return reinterpret_cast<T*>(b.p);
 }
 
+void prefault(void* p, size_t bytes) {
+  const size_t kChunk = 64u << 20;
+  unsigned nt = std::min<unsigned>(8u, (unsigned)((bytes + kChunk - 1) / kChunk));
+  if (nt <= 1) {
+    std::memset(p, 0, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  size_t per = (bytes / nt + 4095) & ~(size_t)4095;
+  for (unsigned i = 0; i < nt; ++i) {
+    size_t a = std::min(bytes, per * i), b = std::min(bytes, per * (i + 1));
+    th.emplace_back([p, a, b] { std::memset(static_cast<char*>(p) + a, 0, b - a); });
+  }
+  for (auto& t : th) t.join();
+}
+
 template <typename T>
 void ensure_pinned(T*& p, size_t& cap, size_t need) {
   if (p && cap >= need) return;
@@ -167,8 +183,10 @@ void ensure_pinned(T*& p, size_t& cap, size_t need) {
   }
   cap = nc;
   // touch every page once: the host maps pinned pages on first write, and
-  // the repair's scattered row writes would otherwise fault page by page
-  std::memset(static_cast<void*>(p), 0, nc * sizeof(T));
+  // the repair's scattered row writes would otherwise fault page by page.
+  // Large buffers are touched by several threads (page faults run in
+  // parallel).
+  prefault(p, nc * sizeof(T));
 }
 
 // per-tet-slot arrays sized for `cap` slots; keep=true preserves contents
