@@ -48,7 +48,7 @@ struct wtet_ctx {
   // point preparation
   DevBuf xyz_in, xyzn, keys, ids, keys2, ids2, sort_tmp, mm6, pts, doff;
   // mesh (per tet slot)
-  DevBuf tet, adj_t, tinfo, tfwd, tvote, fvote, tsplit, tmsg, fit, fround;
+  DevBuf tet, adj_t, tinfo, tfwd, tvote, fvote, tsplit, tpoint, tmsg, fit, fround;
   // lists
   DevBuf uvert[2], utet[2], wlist, act[2], prop, fout, blk, slot2idx, rec;
   // output grouping for many clouds
@@ -103,7 +103,7 @@ struct wtet_ctx {
 
   wtet_ctx() {
     DevBuf* l[] = {&xyz_in, &xyzn,   &keys,   &ids,   &keys2,  &ids2,     &sort_tmp, &mm6,
-                   &pts,    &doff,   &tet,    &adj_t,  &tinfo,  &tfwd,  &tvote,    &fvote,    &tsplit,
+                   &pts,    &doff,   &tet,    &adj_t,  &tinfo,  &tfwd,  &tvote,    &fvote,    &tsplit,  &tpoint,
                    &tmsg,   &fit,    &fround, &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
                    &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
@@ -199,7 +199,8 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
   ensure(c, c->tfwd, cap * 4, keep);
   ensure(c, c->tvote, cap * 8, keep);
   ensure(c, c->fvote, cap * 8, keep);
-  ensure(c, c->tsplit, cap * 16, keep);
+  ensure(c, c->tsplit, cap * 8, keep);
+  ensure(c, c->tpoint, cap * 4, keep);
   ensure(c, c->tmsg, cap * sizeof(FlipMsg), keep);
   ensure(c, c->fit, cap * 4, keep);
   ensure(c, c->fround, cap * 4, keep);
@@ -265,7 +266,8 @@ Mesh mesh_of(wtet_ctx* c, uint32_t inf, int mode) {
   M.tfwd = P<uint32_t>(c->tfwd);
   M.tvote = P<uint64_t>(c->tvote);
   M.fvote = P<uint64_t>(c->fvote);
-  M.tsplit = P<uint4>(c->tsplit);
+  M.tsplit = P<uint2>(c->tsplit);
+  M.tpoint = P<uint32_t>(c->tpoint);
   M.tmsg = P<FlipMsg>(c->tmsg);
   M.fit = P<uint32_t>(c->fit);
   M.fround = P<uint32_t>(c->fround);
@@ -344,6 +346,7 @@ void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, ui
       (uint64_t)(uintptr_t)M.pts,   (uint64_t)(uintptr_t)M.tet,    (uint64_t)(uintptr_t)M.tvote,
       (uint64_t)(uintptr_t)M.adj,   (uint64_t)(uintptr_t)M.tinfo,  (uint64_t)(uintptr_t)M.tfwd,
       (uint64_t)(uintptr_t)M.fvote, (uint64_t)(uintptr_t)M.tsplit, (uint64_t)(uintptr_t)M.tmsg,
+      (uint64_t)(uintptr_t)M.tpoint,
       (uint64_t)(uintptr_t)M.ctr,   (uint64_t)(uintptr_t)M.fit,    (uint64_t)(uintptr_t)M.fround,
       (uint64_t)M.inf,              (uint64_t)M.mode,              (uint64_t)(uintptr_t)L.act[0],
       (uint64_t)(uintptr_t)L.act[1], (uint64_t)(uintptr_t)L.prop,  (uint64_t)(uintptr_t)L.out,
@@ -700,7 +703,7 @@ void dev_output(wtet_ctx* c, Run& R, std::vector<uint32_t>& viol_rec, uint32_t& 
   Mesh M = mesh_of(c, R.V, R.mode);
   launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, s);
   const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
-  timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, R.st, s); });
+  timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, R.st, P<uint32_t>(c->slow_t), s); });
   dev_compact(c, R, T, true, false);
   HIP_CHECK(hipEventRecord(c->ev1, s));
   cstart.resize((size_t)C + 1);
@@ -2229,7 +2232,9 @@ wtet_status wtet_stage_gather(wtet_ctx* c, wtet_stage_info* info) {
     Mesh M = mesh_of(c, S->R.V, S->R.mode);
     launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, c->stream);
     const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
-    timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, S->R.st, c->stream); });
+    timed(c, K_AUDIT, [&] {
+      launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, S->R.st, P<uint32_t>(c->slow_t), c->stream);
+    });
     HIP_CHECK(hipGetLastError());
     read_ctr(c);
     S->violations = c->h_ctr[C_VIOL];

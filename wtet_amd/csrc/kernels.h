@@ -84,8 +84,10 @@ void launch_iter_end(uint32_t* ctr, uint64_t* sums, hipStream_t s);
 void launch_iter_step(uint32_t* ctr, uint64_t* sums, hipStream_t s);
 void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
                      uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s);
+// filtered mode: an FP32-only pass, then the full predicates on the tets it
+// could not decide (slow: T entries of scratch); exact mode: one full pass
 void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st,
-                  hipStream_t s);
+                  uint32_t* slow, hipStream_t s);
 
 // output: live tets grouped by cloud (rows of cloud c from cstart[c],
 // cstart[C] = ctr[C_ALIVE] = rows), cloud-local vertex ids and

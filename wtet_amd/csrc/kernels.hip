@@ -380,7 +380,8 @@ __global__ void __launch_bounds__(kBlock) k_pick_scatter(Mesh M, const uint32_t*
   uint32_t t = utet[i], v = uvert[i];
   if (w) {
     wlist[r] = make_uint2(t, v);
-    M.tsplit[t] = make_uint4(round, M.ctr[C_BASE] + 3u * r, v, 0u);
+    M.tsplit[t] = make_uint2(round, M.ctr[C_BASE] + 3u * r);
+    M.tpoint[t] = v;
   } else {
     uvert2[i - r] = v;
     utet2[i - r] = t;
@@ -676,16 +677,33 @@ void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint3
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_audit(Mesh M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st) {
-  uint32_t t = gid();
+// LEAN: every slot, FP32 filter, undecided tets to slow[ctr[C_SLOW_T]];
+// SLOW: the full predicates on those; FULL: the full predicates on every slot
+template <int FORM>
+__global__ void __launch_bounds__(kBlock) k_audit(Mesh M, uint32_t T, uint32_t* rec, uint32_t cap,
+                                                  uint64_t* st, uint32_t* slow) {
   PredStats ls = {};
-  if (t < T) audit_body(M, t, rec, cap, &ls);
+  if (FORM == SLOW) {
+    uint32_t n = M.ctr[C_SLOW_T], stride = gridDim.x * blockDim.x;
+    for (uint32_t k = gid(); k < n; k += stride) (void)audit_body_t<false>(M, slow[k], rec, cap, &ls);
+  } else {
+    uint32_t t = gid();
+    if (t < T && audit_body_t<FORM == LEAN>(M, t, rec, cap, &ls))
+      slow[append_slot(&M.ctr[C_SLOW_T])] = t;
+  }
   flush_stats(ls, st);
 }
 
 void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st,
-                  hipStream_t s) {
-  if (T) k_audit<<<nblocks(T), kBlock, 0, s>>>(M, T, rec, cap, st);
+                  uint32_t* slow, hipStream_t s) {
+  if (!T) return;
+  if (M.mode == PRED_FILTERED) {
+    k_set_ctr<<<1, 64, 0, s>>>(M.ctr, C_SLOW_T, 0u, -1, 0u);
+    k_audit<LEAN><<<nblocks(T), kBlock, 0, s>>>(M, T, rec, cap, st, slow);
+    k_audit<SLOW><<<std::min<uint32_t>(nblocks(T), 1024u), kBlock, 0, s>>>(M, T, rec, cap, st, slow);
+  } else {
+    k_audit<FULL><<<nblocks(T), kBlock, 0, s>>>(M, T, rec, cap, st, slow);
+  }
 }
 
 // ---------------------------------------------------------------------------

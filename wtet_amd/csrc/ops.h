@@ -78,7 +78,8 @@ struct Mesh {
   uint32_t* tfwd;     // live neighbour of a slot freed by a 3-2 flip
   uint64_t* tvote;    // insertion vote (min key)
   uint64_t* fvote;    // flip vote (min key, iteration-tagged)
-  uint4* tsplit;      // (round, first new slot, inserted point) for tets split this round
+  uint2* tsplit;      // (round, first new slot) for tets split this round
+  uint32_t* tpoint;   // the point inserted into a tet split this round
   FlipMsg* tmsg;      // per slot destroyed by a flip this iteration
   uint32_t* fit;      // flip iteration that destroyed the tet held in the slot
   uint32_t* fround;   // round of the last flip that touched the slot
@@ -261,11 +262,11 @@ WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t ro
       }
     uint32_t e = get4(to, j);
     uint32_t nt = opp_tet(e), nf = opp_face(e);
-    uint4 ns = M.tsplit[nt];
+    uint2 ns = M.tsplit[nt];
     if (ns.x == round) {
       // the neighbour splits too: its child nf holds its point opposite
       set4(no, j, enc_opp(child_slot(nt, ns.y, nf), nf));
-      set4(nov, j, ns.z);
+      set4(nov, j, M.tpoint[nt]);
     } else {
       set4(no, j, e);
       set4(nov, j, get4(tov, j));
@@ -280,7 +281,7 @@ WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t ro
     } else {  // fresh slot: initialise every per-slot field
       INFO(M, c[j]) = kAlive | kQueued;
       M.fvote[c[j]] = kVoteNone;
-      M.tsplit[c[j]] = make_uint4(0u, 0u, 0u, 0u);
+      M.tsplit[c[j]] = make_uint2(0u, 0u);
       FIT(M, c[j]) = 0u;
       FROUND(M, c[j]) = 0u;
     }
@@ -296,7 +297,7 @@ template <bool FAST>
 WT_HD bool split_points_body_t(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
                                uint32_t round, PredStats* st) {
   uint32_t t = utet[i];
-  uint4 sp = M.tsplit[t];
+  uint2 sp = M.tsplit[t];
   if (sp.x != round) return false;
   uint32_t base = sp.y;
   uint4 c0 = TV(M, t);      // (p, v1, v2, v3)
@@ -454,7 +455,7 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
     FROUND(M, n) = round;
     M.tvote[s] = kVoteNone;
     M.fvote[s] = kVoteNone;
-    M.tsplit[s] = make_uint4(0u, 0u, 0u, 0u);
+    M.tsplit[s] = make_uint2(0u, 0u);
     FIT(M, s) = 0u;
     FROUND(M, s) = 0u;
     INFO(M, s) = kAlive | kQueued;  // fresh slot: queued by the caller
@@ -652,17 +653,21 @@ WT_HD void relocate_body(const Mesh& M, const uint32_t* uvert, uint32_t* utet, u
 // ---------------------------------------------------------------------------
 // audit: every finite face, counted once from its lower slot
 // (src/lib.rs:25-78 checks real-only pairs; this covers all finite faces)
-// rec: (t << 2 | f) of violations, capacity cap
+// rec: (t << 2 | f) of violations, capacity cap.  FAST: FP32 filter only;
+// returns true (nothing recorded) when a face is undecided, for the full
+// pass to redo the tet.
 // ---------------------------------------------------------------------------
-WT_HD void audit_body(const Mesh& M, uint32_t t, uint32_t* rec, uint32_t cap, PredStats* st) {
-  if (!(INFO(M, t) & kAlive)) return;
+template <bool FAST>
+WT_HD bool audit_body_t(const Mesh& M, uint32_t t, uint32_t* rec, uint32_t cap, PredStats* st) {
+  if (!(INFO(M, t) & kAlive)) return false;
   uint4 tv = TV(M, t);
-  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return;
+  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return false;
   uint4 to = TN(M, t);
   uint4 tov = TOV(M, t);
   float P[4][3];
   uint32_t R[4];
   for (int k = 0; k < 4; ++k) load_pt(M, get4(tv, k), P[k], &R[k]);
+  int bad[4] = {0, 0, 0, 0};
   for (int f = 0; f < 4; ++f) {
     uint32_t nt = opp_tet(get4(to, f));
     if (nt < t) continue;
@@ -671,12 +676,21 @@ WT_HD void audit_body(const Mesh& M, uint32_t t, uint32_t* rec, uint32_t cap, Pr
     float O[3];
     uint32_t ro;
     load_pt(M, o, O, &ro);
-    if (orient4d(P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, M.mode, st) < 0) {
-      at_add_u32(&M.ctr[C_VIOL], 1u);
-      uint32_t s = at_add_u32(&M.ctr[C_FAILREC], 1u);
-      if (s < cap) rec[s] = (t << 2) | (uint32_t)f;
-    }
+    int s = o4s<FAST>(M, P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, st);
+    if (s == 2) return true;
+    bad[f] = s < 0;
   }
+  for (int f = 0; f < 4; ++f)
+    if (bad[f]) {
+      at_add_u32(&M.ctr[C_VIOL], 1u);
+      uint32_t k = at_add_u32(&M.ctr[C_FAILREC], 1u);
+      if (k < cap) rec[k] = (t << 2) | (uint32_t)f;
+    }
+  return false;
+}
+
+WT_HD void audit_body(const Mesh& M, uint32_t t, uint32_t* rec, uint32_t cap, PredStats* st) {
+  (void)audit_body_t<false>(M, t, rec, cap, st);
 }
 
 }  // namespace wtet

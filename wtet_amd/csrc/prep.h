@@ -291,7 +291,7 @@ WT_HD void start_tet_body(const Mesh& M, const StartTets& S, uint32_t g, const C
   INFO(M, g) = kAlive;
   M.tvote[g] = kVoteNone;
   M.fvote[g] = kVoteNone;
-  M.tsplit[g] = make_uint4(0u, 0u, 0u, 0u);
+  M.tsplit[g] = make_uint2(0u, 0u);
   FIT(M, g) = 0u;
   FROUND(M, g) = 0u;
 }
