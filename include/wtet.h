@@ -307,6 +307,8 @@ typedef struct wtet_stage_info {
   uint32_t checked;    /* tets checked by the last flip iteration           */
   uint32_t flips;      /* flips executed by the last flip iteration         */
   uint32_t violations; /* non-Delaunay faces found by the last gather       */
+  uint32_t deferred;   /* tets the FP32 check of the last flip iteration
+                          left to the full predicates                        */
 } wtet_stage_info;
 
 /* GpuState::new + sort + dispatch_init (src/gpu/mod.rs:48-204, :422-600,

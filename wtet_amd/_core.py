@@ -126,7 +126,7 @@ class _Stats(ctypes.Structure):
 
 class _StageInfo(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in ("round", "uninserted", "winners", "tet_slots", "active",
-                                              "checked", "flips", "violations")]
+                                              "checked", "flips", "violations", "deferred")]
 
 
 _lib = None

@@ -782,7 +782,7 @@ struct wtet_stage_session {
   int phase = 0;
   bool stale = false;        // flips since the last relocation
   bool audit_valid = false;  // the last gather saw the current mesh
-  uint32_t checked = 0, flips = 0, violations = 0, nrec = 0;
+  uint32_t checked = 0, flips = 0, violations = 0, nrec = 0, deferred = 0;
 };
 
 namespace {
@@ -2093,6 +2093,7 @@ void stage_info(wtet_ctx* c, const wtet_stage_session* S, wtet_stage_info* info)
   info->checked = S->checked;
   info->flips = S->flips;
   info->violations = S->violations;
+  info->deferred = S->deferred;
 }
 
 // slots holding tets: between a vote and its split the slots the vote
@@ -2204,6 +2205,8 @@ wtet_status wtet_stage_flip(wtet_ctx* c, int predicate_mode, wtet_stage_info* in
     HIP_CHECK(hipStreamSynchronize(c->stream));
     S->checked = (uint32_t)(after[0] - before[0]);
     S->flips = (uint32_t)(after[1] - before[1]);
+    read_ctr(c);
+    S->deferred = a > 0 ? c->h_ctr[C_SLOW_T] : 0u;
     if (S->flips) {
       S->audit_valid = false;
       if (R.U > 0) S->stale = true;

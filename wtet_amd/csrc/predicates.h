@@ -752,7 +752,9 @@ WT_HDN int orient4d_sos_only(const float* a, const float* b, const float* c, con
     int rk[4] = {s.r0, s.r1, s.r2, s.r3};
     for (int k = 0; k <= (int)s.kind; ++k) perm_coords(pt[rk[k]], s.perm, Q + 3 * k);
     if (s.kind == 3) {
-      r = s.lifted ? exact_orient3d_lifted(Q) : exact_orient3d(Q);
+      // a plain orient3d sign goes through the filters first: the same
+      // exact sign, almost always from the FP32 stage
+      r = s.lifted ? exact_orient3d_lifted(Q) : orient3d_raw(Q, Q + 3, Q + 6, Q + 9, PRED_FILTERED, nullptr);
     } else if (s.kind == 2) {
       r = s.lifted ? exact_orient2d_lifted(Q)
                    : dsign(orient2d_plain(Q[0], Q[1], Q[3], Q[4], Q[6], Q[7]));
