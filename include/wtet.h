@@ -281,10 +281,11 @@ wtet_status wtet_mlp_train(wtet_ctx *ctx, float *params, const float *feat, cons
 
 /* Per-kernel-group timing with HIP events on the context's stream (the
  * reference's GpuProfiler/CpuProfiler scopes, src/profiler.rs:5-224).
- * wtet_set_profiling resets the totals; kinds 0..11 are prepare_points,
+ * wtet_set_profiling resets the totals; kinds 0..12 are prepare_points,
  * vote, pick, split, split_points, flip_check (the FP32-only check),
  * flip_check_deferred (full predicates on the tets it deferred),
- * flip_decide_exec, flip_update, relocate, audit, output.  Totals
+ * flip_decide_exec, flip_update, relocate, audit, output, flip_small (the
+ * one-workgroup iterations of short lists).  Totals
  * accumulate over calls. */
 wtet_status wtet_set_profiling(wtet_ctx *ctx, int on);
 wtet_status wtet_kernel_times(const wtet_ctx *ctx, uint32_t kind, const char **name,

@@ -374,13 +374,15 @@ class Context:
     def kernel_times(self) -> dict:
         """{kernel group: (launches, total ms)} accumulated since set_profiling."""
         out = {}
-        for k in range(12):
+        k = 0
+        while True:
             nm = ctypes.c_char_p()
             cnt = ctypes.c_uint32()
             ms = ctypes.c_double()
-            lib().wtet_kernel_times(self._h, k, ctypes.byref(nm), ctypes.byref(cnt), ctypes.byref(ms))
+            if lib().wtet_kernel_times(self._h, k, ctypes.byref(nm), ctypes.byref(cnt), ctypes.byref(ms)) != 0:
+                return out
             out[nm.value.decode()] = (cnt.value, ms.value)
-        return out
+            k += 1
 
     def info(self) -> str:
         buf = ctypes.create_string_buffer(512)

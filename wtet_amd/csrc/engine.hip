@@ -217,11 +217,11 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
 
 enum KernelKind : int {
   K_PREP = 0, K_VOTE, K_PICK, K_SPLIT, K_SPLIT_POINTS, K_FLIP_CHECK, K_FLIP_CHECK_DEFERRED,
-  K_FLIP_EXEC, K_FLIP_UPDATE, K_RELOCATE, K_AUDIT, K_OUTPUT, K_NKINDS
+  K_FLIP_EXEC, K_FLIP_UPDATE, K_RELOCATE, K_AUDIT, K_OUTPUT, K_FLIP_SMALL, K_NKINDS
 };
 const char* kKernelNames[K_NKINDS] = {"prepare_points", "vote", "pick", "split", "split_points",
                                       "flip_check", "flip_check_deferred", "flip_decide_exec", "flip_update",
-                                      "relocate", "audit", "output"};
+                                      "relocate", "audit", "output", "flip_small"};
 
 // brackets the launches issued by fn with an event pair when profiling
 template <typename Fn>
@@ -330,10 +330,23 @@ uint32_t grid_for(const wtet_ctx* c, uint64_t a, uint32_t lo) {
 // expected (from the previous round).  The deferred check runs the full
 // predicates, whose exact stage keeps a thread busy for microseconds, so its
 // grid follows the deferred count rather than the list.
+// Lists this short run in one workgroup (k_flip_small): launches, not
+// work, set the time of their iterations.  The list may grow a few times
+// within the batch; past kSmallLimit the rest is left to the next batch.
+constexpr uint64_t kSmallList = 512;
+constexpr uint32_t kSmallLimit = 2048;
+
 void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, uint64_t a_hint,
                     uint64_t d_hint) {
   Mesh M = mesh_of(c, inf, mode);
   FlipLists L = lists_of(c);
+  if (a_hint <= kSmallList) {
+    timed(c, K_FLIP_SMALL, [&] {
+      launch_flip_small(M, L, st, P<uint64_t>(c->isums), k, kSmallLimit, c->stream);
+    });
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   uint32_t grid = grid_for(c, a_hint, 16);
   uint32_t dgrid = std::min(grid_for(c, d_hint, 1), grid);
   if (!c->use_graphs || c->profiling) {

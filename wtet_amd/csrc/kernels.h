@@ -78,6 +78,10 @@ void launch_flip_check_deferred(const Mesh& M, const FlipLists& L, uint32_t grid
                                 hipStream_t s);
 void launch_flip_decide_exec(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s);
 void launch_flip_update(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s);
+// up to k iterations of a list of at most `limit` entries in one workgroup
+// (no deferral: the full predicates inline); sums as launch_iter_end
+void launch_flip_small(const Mesh& M, const FlipLists& L, uint64_t* st, uint64_t* sums, int k,
+                       uint32_t limit, hipStream_t s);
 // sums[0] += checked tets, sums[1] += flips, sums[2] += iterations
 void launch_iter_end(uint32_t* ctr, uint64_t* sums, hipStream_t s);
 // end of one iteration followed by the begin of the next

@@ -614,6 +614,69 @@ __global__ void __launch_bounds__(kBlock) k_flip_update(Mesh M, FlipLists L) {
   for (uint32_t k = gid(); k < F; k += stride) flip_update_body(M, L.out, k, it);
 }
 
+// Up to k flip iterations of a short list in one workgroup: the phases of
+// an iteration (check with the full predicates, decide + execute, adjacency
+// update, end) are separated by workgroup barriers instead of launches.
+// Stops at the first iteration whose list is empty or longer than `limit`;
+// a longer list stays for the next batch of iterations.  Counters that
+// other threads of the block change are read with atomic loads.
+__global__ void __launch_bounds__(kBlock) k_flip_small(Mesh M, FlipLists L, uint64_t* st,
+                                                       unsigned long long* sums, int k, uint32_t limit) {
+  __shared__ uint32_t run;
+  PredStats ls = {};
+  uint32_t* ctr = M.ctr;
+  auto ld = [&](int i) { return __atomic_load_n(&ctr[i], __ATOMIC_RELAXED); };
+  const uint32_t round = ld(C_ROUND);
+  for (int i = 0; i < k; ++i) {
+    if (threadIdx.x == 0) {
+      uint32_t A = ld(C_A);
+      uint32_t go = 0u;
+      if (A > 0u && A <= limit && !ld(C_STOP)) {
+        if ((uint64_t)ld(C_TETS) + A + 64u > (uint64_t)ld(C_CAP)) {
+          __atomic_store_n(&ctr[C_STOP], 1u, __ATOMIC_RELAXED);
+        } else {
+          go = 1u;
+          __atomic_store_n(&ctr[C_IT], ld(C_IT) + 1u, __ATOMIC_RELAXED);
+          __atomic_store_n(&ctr[C_ACT_NEXT], 0u, __ATOMIC_RELAXED);
+          __atomic_store_n(&ctr[C_FLIPS], 0u, __ATOMIC_RELAXED);
+          __atomic_store_n(&ctr[C_SLOW_T], 0u, __ATOMIC_RELAXED);
+          if (A > ld(C_PEAKA)) __atomic_store_n(&ctr[C_PEAKA], A, __ATOMIC_RELAXED);
+        }
+      }
+      __atomic_store_n(&ctr[C_RUN], go, __ATOMIC_RELAXED);
+      run = go;
+    }
+    __syncthreads();
+    if (!run) break;
+    uint32_t A = ld(C_A), it = ld(C_IT), par = ld(C_PAR) & 1u;
+    const uint32_t* cur = L.act[par];
+    uint32_t* nxt = L.act[par ^ 1u];
+    for (uint32_t j = threadIdx.x; j < A; j += blockDim.x)
+      (void)flip_check_body_t<false>(M, cur, L.prop, j, it, &ls);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < A; j += blockDim.x)
+      flip_decide_exec_body(M, L.prop, j, it, round, L.out, nxt);
+    __syncthreads();
+    uint32_t F = ld(C_FLIPS);
+    for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) flip_update_body(M, L.out, f, it);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      sums[0] += A;
+      sums[1] += F;
+      sums[2] += 1ull;
+      __atomic_store_n(&ctr[C_A], ld(C_ACT_NEXT), __ATOMIC_RELAXED);
+      __atomic_store_n(&ctr[C_PAR], par ^ 1u, __ATOMIC_RELAXED);
+    }
+    __syncthreads();
+  }
+  flush_stats(ls, st);
+}
+
+void launch_flip_small(const Mesh& M, const FlipLists& L, uint64_t* st, uint64_t* sums, int k,
+                       uint32_t limit, hipStream_t s) {
+  k_flip_small<<<1, kBlock, 0, s>>>(M, L, st, reinterpret_cast<unsigned long long*>(sums), k, limit);
+}
+
 __global__ void k_set_ctr(uint32_t* ctr, int idx, uint32_t v, int idx2, uint32_t v2) {
   if (threadIdx.x == 0) {
     ctr[idx] = v;
