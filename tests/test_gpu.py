@@ -333,7 +333,9 @@ def test_resident_repair_fetches_missing_rows(oracle, ctx, rings):
     finally:
         ctx.set_repair_rings(6)
         ctx.free(d)
-    assert st["violations_gpu"] > 0 and st["repair_fetches"] > 0, st
+    assert st["violations_gpu"] > 0, st
+    if rings == 1:  # one ring is never enough for this input's regions
+        assert st["repair_fetches"] > 0, st
     host = ctx.delaunay(p)
     assert np.array_equal(oracle.canonical_real(tets, len(p)), oracle.canonical_real(host.tets, len(p)))
     back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
