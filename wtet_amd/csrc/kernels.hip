@@ -578,8 +578,7 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
     uint4 pr = make_uint4(kInvalid, 0u, 0u, 0u);
     if (j < A) pr = L.prop[j];
     uint32_t qid[2], nq = 0, nslot = 0;
-    bool win = flip_decide_plain(M, pr, it, qid, &nq, &nslot);
-    if (!win && pr.x != kInvalid) L.prop[j].y = pr.y | kLost;
+    bool win = flip_decide_body(M, pr, it, qid, &nq, &nslot);
     uint32_t nqa = nq + nslot;  // the fresh slot is queued too
     uint32_t tot;
     uint32_t r = block_prefix3(win ? 1u : 0u, nslot, nqa, &tot, wsum);
@@ -608,28 +607,11 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
   }
 }
 
-// The adjacency update, then the losing proposals' tets back on the list
-// (one returning atomic per wave that has any).
 __global__ void __launch_bounds__(kBlock) k_flip_update(Mesh M, FlipLists L) {
   if (!M.ctr[C_RUN]) return;
-  uint32_t F = M.ctr[C_FLIPS], it = M.ctr[C_IT], A = M.ctr[C_A];
-  uint32_t* act_next = L.act[(M.ctr[C_PAR] & 1u) ^ 1u];
+  uint32_t F = M.ctr[C_FLIPS], it = M.ctr[C_IT];
   uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t k = gid(); k < F; k += stride) flip_update_body(M, L.out, k, it);
-  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < A; b0 += stride) {
-    uint32_t j = b0 + threadIdx.x;
-    uint4 pr = make_uint4(kInvalid, 0u, 0u, 0u);
-    if (j < A) pr = L.prop[j];
-    bool q = flip_requeue_loser(M, pr, it);
-    uint64_t m = __ballot(q);
-    if (!m) continue;
-    uint32_t lane = __lane_id();
-    uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&M.ctr[C_ACT_NEXT], (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    if (q) act_next[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = pr.x;
-  }
 }
 
 // Up to k flip iterations of a short list in one workgroup: the phases of

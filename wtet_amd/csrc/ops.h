@@ -533,42 +533,6 @@ WT_HD bool flip_decide_body(const Mesh& M, const uint4& pr, uint32_t it, uint32_
   return win;
 }
 
-// Decide without atomics (the list kernels): a winner claims its two reused
-// slots with plain stores (no other proposal involves them); a loser is
-// marked kLost in its proposal and left to flip_requeue_loser, which runs
-// after every flip of the iteration is executed.
-constexpr uint32_t kLost = 1u << 31;
-
-WT_HD bool flip_decide_plain(const Mesh& M, const uint4& pr, uint32_t it, uint32_t qid[2],
-                             uint32_t* nq, uint32_t* nslot) {
-  *nq = 0;
-  *nslot = 0;
-  if (pr.x == kInvalid) return false;
-  uint64_t key = flip_key(it, pr.x);
-  bool win = M.fvote[pr.x] == key && M.fvote[pr.z] == key;
-  if (((pr.y >> 2) & 1u) && M.fvote[pr.w] != key) win = false;
-  if (win) {
-    INFO(M, pr.x) = kAlive | kQueued;
-    INFO(M, pr.z) = kAlive | kQueued;
-    qid[0] = pr.x;
-    qid[1] = pr.z;
-    *nq = 2;
-    if (((pr.y >> 2) & 1u) == 0u) *nslot = 1;
-  }
-  return win;
-}
-
-// A losing proposal's tet goes back on the list unless a winning flip of
-// this iteration reused or freed its slot (those slots carry fit == it, and
-// the winner queued the reused ones).  Each tet proposes at most once, so no
-// other thread queues it.
-WT_HD bool flip_requeue_loser(const Mesh& M, const uint4& pr, uint32_t it) {
-  if (pr.x == kInvalid || !(pr.y & kLost)) return false;
-  if (FIT(M, pr.x) == it) return false;
-  INFO(M, pr.x) |= kQueued;
-  return true;
-}
-
 // Sequential form (one proposal at a time, host callers): counters advance
 // with one atomic each.
 WT_HD void flip_decide_exec_body(const Mesh& M, const uint4* prop, uint32_t j, uint32_t it,
