@@ -120,6 +120,18 @@ def main():
     print(f"filtered: {ms:.2f} ms/step, {value:.4g} points/s", file=sys.stderr, flush=True)
     st_t = ctx.stats()
 
+    # end to end: host points in, host tets and adjacency out (PCIe both ways)
+    ctx.delaunay(pts, filt)
+    t0 = time.perf_counter()
+    e2e_k = max(1, min(args.steps, 3))
+    for _ in range(e2e_k):
+        r_e = ctx.delaunay(pts, filt)
+    ms_e2e = (time.perf_counter() - t0) * 1e3 / e2e_k
+    end_to_end = {"value": round(args.n / (ms_e2e / 1e3), 1), "ms_per_step": round(ms_e2e, 3),
+                  "steps": e2e_k, "bytes_in": int(pts.nbytes),
+                  "bytes_out": int(r_e.tets.nbytes + r_e.adjacency.nbytes)}
+    print(f"end to end (host in, host out): {ms_e2e:.2f} ms/step", file=sys.stderr, flush=True)
+
     # live per-kernel timing (HIP events on the library's stream)
     ctx.set_profiling(True)
     ctx.delaunay_resident(d_pts, args.n, filt)
@@ -279,6 +291,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_exact": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
+        "end_to_end": end_to_end,
         "batched": batch,
         "stress": stress,
         "presize": presized,

@@ -341,24 +341,27 @@ template <bool FAST>
 WT_HD bool flip_check_body_t(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t j,
                              uint32_t it, PredStats* st) {
   uint32_t t = act[j];
+  // Two memory round trips: the slot's info, vertices and opposite vertices
+  // together (all in the slot's own records), then all eight points (the
+  // point array includes the infinity vertex, so every id is loadable).
+  uint32_t info = INFO(M, t);
+  uint4 tv = TV(M, t);
+  uint4 tov = TOV(M, t);
   // each tet is on the list once and no other thread of this kernel writes
   // its info word: a plain read-modify-write clears the queued bit
-  uint32_t info = INFO(M, t);
   INFO(M, t) = info & ~kQueued;
   prop[j].x = kInvalid;
   if (!(info & kAlive)) return false;
-  uint4 tv = TV(M, t);
   if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) return false;
-  uint4 tov = TOV(M, t);
-  float P[4][3];
-  uint32_t R[4];
+  float P[4][3], Q[4][3];
+  uint32_t R[4], RQ[4];
   for (int k = 0; k < 4; ++k) load_pt(M, get4(tv, k), P[k], &R[k]);
+  for (int k = 0; k < 4; ++k) load_pt(M, get4(tov, k), Q[k], &RQ[k]);
   for (int f = 0; f < 4; ++f) {
     uint32_t o = get4(tov, f);
     if (o == M.inf) continue;
-    float O[3];
-    uint32_t ro;
-    load_pt(M, o, O, &ro);
+    const float* O = Q[f];
+    uint32_t ro = RQ[f];
     int ins = o4s<FAST>(M, P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, st);
     if (ins == 2) return true;
     if (ins > 0) continue;
