@@ -242,14 +242,25 @@ WT_HD uint32_t child_slot(uint32_t t, uint32_t base, uint32_t j) {
   return j == 0 ? t : base + j - 1;
 }
 
-WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t round,
-                      uint32_t* act_next) {
+// c[0..3] receives the children's slots; returns which of them join the
+// active list (bit j for child j; slot t only if no earlier list holds it).
+// The neighbours' split records are loaded before any store: this kernel
+// writes those of fresh slots only, never a neighbour's.
+WT_HD uint32_t split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t round,
+                          uint32_t c[4]) {
   uint32_t t = wlist[k].x, p = wlist[k].y;
   uint32_t base = M.tsplit[t].y;
   uint4 tv = TV(M, t);
   uint4 to = TN(M, t);
   uint4 tov = TOV(M, t);
-  uint32_t c[4] = {t, base, base + 1, base + 2};
+  c[0] = t, c[1] = base, c[2] = base + 1, c[3] = base + 2;
+  uint2 ns[4];
+  uint32_t np[4];
+  for (int j = 0; j < 4; ++j) {
+    uint32_t nt = opp_tet(get4(to, j));
+    ns[j] = M.tsplit[nt];
+    np[j] = M.tpoint[nt];  // read for every neighbour, used where it splits
+  }
   for (int j = 0; j < 4; ++j) {
     uint4 nv = tv;
     set4(nv, j, p);
@@ -262,11 +273,10 @@ WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t ro
       }
     uint32_t e = get4(to, j);
     uint32_t nt = opp_tet(e), nf = opp_face(e);
-    uint2 ns = M.tsplit[nt];
-    if (ns.x == round) {
+    if (ns[j].x == round) {
       // the neighbour splits too: its child nf holds its point opposite
-      set4(no, j, enc_opp(child_slot(nt, ns.y, nf), nf));
-      set4(nov, j, M.tpoint[nt]);
+      set4(no, j, enc_opp(child_slot(nt, ns[j].y, nf), nf));
+      set4(nov, j, np[j]);
     } else {
       set4(no, j, e);
       set4(nov, j, get4(tov, j));
@@ -275,18 +285,16 @@ WT_HD void split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_t ro
     TV(M, c[j]) = nv;
     set_adj(M, c[j], no, nov);
     M.tvote[c[j]] = kVoteNone;
-    uint32_t old = 0u;
-    if (j == 0) {
-      old = at_or_u32(&INFO(M, t), kQueued);
-    } else {  // fresh slot: initialise every per-slot field
+    if (j > 0) {  // fresh slot: initialise every per-slot field
       INFO(M, c[j]) = kAlive | kQueued;
       M.fvote[c[j]] = kVoteNone;
       M.tsplit[c[j]] = make_uint2(0u, 0u);
       FIT(M, c[j]) = 0u;
       FROUND(M, c[j]) = 0u;
     }
-    if (!(old & kQueued)) act_next[append_slot(&M.ctr[C_ACT_NEXT])] = c[j];
   }
+  uint32_t old = at_or_u32(&INFO(M, t), kQueued);
+  return (old & kQueued) ? 0xEu : 0xFu;
 }
 
 // ---------------------------------------------------------------------------
@@ -551,39 +559,39 @@ WT_HD void flip_decide_exec_body(const Mesh& M, const uint4* prop, uint32_t j, u
   if (nslot) act_next[at_add_u32(&M.ctr[C_ACT_NEXT], 1u)] = s;
 }
 
-// vT: T's vertex opposite face e
-WT_HD void fix_face(const Mesh& M, uint32_t T, uint32_t e, uint32_t it, uint32_t vT) {
-  uint32_t ptr = *opp_word(M, T, e);
-  uint32_t q = opp_tet(ptr), g = opp_face(ptr);
-  if (FIT(M, q) == it) {
-    // the neighbour flipped too: the tet that now holds the face, and its
-    // vertex opposite (written by the other flip before this kernel)
-    uint2 w = reinterpret_cast<const uint2*>(&M.tmsg[q])[g];
-    set_face(M, T, e, w.x, w.y);
-  } else {
-    set_face(M, q, g, enc_opp(T, e), vT);
-  }
-}
-
-// src/shaders/update_opp.wgsl:73-220
+// src/shaders/update_opp.wgsl:73-220.  The flip's six external faces
+// (tet T, face e, T's vertex vT opposite e): a face whose neighbour q
+// flipped too takes the tet and vertex that q's flip left in its message;
+// otherwise q's face points back at T.  Every load comes before the first
+// store (the flip record, the pointers, then tags and messages: three round
+// trips): no thread of this kernel
+// stores a word that one of these loads reads (a flipped tet's external face
+// is rewritten only by its own flip, after the read; an unflipped
+// neighbour's face is never read; tags and messages are read-only here).
 WT_HD void flip_update_body(const Mesh& M, const uint4* out, uint32_t k, uint32_t it) {
   uint4 r = out[2 * (size_t)k];
   uint4 w = out[2 * (size_t)k + 1];  // o, a, z
-  if (r.w == 0u) {
-    // new tets (x, y, o, a): external faces 2 (apex o) and 3 (apex a)
-    uint32_t T[3] = {r.x, r.y, r.z};
-    for (int i = 0; i < 3; ++i) {
-      fix_face(M, T[i], 2, it, w.x);
-      fix_face(M, T[i], 3, it, w.y);
-    }
-  } else {
-    // A = (a, z, o, x), B = (z, a, o, y): external faces 0, 1, 2
-    fix_face(M, r.x, 0, it, w.y);
-    fix_face(M, r.x, 1, it, w.z);
-    fix_face(M, r.x, 2, it, w.x);
-    fix_face(M, r.y, 0, it, w.z);
-    fix_face(M, r.y, 1, it, w.y);
-    fix_face(M, r.y, 2, it, w.x);
+  // 2-3: new tets (x, y, o, a), external faces 2 (apex o) and 3 (apex a);
+  // 3-2: A = (a, z, o, x), B = (z, a, o, y), external faces 0, 1, 2.
+  // Selected without a branch, so the record is read in one round trip.
+  bool k32 = r.w != 0u;
+  uint32_t T[6] = {r.x, r.x, k32 ? r.x : r.y, r.y, k32 ? r.y : r.z, k32 ? r.y : r.z};
+  uint32_t E[6] = {k32 ? 0u : 2u, k32 ? 1u : 3u, 2u, k32 ? 0u : 3u, k32 ? 1u : 2u, k32 ? 2u : 3u};
+  uint32_t V[6] = {k32 ? w.y : w.x, k32 ? w.z : w.y, w.x, k32 ? w.z : w.y, k32 ? w.y : w.x,
+                   k32 ? w.x : w.y};
+  uint32_t ptr[6], fq[6];
+  uint2 msg[6];
+  for (int i = 0; i < 6; ++i) ptr[i] = *opp_word(M, T[i], E[i]);
+  for (int i = 0; i < 6; ++i) fq[i] = FIT(M, opp_tet(ptr[i]));
+  // every neighbour is a valid slot: its message is loaded unconditionally
+  // (stale where it did not flip, then unused), so no branch splits the loads
+  for (int i = 0; i < 6; ++i)
+    msg[i] = reinterpret_cast<const uint2*>(&M.tmsg[opp_tet(ptr[i])])[opp_face(ptr[i])];
+  // one 8-byte store per face, its slot and value chosen without a branch
+  for (int i = 0; i < 6; ++i) {
+    bool fl = fq[i] == it;
+    uint32_t dt = fl ? T[i] : opp_tet(ptr[i]), df = fl ? E[i] : opp_face(ptr[i]);
+    set_face(M, dt, df, fl ? msg[i].x : enc_opp(T[i], E[i]), fl ? msg[i].y : V[i]);
   }
 }
 
