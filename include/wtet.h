@@ -89,6 +89,11 @@ typedef struct wtet_stats {
   double ms_repair_staging;       /* of ms_repair: rows and points to the host */
   double ms_repair_compute;       /* of ms_repair: the host re-triangulation   */
   uint32_t repair_fetches;        /* rows fetched on demand during the repair */
+  uint32_t repair_retries;        /* device phases rerun with a wider insertion
+                                     ring because a repair region outgrew its
+                                     cap (the stats above are those of the
+                                     last run)                                */
+  uint32_t repair_max_region;     /* tets in the largest repair region        */
 } wtet_stats;
 
 typedef struct wtet_ctx wtet_ctx;
@@ -222,6 +227,12 @@ wtet_status wtet_set_insertion_warmup(wtet_ctx *ctx, uint32_t rounds);
  * repair that needs more falls back to every row.  No counterpart in the
  * reference. */
 wtet_status wtet_set_repair_rings(wtet_ctx *ctx, int rings);
+/* A single-cloud call whose host repair grows a region past `tets` tets
+ * (default 16,384; 0: no limit) stops that repair and reruns the device
+ * phase with the insertion ring one wider (at most 2), whose repair then
+ * has no limit.  The triangulation does not depend on it; the stats count
+ * the reruns (repair_retries).  No counterpart in the reference. */
+wtet_status wtet_set_repair_region_cap(wtet_ctx *ctx, uint32_t tets);
 
 /* The flip iterations of a round run on the device without host reads
  * (counts, iteration tag and list parity live in device counters); they

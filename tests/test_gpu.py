@@ -317,6 +317,35 @@ def test_hilbert_order_in_a_batch(oracle, ctx):
         assert np.array_equal(oracle.canonical_real(res[c].tets, len(p)), ref), c
 
 
+def test_oversized_repair_region_reruns_with_wider_ring(oracle, ctx):
+    """A repair region past the cap stops the host repair and reruns the
+    device phase with the insertion ring one wider; the rerun's repair has
+    no cap.  A cap of one tet forces it whenever the GPU leaves a stuck face
+    (200,000 points on a sphere always leave some).  The host-result and the
+    resident calls both give the oracle's tets."""
+    p = W.sphere(200000, 5)
+    n = len(p)
+    d = ctx.upload(p)
+    ctx.set_repair_region_cap(1)
+    try:
+        forced = ctx.delaunay(p)
+        st = ctx.stats()
+        m, dt, da = ctx.delaunay_resident(d, n)
+        st_r = ctx.stats()
+        tets = ctx.download(dt, (m, 4), np.uint32)
+        adj = ctx.download(da, (m, 4), np.uint32)
+    finally:
+        ctx.set_repair_region_cap(1 << 14)
+        ctx.free(d)
+    assert st["repair_retries"] == 1 and st_r["repair_retries"] == 1, (st, st_r)
+    pn, _, _ = oracle.normalize_points(p)
+    want = oracle.canonical_real(oracle.delaunay(pn), n)
+    assert np.array_equal(oracle.canonical_real(forced.tets, n), want)
+    assert np.array_equal(oracle.canonical_real(tets, n), want)
+    back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
+    assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))
+
+
 @pytest.mark.parametrize("rings", [1, 2])
 def test_resident_repair_fetches_missing_rows(oracle, ctx, rings):
     """With a thin staged neighbourhood the repair reaches rows the host

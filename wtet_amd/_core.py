@@ -121,6 +121,8 @@ class _Stats(ctypes.Structure):
         ("ms_repair_staging", ctypes.c_double),
         ("ms_repair_compute", ctypes.c_double),
         ("repair_fetches", ctypes.c_uint32),
+        ("repair_retries", ctypes.c_uint32),
+        ("repair_max_region", ctypes.c_uint32),
     ]
 
 
@@ -180,6 +182,7 @@ def lib():
             L.wtet_set_insertion_ring.argtypes = [P, ctypes.c_int]
             L.wtet_set_insertion_warmup.argtypes = [P, u32]
             L.wtet_set_repair_rings.argtypes = [P, ctypes.c_int]
+            L.wtet_set_repair_region_cap.argtypes = [P, u32]
             L.wtet_set_flip_graphs.argtypes = [P, ctypes.c_int]
             L.wtet_set_flip_items.argtypes = [P, u32]
             L.wtet_get_flip_items.argtypes = [P, P, P]
@@ -337,6 +340,12 @@ class Context:
         """Rows within `rings` face-steps of each stuck face go to the host repair."""
         if lib().wtet_set_repair_rings(self._h, int(rings)) != 0:
             raise ValueError("rings must be in 1..16")
+
+    def set_repair_region_cap(self, tets: int):
+        """A repair region past `tets` tets reruns the device phase with the
+        insertion ring one wider (0: no limit; default 16,384)."""
+        if lib().wtet_set_repair_region_cap(self._h, int(tets)) != 0:
+            raise ValueError(self.error())
 
     def set_flip_graphs(self, on: bool):
         """Replay the captured graph of a round's flip iterations (default) or

@@ -95,6 +95,9 @@ struct wtet_ctx {
   // the largest cloud (warmup_rounds)
   uint32_t ring0_rounds = 0xFFFFFFFFu;
   int repair_rings = 6;       // face-steps of rows staged around each stuck face
+  // a single-cloud repair region above this many tets stops the repair and
+  // reruns the device phase with a wider insertion ring (0: never)
+  uint32_t region_cap = 1u << 14;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
   std::vector<int> evkind;
   size_t evused = 0;
@@ -957,9 +960,14 @@ std::vector<uint32_t> failed_vertices(const uint32_t* tets, const uint32_t* adj,
 // Host side after the device phase of one cloud: failed_verts, the host
 // repair (src/lib.rs:227-307) and the result, as host arrays (out) or as
 // device rows (dout).
+// Thrown by finish_single when a repair region outgrew region_cap: the
+// caller reruns the device phase with a wider insertion ring.
+struct WiderRing {};
+
 void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counters& K,
                    std::vector<uint32_t>& rec, uint32_t nviol, uint32_t mout, wtet_result* out,
-                   wtet_device_result* dout, std::chrono::steady_clock::time_point t0) {
+                   wtet_device_result* dout, std::chrono::steady_clock::time_point t0,
+                   uint32_t region_cap) {
   const bool resident = dout != nullptr;
   c->stats.violations_gpu = nviol;
   auto t1 = std::chrono::steady_clock::now();
@@ -989,15 +997,17 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     auto tc = std::chrono::steady_clock::now();
     c->stats.ms_repair_staging = std::chrono::duration<double, std::milli>(tc - t1).count();
     RepairReport rr;
-    wtet_status rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr);
-    if (rs != WTET_OK && rr.needs_full) {
+    wtet_status rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr, region_cap);
+    if (rs != WTET_OK && rr.needs_full && !rr.too_large) {
       // the neighbourhood was too small: every row, and again from the start
       download_rows(c, mout);
       mb = MeshBuf{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
       rr = RepairReport{};
-      rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr);
+      rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr, region_cap);
       c->stats.repair_full_downloads++;
     }
+    c->stats.repair_max_region = rr.max_region;
+    if (rs != WTET_OK && rr.too_large) throw WiderRing{};
     if (rs != WTET_OK) throw Fail{rs, "host repair failed: " + rr.message};
     c->stats.repair_regions = rr.regions;
     c->stats.repair_fetches = std::min(fetches, kMaxFetches);
@@ -1072,12 +1082,36 @@ void delaunay_impl(void* a_) {
     HIP_CHECK(hipMemcpyAsync(c->xyz_in.p, a.xyz, (size_t)a.n * 12, hipMemcpyHostToDevice, c->stream));
     d_xyz = P<float>(c->xyz_in);
   }
-  Counters K;
-  std::vector<uint32_t> rec, cstart;
-  uint32_t nviol = 0, mout = 0;
   bool resident = a.dout != nullptr;
-  run_device(c, d_xyz, {0u, a.n}, a.cfg, K, rec, nviol, resident, mout, cstart);
-  finish_single(c, a.n, a.cfg, K, rec, nviol, mout, a.out, a.dout, t0);
+  // Dense stuck faces (a flip trajectory that left thousands, as happens on
+  // co-spherical inputs) can merge the repair's regions into one that spans
+  // the mesh, a host re-triangulation of every point.  A region past
+  // region_cap instead reruns the device phase with the insertion ring one
+  // wider (fewer simultaneous insertions near each other, fewer stuck
+  // faces); the rerun's repair has no cap.
+  const int ring0 = c->ring;
+  struct RingGuard {
+    wtet_ctx* c;
+    int ring;
+    ~RingGuard() { c->ring = ring; }
+  } guard{c, ring0};
+  uint32_t retries = 0;
+  for (;;) {
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    Counters K;
+    std::vector<uint32_t> rec, cstart;
+    uint32_t nviol = 0, mout = 0;
+    run_device(c, d_xyz, {0u, a.n}, a.cfg, K, rec, nviol, resident, mout, cstart);
+    uint32_t cap = c->ring < 2 ? c->region_cap : 0u;
+    try {
+      finish_single(c, a.n, a.cfg, K, rec, nviol, mout, a.out, a.dout, t0, cap);
+      break;
+    } catch (const WiderRing&) {
+      ++retries;
+      c->ring += 1;
+    }
+  }
+  c->stats.repair_retries = retries;
 }
 
 
@@ -1960,6 +1994,12 @@ wtet_status wtet_set_repair_rings(wtet_ctx* c, int rings) {
   return WTET_OK;
 }
 
+wtet_status wtet_set_repair_region_cap(wtet_ctx* c, uint32_t tets) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  c->region_cap = tets;
+  return WTET_OK;
+}
+
 wtet_status wtet_set_insertion_warmup(wtet_ctx* c, uint32_t rounds) {
   if (!c) return WTET_ERR_INVALID_ARGUMENT;
   c->ring0_rounds = rounds;
@@ -2346,7 +2386,7 @@ wtet_status wtet_stage_finish(wtet_ctx* c, wtet_result* out) {
     uint32_t n = R.n;
     wtet_config cfg = R.cfg;
     end_stage(c);
-    finish_single(c, n, cfg, K, rec, nviol, mout, out, nullptr, t0);
+    finish_single(c, n, cfg, K, rec, nviol, mout, out, nullptr, t0, 0u);
   });
   if (st != WTET_OK) {
     wtet_result_free(out);

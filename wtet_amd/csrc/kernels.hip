@@ -410,24 +410,30 @@ void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uin
   if (ring > 0) k_clear_votes<<<nb, kBlock, 0, s>>>(M, utet2, M.ctr);
 }
 
-// The children join the active list with one counter atomic per wave.
+// The children join the active list with one counter atomic per wave, in
+// child-major order within the wave (the wave's child-0 slots, then its
+// child-1 slots, ...).  The order matters: the fresh slots of the flips
+// that follow are numbered in list order, and lower slot ids win flip
+// votes.  On co-spherical inputs a tet-major order left about four times
+// as many faces stuck (1M points on a sphere: 3,150 against 866).
 __global__ void __launch_bounds__(kBlock) k_split(Mesh M, const uint2* wlist, uint32_t W, uint32_t round, FlipLists L) {
   uint32_t k = gid();
   uint32_t c[4], m = 0u;
   if (k < W) m = split_body(M, wlist, k, round, c);
-  // wave prefix of the append counts (0, 3 or 4)
-  uint32_t n = (uint32_t)__popc(m);
   uint64_t lt = (1ull << __lane_id()) - 1ull;
-  uint64_t b0 = __ballot(n & 1u), b1 = __ballot(n & 2u), b2 = __ballot(n & 4u);
-  uint32_t off = __popcll(b0 & lt) + 2u * __popcll(b1 & lt) + 4u * __popcll(b2 & lt);
-  uint32_t tot = __popcll(b0) + 2u * __popcll(b1) + 4u * __popcll(b2);
+  uint64_t b0 = __ballot(m & 1u), bv = __ballot(m != 0u);
+  uint32_t n0 = __popcll(b0), nv = __popcll(bv);
+  uint32_t tot = n0 + 3u * nv;
   if (tot == 0u) return;
   uint32_t base = 0u;
   if (__lane_id() == 0) base = atomicAdd(&M.ctr[C_ACT_NEXT], tot);
   base = __shfl(base, 0);
   uint32_t* act_next = L.act[M.ctr[C_PAR] & 1u];
-  for (int j = 0; j < 4; ++j)
-    if (m & (1u << j)) act_next[base + off++] = c[j];
+  if (m & 1u) act_next[base + __popcll(b0 & lt)] = c[0];
+  if (m) {
+    uint32_t r = __popcll(bv & lt);
+    for (uint32_t j = 1; j < 4; ++j) act_next[base + n0 + (j - 1u) * nv + r] = c[j];
+  }
 }
 
 // ctr[dst] = ctr[src]

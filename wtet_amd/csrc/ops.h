@@ -305,25 +305,42 @@ template <bool FAST>
 WT_HD bool split_points_body_t(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
                                uint32_t round, PredStats* st) {
   uint32_t t = utet[i];
+  uint32_t q = uvert[i];
   uint2 sp = M.tsplit[t];
   if (sp.x != round) return false;
   uint32_t base = sp.y;
   uint4 c0 = TV(M, t);      // (p, v1, v2, v3)
   uint32_t v0 = TV(M, base).x;  // child 1 = (v0, p, v2, v3)
   uint32_t p = c0.x, v1 = c0.y, v2 = c0.z, v3 = c0.w;
-  uint32_t q = uvert[i];
+  // the six points in one round trip
+  float Pp[3], P0[3], P1[3], P2[3], P3[3], Q[3];
+  uint32_t rp, r0, r1, r2, r3, rq;
+  load_pt(M, p, Pp, &rp);
+  load_pt(M, v0, P0, &r0);
+  load_pt(M, v1, P1, &r1);
+  load_pt(M, v2, P2, &r2);
+  load_pt(M, v3, P3, &r3);
+  load_pt(M, q, Q, &rq);
   // children 0 | 1 across plane (p, v2, v3): +1 -> v0's side -> not child 0
-  int sa = o3v<FAST>(M, p, v3, v2, q, st);
+  int sa = o3s<FAST>(M, Pp, P3, P2, Q, rp, r3, r2, rq, st);
   if (sa == 2) return true;
   int a = sa > 0 ? 1 : 0;
   // children 2 | 3 across plane (v0, p, v1): +1 -> v2's side -> not child 2
-  int sb = o3v<FAST>(M, v0, p, v1, q, st);
+  int sb = o3s<FAST>(M, P0, Pp, P1, Q, r0, rp, r1, rq, st);
   if (sb == 2) return true;
   int b = sb > 0 ? 3 : 2;
-  // decide between a and b on their shared face (child a's face b)
-  uint32_t vv[4] = {v0, v1, v2, v3};
-  vv[a] = p;
-  int s = o3v<FAST>(M, vv[face_vi(b, 0)], vv[face_vi(b, 1)], vv[face_vi(b, 2)], q, st);
+  // decide between a and b on their shared face: face b of (v0, v1, v2, v3)
+  // with vertex a replaced by p -- (w0, v3, w1) for b = 2, (w0, w1, v2) for
+  // b = 3, where w0 = a ? v0 : p and w1 = a ? p : v1 (chosen by value)
+  float X[3], Y[3], Z[3];
+  for (int k = 0; k < 3; ++k) {
+    float w0 = a ? P0[k] : Pp[k], w1 = a ? Pp[k] : P1[k];
+    X[k] = w0;
+    Y[k] = b == 2 ? P3[k] : w1;
+    Z[k] = b == 2 ? w1 : P2[k];
+  }
+  uint32_t rw0 = a ? r0 : rp, rw1 = a ? rp : r1;
+  int s = o3s<FAST>(M, X, Y, Z, Q, rw0, b == 2 ? r3 : rw1, b == 2 ? rw1 : r2, rq, st);
   if (s == 2) return true;
   int child = s > 0 ? a : b;
   utet[i] = child_slot(t, base, (uint32_t)child);
@@ -613,30 +630,32 @@ WT_HD uint32_t walk_to_t(const Mesh& M, uint32_t t, uint32_t q, uint32_t seed, P
     uint4 tv = TV(M, t);
     uint4 to = TN(M, t);
     uint32_t r0 = hash32(seed + step * 0x9E3779B9u) & 3u;
-    bool moved = false;
-    for (uint32_t kk = 0; kk < 4; ++kk) {
-      int f = (int)((kk + r0) & 3u);
-      uint32_t x = get4(tv, face_vi(f, 0)), y = get4(tv, face_vi(f, 1)), z = get4(tv, face_vi(f, 2));
-      float X[3], Y[3], Z[3];
-      uint32_t rx, ry, rz;
-      load_pt(M, x, X, &rx);
-      load_pt(M, y, Y, &ry);
-      load_pt(M, z, Z, &rz);
-      int o = o3s<FAST>(M, X, Y, Z, Q, rx, ry, rz, rq, st);
-      if (o == 2) {
-        *state = 2;
-        return t;
-      }
-      if (o < 0) {
-        t = opp_tet(get4(to, f));
-        moved = true;
-        break;
-      }
+    // the tet's four points in one round trip, all four face tests, then
+    // the step takes the first face in its random order (r0, r0 + 1, ...)
+    // that q lies beyond, or defers if an undecided face comes first
+    float P[4][3];
+    uint32_t R[4];
+    for (int k = 0; k < 4; ++k) load_pt(M, get4(tv, k), P[k], &R[k]);
+    uint32_t beyond = 0u, undecided = 0u;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      int a = face_vi(f, 0), b = face_vi(f, 1), c = face_vi(f, 2);
+      int o = o3s<FAST>(M, P[a], P[b], P[c], Q, R[a], R[b], R[c], rq, st);
+      if (o == 2) undecided |= 1u << f;
+      else if (o < 0) beyond |= 1u << f;
     }
-    if (!moved) {
+    uint32_t m = beyond | undecided;
+    if (!m) {
       *state = 0;
       return t;
     }
+    uint32_t rot = ((m >> r0) | (m << (4u - r0))) & 0xFu;
+    uint32_t f = ((uint32_t)ctz32(rot) + r0) & 3u;
+    if (undecided & (1u << f)) {
+      *state = 2;
+      return t;
+    }
+    t = opp_tet(get4(to, (int)f));
   }
   *state = 1;
   return t;

@@ -290,12 +290,14 @@ bool face_bad(const PointSet& S, const Mesh32& W, uint32_t t, int f, int mode) {
 }  // namespace
 
 static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
-                               const std::vector<uint32_t>& viol, int mode, RepairReport* rep);
+                               const std::vector<uint32_t>& viol, int mode, RepairReport* rep,
+                               uint32_t region_cap);
 
 wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
-                            const std::vector<uint32_t>& viol, int mode, RepairReport* rep) {
+                            const std::vector<uint32_t>& viol, int mode, RepairReport* rep,
+                            uint32_t region_cap) {
   try {
-    return repair_impl(xyzn, n, mesh, viol, mode, rep);
+    return repair_impl(xyzn, n, mesh, viol, mode, rep, region_cap);
   } catch (const NotPresent&) {
     if (rep) {
       rep->needs_full = true;
@@ -306,7 +308,8 @@ wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
 }
 
 static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
-                               const std::vector<uint32_t>& viol, int mode, RepairReport* rep) {
+                               const std::vector<uint32_t>& viol, int mode, RepairReport* rep,
+                               uint32_t region_cap) {
   PointSet S;
   S.xyzn = xyzn;
   S.n = n;
@@ -344,6 +347,14 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
         for (int h = 0; h < 4; ++h)
           if (W.ta(x, h) != kInvalid) ring.push_back(W.ta(x, h) >> 5);
       for (uint32_t x : ring) add(x);
+      if (region_cap && R.size() > region_cap) {
+        if (rep) {
+          rep->too_large = true;
+          rep->message = "a repair region outgrew its cap";
+        }
+        fail = WTET_ERR_NOT_CONVERGED;
+        return;
+      }
       if (R.size() * 2 > W.count())
         for (uint32_t x = 0; x < W.count(); ++x) add(x);
       std::vector<uint32_t> real;

@@ -16,6 +16,7 @@ struct RepairReport {
   uint32_t regions = 0;      // regions re-triangulated
   uint32_t max_region = 0;   // largest region (tets)
   bool needs_full = false;   // the repair reached a row the sparse view lacks
+  bool too_large = false;    // a region outgrew the cap the caller set
   uint32_t fetches = 0;      // rows fetched into the sparse view on demand
   std::string message;
 };
@@ -44,6 +45,9 @@ struct MeshBuf {
 
 // xyzn: n normalized points (input id order).  viol: (t << 2) | f of
 // non-Delaunay faces.  On success the mesh is the exact Delaunay
-// triangulation.
+// triangulation.  region_cap > 0: stop with WTET_ERR_NOT_CONVERGED and
+// rep->too_large once a region holds more than region_cap tets (the mesh is
+// then partly repaired and only fit to be discarded); 0: no limit.
 wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
-                            const std::vector<uint32_t>& viol, int mode, RepairReport* rep);
+                            const std::vector<uint32_t>& viol, int mode, RepairReport* rep,
+                            uint32_t region_cap = 0);
