@@ -580,17 +580,15 @@ __device__ __forceinline__ uint32_t block_prefix3(uint32_t a, uint32_t b, uint32
   return off + r;
 }
 
-// Decide every proposal, then claim list space for the whole block with
-// one atomic per counter (flips, fresh slots, next active entries), then
-// execute the winners.  Counters share a cache line, so per-wave atomics
-// from every wave serialize; per-block claims keep them few.  About a third
-// of the proposals win: the winners are packed through LDS so that the
-// flips run on dense waves.
+// Decide every proposal; a winner loads its flip's tets at once, so those
+// loads overlap its queue claims and the block's counter claims.  Then the
+// block claims list space with one atomic per counter (flips, fresh slots,
+// next active entries) -- the counters share a cache line, so per-wave
+// atomics from every wave would serialize -- and every winner executes its
+// own flip from registers.
 __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L) {
   __shared__ uint32_t wsum[kBlock / 64];
   __shared__ uint32_t base[3];
-  __shared__ uint4 wpr[kBlock];      // winning proposals of the tile, packed
-  __shared__ uint32_t wsl[kBlock];   // their fresh slots
   if (!M.ctr[C_RUN]) return;
   uint32_t A = M.ctr[C_A], it = M.ctr[C_IT], round = M.ctr[C_ROUND];
   uint32_t* act_next = L.act[(M.ctr[C_PAR] & 1u) ^ 1u];
@@ -599,7 +597,8 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
     uint4 pr = make_uint4(kInvalid, 0u, 0u, 0u);
     if (j < A) pr = L.prop[j];
     uint32_t qid[2], nq = 0, nslot = 0;
-    bool win = flip_decide_body(M, pr, it, qid, &nq, &nslot);
+    FlipTets F;
+    bool win = flip_decide_load(M, pr, it, qid, &nq, &nslot, F);
     uint32_t nqa = nq + nslot;  // the fresh slot is queued too
     uint32_t tot;
     uint32_t r = block_prefix3(win ? 1u : 0u, nslot, nqa, &tot, wsum);
@@ -616,15 +615,9 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
     if (win) {
       uint32_t sl = nslot ? base[1] + rs : kInvalid;
       if (nslot) act_next[q + nq] = sl;
-      wpr[rf] = pr;
-      wsl[rf] = sl;
+      flip_exec_loaded(M, pr, F, it, round, sl, &L.out[2 * (size_t)(base[0] + rf)]);
     }
-    __syncthreads();
-    uint32_t f0 = base[0];
-    for (uint32_t i = threadIdx.x; i < tf; i += kBlock)
-      flip_exec_body(M, wpr[i], it, round, wsl[i], &L.out[2 * (size_t)(f0 + i)]);
-    // the next tile's prefix sums synchronize before wpr, wsl and base are
-    // rewritten
+    // the next tile's prefix sums synchronize before base is rewritten
   }
 }
 

@@ -443,12 +443,28 @@ WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint
 // outk[0]: x,y,z = slots of the new tets (z unused for 3-2), w = kind;
 // outk[1]: the flip's vertices o, a and (3-2) z, for the adjacency update
 // ---------------------------------------------------------------------------
-WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t round,
-                          uint32_t s, uint4* outk) {
+// The records a flip reads: vertices, neighbours and opposite vertices of
+// its two (2-3) or three (3-2) tets.  Loading them is split from the flip so
+// that a caller can issue the loads early and overlap them with other work.
+struct FlipTets {
+  uint4 tv, to, tov, nv, no, nov, mv, mo, mov;
+};
+
+WT_HD void flip_load(const Mesh& M, const uint4& pr, FlipTets& F) {
+  // a 2-3 flip has no third tet: its slot is replaced by t's, so every load
+  // is of a valid slot and none depends on a branch
+  uint32_t m = ((pr.y >> 2) & 1u) ? pr.w : pr.x;
+  F.tv = TV(M, pr.x), F.to = TN(M, pr.x), F.tov = TOV(M, pr.x);
+  F.nv = TV(M, pr.z), F.no = TN(M, pr.z), F.nov = TOV(M, pr.z);
+  F.mv = TV(M, m), F.mo = TN(M, m), F.mov = TOV(M, m);
+}
+
+WT_HD void flip_exec_loaded(const Mesh& M, const uint4 pr, const FlipTets& F, uint32_t it,
+                            uint32_t round, uint32_t s, uint4* outk) {
   uint32_t t = pr.x, f = pr.y & 3u, kind = (pr.y >> 2) & 1u, edge = (pr.y >> 3) & 3u;
   uint32_t n = pr.z;
-  uint4 tv = TV(M, t), to = TN(M, t), tov = TOV(M, t);
-  uint4 nv = TV(M, n), no = TN(M, n), nov = TOV(M, n);
+  const uint4 tv = F.tv, to = F.to, tov = F.tov;
+  const uint4 nv = F.nv, no = F.no, nov = F.nov;
   uint32_t nf = opp_face(get4(to, (int)f));
   uint32_t a = get4(tv, (int)f), o = get4(nv, (int)nf);
   int pos[3] = {face_vi((int)f, 0), face_vi((int)f, 1), face_vi((int)f, 2)};
@@ -495,7 +511,7 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
              z = get4(tv, pos[(edge + 2) % 3]);
     int px = pos[edge], py = pos[(edge + 1) % 3];
     uint32_t m = pr.w;
-    uint4 mv = TV(M, m), mo = TN(M, m), mov = TOV(M, m);
+    const uint4 mv = F.mv, mo = F.mo, mov = F.mov;
     int qx = find4(nv, x), qy = find4(nv, y);
     int mx = find4(mv, x), my = find4(mv, y);
     uint32_t A = t, B = n;
@@ -530,6 +546,13 @@ WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t r
   }
 }
 
+WT_HD void flip_exec_body(const Mesh& M, const uint4 pr, uint32_t it, uint32_t round,
+                          uint32_t s, uint4* outk) {
+  FlipTets F;
+  flip_load(M, pr, F);
+  flip_exec_loaded(M, pr, F, it, round, s, outk);
+}
+
 // A proposal wins when it holds the vote of every tet it involves; the
 // winner executes at once (its tets belong to no other winner), losers go
 // back to the active list (src/shaders/mark_rejected_flips.wgsl:61-178 and
@@ -553,6 +576,31 @@ WT_HD bool flip_decide_body(const Mesh& M, const uint4& pr, uint32_t it, uint32_
       uint32_t old = at_or_u32(&INFO(M, reuse[k]), kAlive | kQueued);
       if (!(old & kQueued)) qid[(*nq)++] = reuse[k];
     }
+    if (((pr.y >> 2) & 1u) == 0u) *nslot = 1;
+  } else {
+    uint32_t old = at_or_u32(&INFO(M, pr.x), kQueued);
+    if (!(old & kQueued)) qid[(*nq)++] = pr.x;
+  }
+  return win;
+}
+
+// flip_decide_body for the list kernel: a winner also loads its flip's
+// tets (into F), issued after its queue claims so that the loads and the
+// claims' round trips overlap.  Same decisions and queue entries.
+WT_HD bool flip_decide_load(const Mesh& M, const uint4& pr, uint32_t it, uint32_t qid[2],
+                            uint32_t* nq, uint32_t* nslot, FlipTets& F) {
+  *nq = 0;
+  *nslot = 0;
+  if (pr.x == kInvalid) return false;
+  uint64_t key = flip_key(it, pr.x);
+  bool win = M.fvote[pr.x] == key && M.fvote[pr.z] == key;
+  if (((pr.y >> 2) & 1u) && M.fvote[pr.w] != key) win = false;
+  if (win) {
+    uint32_t o0 = at_or_u32(&INFO(M, pr.x), kAlive | kQueued);
+    uint32_t o1 = at_or_u32(&INFO(M, pr.z), kAlive | kQueued);
+    flip_load(M, pr, F);
+    if (!(o0 & kQueued)) qid[(*nq)++] = pr.x;
+    if (!(o1 & kQueued)) qid[(*nq)++] = pr.z;
     if (((pr.y >> 2) & 1u) == 0u) *nslot = 1;
   } else {
     uint32_t old = at_or_u32(&INFO(M, pr.x), kQueued);
