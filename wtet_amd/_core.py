@@ -49,6 +49,26 @@ class _Config(ctypes.Structure):
     ]
 
 
+class _ResultOwner:
+    """Holds a wtet_result until no array over its memory is left."""
+
+    def __init__(self, res):
+        self.res = res
+
+    def __del__(self):
+        try:
+            lib().wtet_result_free(ctypes.byref(self.res))
+        except Exception:  # interpreter shutdown: the process frees it
+            pass
+
+
+def _owned_rows(ptr, m: int, owner: _ResultOwner) -> np.ndarray:
+    """(m, 4) uint32 view of library memory that keeps `owner` alive."""
+    buf = (ctypes.c_uint32 * (4 * m)).from_address(ctypes.addressof(ptr.contents))
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=np.uint32).reshape(m, 4)
+
+
 class _Result(ctypes.Structure):
     _fields_ = [
         ("num_tets", ctypes.c_uint32),
@@ -417,18 +437,24 @@ class Context:
         return self._take(res)
 
     def _take(self, res) -> "DelaunayResult":
-        """Copies a wtet_result into numpy arrays and frees it."""
+        """Numpy arrays over a wtet_result: tets and adjacency use the
+        library's arrays in place (no copy of the rows); the result is freed
+        when the last array over them is gone."""
+        m = res.num_tets
+        if not m:
+            lib().wtet_result_free(ctypes.byref(res))
+            z = np.zeros((0, 4), np.uint32)
+            return DelaunayResult(z, z.copy(), np.zeros((0,), np.uint32), self.stats())
         try:
-            m = res.num_tets
-            tets = np.ctypeslib.as_array(res.tets, shape=(m * 4,)).copy().reshape(m, 4) if m else \
-                np.zeros((0, 4), np.uint32)
-            adj = np.ctypeslib.as_array(res.adjacency, shape=(m * 4,)).copy().reshape(m, 4) if m else \
-                np.zeros((0, 4), np.uint32)
             k = res.num_failed
             failed = np.ctypeslib.as_array(res.failed_verts, shape=(k,)).copy() if k else \
                 np.zeros((0,), np.uint32)
-        finally:
+        except BaseException:
             lib().wtet_result_free(ctypes.byref(res))
+            raise
+        owner = _ResultOwner(res)
+        tets = _owned_rows(res.tets, m, owner)
+        adj = _owned_rows(res.adjacency, m, owner)
         return DelaunayResult(tets, adj, failed, self.stats())
 
     # -- stage-level calls (one step of the device loop at a time) ---------------

@@ -172,6 +172,26 @@ void prefault(void* p, size_t bytes) {
   for (auto& t : th) t.join();
 }
 
+// memcpy on up to 8 threads (64 MiB or more each): a fresh destination's
+// first-touch page faults are then taken in parallel
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  const size_t kChunk = 64u << 20;
+  unsigned nt = std::min<unsigned>(8u, (unsigned)((bytes + kChunk - 1) / kChunk));
+  if (nt <= 1) {
+    if (bytes) std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  size_t per = (bytes / nt + 4095) & ~(size_t)4095;
+  for (unsigned i = 0; i < nt; ++i) {
+    size_t a = std::min(bytes, per * i), b = std::min(bytes, per * (i + 1));
+    th.emplace_back([dst, src, a, b] {
+      std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+    });
+  }
+  for (auto& t : th) t.join();
+}
+
 template <typename T>
 void ensure_pinned(T*& p, size_t& cap, size_t need) {
   if (p && cap >= need) return;
@@ -1056,15 +1076,16 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return;
   }
-  std::vector<uint32_t> tets(mb.tets, mb.tets + 4 * mb.count), adj(mb.adj, mb.adj + 4 * mb.count);
-  out->num_tets = (uint32_t)(tets.size() / 4);
-  out->tets = (uint32_t*)std::malloc(std::max<size_t>(tets.size(), 1) * 4);
-  out->adjacency = (uint32_t*)std::malloc(std::max<size_t>(adj.size(), 1) * 4);
+  // straight from the staged rows into the caller's arrays
+  const size_t words = 4 * mb.count;
+  out->num_tets = (uint32_t)mb.count;
+  out->tets = (uint32_t*)std::malloc(std::max<size_t>(words, 1) * 4);
+  out->adjacency = (uint32_t*)std::malloc(std::max<size_t>(words, 1) * 4);
   out->num_failed = (uint32_t)failed.size();
   out->failed_verts = (uint32_t*)std::malloc(std::max<size_t>(failed.size(), 1) * 4);
   if (!out->tets || !out->adjacency || !out->failed_verts) throw std::bad_alloc();
-  std::memcpy(out->tets, tets.data(), tets.size() * 4);
-  std::memcpy(out->adjacency, adj.data(), adj.size() * 4);
+  parallel_copy(out->tets, mb.tets, words * 4);
+  parallel_copy(out->adjacency, mb.adj, words * 4);
   if (!failed.empty()) std::memcpy(out->failed_verts, failed.data(), failed.size() * 4);
   c->stats.num_tets_out = out->num_tets;
   c->stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
