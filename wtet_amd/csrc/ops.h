@@ -182,24 +182,26 @@ WT_HD void set_face(const Mesh& M, uint32_t t, uint32_t f, uint32_t n, uint32_t 
 // ---------------------------------------------------------------------------
 // returns kVoteNone when the point must not vote
 WT_HD uint64_t vote_key(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t i) {
-  uint32_t t = utet[i];
+  // The key is computed before the infinity test (every id, infinity
+  // included, is a loadable point), so that the point's own id and
+  // coordinates load beside its tet's vertices: two round trips.
+  uint32_t t = utet[i], q = uvert[i];
   uint4 tv = TV(M, t);
-  if (tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf) {
-    at_add_u32(&M.ctr[C_BADTET], 1u);
-    return kVoteNone;
-  }
   float a[3], b[3], c[3], d[3], p[3];
   uint32_t r;
+  load_pt(M, q, p, &r);
   load_pt(M, tv.x, a, &r);
   load_pt(M, tv.y, b, &r);
   load_pt(M, tv.z, c, &r);
   load_pt(M, tv.w, d, &r);
-  load_pt(M, uvert[i], p, &r);
   double det = insphere_eval<double>(a, b, c, d, p).det;
   double val = -det;
   if (!(val > 0.0)) val = 0.0;
   float fv = (float)val;
-  return ((uint64_t)(0xFFFFFFFFu - f32_bits(fv)) << 32) | i;
+  uint64_t key = ((uint64_t)(0xFFFFFFFFu - f32_bits(fv)) << 32) | i;
+  bool bad = tv.x == M.inf || tv.y == M.inf || tv.z == M.inf || tv.w == M.inf;
+  if (bad) at_add_u32(&M.ctr[C_BADTET], 1u);
+  return bad ? kVoteNone : key;
 }
 
 WT_HD void vote_body(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t i) {
