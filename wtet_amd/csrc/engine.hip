@@ -95,6 +95,10 @@ struct wtet_ctx {
   // the largest cloud (warmup_rounds)
   uint32_t ring0_rounds = 0xFFFFFFFFu;
   int repair_rings = 6;       // face-steps of rows staged around each stuck face
+  // rows of the host mirror that hold staged rows (sparse repair); kept
+  // across calls and cleared entry by entry (present_rows lists the set ones)
+  std::vector<uint8_t> present;
+  std::vector<uint32_t> present_rows;
   // a single-cloud repair region above this many tets stops the repair and
   // reruns the device phase with a wider insertion ring (0: never)
   uint32_t region_cap = 1u << 14;
@@ -872,7 +876,7 @@ constexpr int kRepairRingsMax = 16;
 constexpr uint32_t kRepairTail = 4096;
 
 // packed rows list[from, to) of the neighbourhood into the host mirror
-void scatter_rows(wtet_ctx* c, uint32_t from, uint32_t to, std::vector<uint8_t>& present) {
+void scatter_rows(wtet_ctx* c, uint32_t from, uint32_t to) {
   hipStream_t s = c->stream;
   uint32_t n = to - from;
   if (n == 0) return;
@@ -883,13 +887,32 @@ void scatter_rows(wtet_ctx* c, uint32_t from, uint32_t to, std::vector<uint8_t>&
   ensure_pinned(c->h_patch, c->h_patch_cap, (size_t)n * 9 + 9);
   HIP_CHECK(hipMemcpyAsync(c->h_patch, c->nbr_pack.p, (size_t)n * 36, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  for (uint32_t i = 0; i < n; ++i) {
-    const uint32_t* q = c->h_patch + 9 * (size_t)i;
-    uint32_t r = q[0];
-    std::memcpy(c->h_tets + 4 * (size_t)r, q + 1, 16);
-    std::memcpy(c->h_adj + 4 * (size_t)r, q + 5, 16);
-    present[r] = 1;
+  // rows land at random places of the mirror: up to 8 threads keep more of
+  // those cache-missing stores in flight (distinct rows, distinct bytes)
+  const uint32_t* hp = c->h_patch;
+  uint32_t* ht = c->h_tets;
+  uint32_t* ha = c->h_adj;
+  uint8_t* pres = c->present.data();
+  auto part = [hp, ht, ha, pres](size_t a, size_t b) {
+    for (size_t i = a; i < b; ++i) {
+      const uint32_t* q = hp + 9 * i;
+      uint32_t r = q[0];
+      std::memcpy(ht + 4 * (size_t)r, q + 1, 16);
+      std::memcpy(ha + 4 * (size_t)r, q + 5, 16);
+      pres[r] = 1;
+    }
+  };
+  unsigned nt = n >= (1u << 15) ? 8u : 1u;
+  if (nt == 1) {
+    part(0, n);
+  } else {
+    std::vector<std::thread> th;
+    size_t per = (n + nt - 1) / nt;
+    for (unsigned k = 0; k < nt; ++k)
+      th.emplace_back(part, std::min<size_t>(n, per * k), std::min<size_t>(n, per * (k + 1)));
+    for (auto& t : th) t.join();
   }
+  for (uint32_t i = 0; i < n; ++i) c->present_rows.push_back(hp[9 * (size_t)i]);
   c->stats.repair_rows_staged += n;
 }
 
@@ -902,7 +925,7 @@ uint32_t listed_rows(wtet_ctx* c, uint32_t M) {
   return n;
 }
 
-void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M, std::vector<uint8_t>& present) {
+void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
   hipStream_t s = c->stream;
   size_t cap = (size_t)M + M / 32 + 65536;
   size_t c1 = c->h_cap, c2 = c->h_cap;
@@ -918,9 +941,12 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M, std::vector<uin
                        P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), cnt, cnt + 1, false, s);
   HIP_CHECK(hipGetLastError());
   uint32_t n = listed_rows(c, M);
-  present.assign(M, 0);
+  // clear what the previous call marked instead of zeroing M bytes afresh
+  for (uint32_t r : c->present_rows) c->present[r] = 0;
+  c->present_rows.clear();
+  if (c->present.size() < M) c->present.resize(M, 0);
   c->stats.repair_rows_staged = 0;
-  scatter_rows(c, 0, n, present);
+  scatter_rows(c, 0, n);
 }
 
 // the repair reached `row`, which the host lacks: the rows within the
@@ -929,9 +955,8 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M, std::vector<uin
 // the host, it returns false and the repair takes every row instead.
 constexpr uint32_t kMaxFetches = 4;
 
-bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, std::vector<uint8_t>& present,
-                uint32_t& fetches) {
-  if (row >= M || present[row] || ++fetches > kMaxFetches) return false;
+bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
+  if (row >= M || c->present[row] || ++fetches > kMaxFetches) return false;
   uint32_t before = listed_rows(c, M);
   if (before > M / 4) return false;
   uint32_t* misc = P<uint32_t>(c->nbr_misc);
@@ -942,8 +967,8 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, std::vector<uint8_t>& pre
                        P<uint32_t>(c->nbr_list), misc, misc + 1, true, c->stream);
   HIP_CHECK(hipGetLastError());
   uint32_t after = listed_rows(c, M);
-  scatter_rows(c, before, after, present);
-  return present[row] != 0;
+  scatter_rows(c, before, after);
+  return c->present[row] != 0;
 }
 
 void download_rows(wtet_ctx* c, uint32_t M) {
@@ -991,19 +1016,21 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
   const bool resident = dout != nullptr;
   c->stats.violations_gpu = nviol;
   auto t1 = std::chrono::steady_clock::now();
-  std::vector<uint8_t> present;
+  bool sparse = false;
   if (resident && nviol > 0) {
-    if (rec.size() < nviol) download_rows(c, mout);  // records overflowed: repair needs all
-    else stage_neighbourhood(c, (uint32_t)rec.size(), mout, present);
+    if (rec.size() < nviol) {
+      download_rows(c, mout);  // records overflowed: repair needs all
+    } else {
+      stage_neighbourhood(c, (uint32_t)rec.size(), mout);
+      sparse = true;
+    }
   }
   MeshBuf mb{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
   uint32_t fetches = 0;
-  if (!present.empty()) {
-    mb.present = present.data();
+  if (sparse) {
+    mb.present = c->present.data();
     mb.known = mout;
-    mb.fetch = [c, mout, &present, &fetches](uint32_t row) {
-      return fetch_rows(c, row, mout, present, fetches);
-    };
+    mb.fetch = [c, mout, &fetches](uint32_t row) { return fetch_rows(c, row, mout, fetches); };
   }
   bool on_host = !resident || nviol > 0;
   std::vector<uint32_t> failed;
