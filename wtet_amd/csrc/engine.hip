@@ -880,11 +880,14 @@ void scatter_rows(wtet_ctx* c, uint32_t from, uint32_t to) {
   hipStream_t s = c->stream;
   uint32_t n = to - from;
   if (n == 0) return;
-  ensure(c, c->nbr_pack, (size_t)n * 36 + 16);
+  // room for a sixteenth of the mesh at once, so that the staged row count
+  // varying from call to call does not reallocate these buffers
+  size_t room = std::max<size_t>(n, c->present.size() / 16);
+  ensure(c, c->nbr_pack, room * 36 + 16);
   launch_nbr_gather(P<uint32_t>(c->nbr_list) + from, n, P<uint32_t>(c->prop), P<uint32_t>(c->fout),
                     P<uint32_t>(c->nbr_pack), s);
   HIP_CHECK(hipGetLastError());
-  ensure_pinned(c->h_patch, c->h_patch_cap, (size_t)n * 9 + 9);
+  ensure_pinned(c->h_patch, c->h_patch_cap, room * 9 + 9);
   HIP_CHECK(hipMemcpyAsync(c->h_patch, c->nbr_pack.p, (size_t)n * 36, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   // rows land at random places of the mirror: up to 8 threads keep more of
@@ -952,8 +955,10 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
 // the repair reached `row`, which the host lacks: the rows within the
 // staging rings of it that are not on the host yet.  Each fetch restarts a
 // region, so after kMaxFetches of them, or once a quarter of the mesh is on
-// the host, it returns false and the repair takes every row instead.
-constexpr uint32_t kMaxFetches = 4;
+// the host, it returns false and the repair takes every row instead.  A
+// fetch costs a fraction of a millisecond at 1M points, the full download
+// and repair 10-15 ms.
+constexpr uint32_t kMaxFetches = 8;
 
 bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   if (row >= M || c->present[row] || ++fetches > kMaxFetches) return false;
