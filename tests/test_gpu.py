@@ -360,7 +360,7 @@ def test_resident_repair_fetches_missing_rows(oracle, ctx, rings):
         tets = ctx.download(dt, (m, 4), np.uint32)
         adj = ctx.download(da, (m, 4), np.uint32)
     finally:
-        ctx.set_repair_rings(6)
+        ctx.set_repair_rings(7)
         ctx.free(d)
     assert st["violations_gpu"] > 0, st
     if rings == 1:  # one ring is never enough for this input's regions

@@ -94,7 +94,7 @@ struct wtet_ctx {
   // the first rounds insert one point per tet (ring 0); kAutoWarmup: from
   // the largest cloud (warmup_rounds)
   uint32_t ring0_rounds = 0xFFFFFFFFu;
-  int repair_rings = 6;       // face-steps of rows staged around each stuck face
+  int repair_rings = 7;       // face-steps of rows staged around each stuck face
   // rows of the host mirror that hold staged rows (sparse repair); kept
   // across calls and cleared entry by entry (present_rows lists the set ones)
   std::vector<uint8_t> present;
