@@ -145,6 +145,34 @@ def test_exact_and_filtered_agree_on_lattice(oracle, ctx):
     assert np.array_equal(oracle.canonical_real(a.tets, len(p)), oracle.canonical_real(b.tets, len(p)))
 
 
+def test_result_arrays_outlive_the_result(oracle, ctx):
+    """The tets and adjacency arrays of a host result use the library's
+    memory without a copy: they stay valid, unchanged and writable after the
+    result object is gone and other calls have allocated and freed results,
+    and a view keeps the memory alive on its own."""
+    import gc
+    p = W.uniform(3000, 7)
+    r = ctx.delaunay(p)
+    tets, adj = r.tets, r.adjacency
+    want_t, want_a = tets.copy(), adj.copy()
+    view = r.tets[10:20]
+    want_v = view.copy()
+    del r
+    gc.collect()
+    for s in range(4):
+        ctx.delaunay(W.uniform(3500 + 500 * s, 100 + s))
+        gc.collect()
+    assert np.array_equal(tets, want_t) and np.array_equal(adj, want_a)
+    del tets
+    gc.collect()
+    ctx.delaunay(W.uniform(4000, 9))
+    assert np.array_equal(view, want_v)
+    assert adj.flags.writeable and adj.dtype == np.uint32 and adj.flags.c_contiguous
+    n = len(p)
+    pn, _, _ = oracle.normalize_points(p)
+    assert np.array_equal(oracle.canonical_real(want_t, n), oracle.canonical_real(oracle.delaunay(pn), n))
+
+
 def test_uniform_200k_matches_oracle(oracle, ctx):
     _check(oracle, ctx, W.uniform(200000, 11))
 
