@@ -94,6 +94,15 @@ typedef struct wtet_stats {
                                      cap (the stats above are those of the
                                      last run)                                */
   uint32_t repair_max_region;     /* tets in the largest repair region        */
+  uint32_t points_left;           /* points not inserted when max_insert_iterations
+                                     rounds had run (src/phase1.rs:26); the host
+                                     phase inserts them when enable_splaying is set
+                                     and failed_verts lists them                */
+  uint64_t flips_23;              /* executed 2-3 flips (of sum_flips)         */
+  uint64_t flips_32;              /* executed 3-2 flips (of sum_flips)         */
+  uint64_t small_checked;         /* of sum_active: checked in the one-workgroup
+                                     iterations of short lists              */
+  uint64_t small_flips;           /* of sum_flips: executed in those iterations */
 } wtet_stats;
 
 typedef struct wtet_ctx wtet_ctx;

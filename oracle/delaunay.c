@@ -462,3 +462,174 @@ int64_t o_count_violations(const float *pts, int64_t n,
   free(P);
   return viol;
 }
+
+/* ------------------------------------------------------------------------ */
+/* full_bw_reconstruction as written (src/phase2.rs:383-606)                 */
+/* ------------------------------------------------------------------------ */
+
+static int cmp_i64(const void *x, const void *y) {
+  int64_t a = *(const int64_t *)x, b = *(const int64_t *)y;
+  return a < b ? -1 : (a > b);
+}
+
+static void sort3(uint32_t *f) {
+  for (int a = 1; a < 3; ++a)
+    for (int b = a; b > 0 && f[b] < f[b - 1]; --b) {
+      uint32_t s = f[b]; f[b] = f[b - 1]; f[b - 1] = s;
+    }
+}
+
+int64_t o_delaunay_bw_reference(const float *pts, int64_t n, uint32_t **tets_out) {
+  const uint32_t INV = 0xFFFFFFFFu;
+  /* face f = the vertices other than f, in ascending position
+   * (src/phase2.rs:395-397) */
+  static const int fi[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  double *P = (double *)malloc(sizeof(double) * 3 * (size_t)(n + 4));
+  for (int64_t i = 0; i < 3 * n; ++i) P[i] = (double)pts[i];
+  const double sup[4][3] = {{-100, -100, -100}, {400, -100, -100},
+                            {-100, 400, -100}, {-100, -100, 400}};
+  memcpy(P + 3 * n, sup, sizeof(sup));
+  Pts PP = {P, n};
+  int64_t cap = 1024, len = 1;
+  uint32_t *T = (uint32_t *)malloc(16 * (size_t)cap);  /* tets, 4 each */
+  uint32_t *A = (uint32_t *)malloc(16 * (size_t)cap);  /* (t << 5) | f */
+  int64_t *stamp = (int64_t *)malloc(8 * (size_t)cap);
+  uint32_t s0 = (uint32_t)n, s1 = s0 + 1, s2 = s0 + 2, s3 = s0 + 3;
+  int o = o_orient3d_sos(pt(&PP, s0), pt(&PP, s1), pt(&PP, s2), pt(&PP, s3), s0, s1, s2, s3);
+  uint32_t init[4] = {s0, s1, s2, s3};
+  if (!(o > 0)) { init[0] = s1; init[1] = s0; }
+  memcpy(T, init, 16);
+  for (int f = 0; f < 4; ++f) A[f] = INV;
+  stamp[0] = -1;
+  int64_t *cav = NULL, *queue = NULL;
+  int64_t cav_cap = 0;
+  typedef struct { uint32_t f[3]; int64_t nt; uint32_t nf; } BFace;
+  BFace *bd = NULL;
+  uint32_t *NT = NULL, *NA = NULL;
+  int ok = 1;
+  for (int64_t pi = 0; pi < n && ok; ++pi) {
+    uint32_t v = (uint32_t)pi;
+    /* step 1: the first live tet (in creation order) whose circumsphere holds v */
+    int64_t seed = -1;
+    for (int64_t ti = 0; ti < len; ++ti) {
+      const uint32_t *t = T + 4 * ti;
+      if (t[0] == INV || t[1] == INV || t[2] == INV || t[3] == INV) continue;
+      if (o_orient4d(pt(&PP, t[0]), pt(&PP, t[1]), pt(&PP, t[2]), pt(&PP, t[3]), pt(&PP, v),
+                     t[0], t[1], t[2], t[3], v) < 0) { seed = ti; break; }
+    }
+    if (seed < 0) continue; /* outside every circumsphere: skipped */
+    /* step 2: BFS cavity */
+    if (cav_cap < len + 1) {
+      cav_cap = 2 * (len + 1);
+      cav = (int64_t *)realloc(cav, 8 * (size_t)cav_cap);
+      queue = (int64_t *)realloc(queue, 8 * (size_t)cav_cap);
+    }
+    int64_t nc = 0, nq = 0;
+    cav[nc++] = seed;
+    queue[nq++] = seed;
+    stamp[seed] = pi;
+    while (nq > 0) {
+      int64_t ti = queue[--nq];
+      for (int f = 0; f < 4; ++f) {
+        uint32_t pk = A[4 * ti + f];
+        if (pk == INV) continue;
+        int64_t nb = pk >> 5;
+        if (nb >= len || stamp[nb] == pi) continue;
+        const uint32_t *t = T + 4 * nb;
+        if (t[0] == INV || t[1] == INV || t[2] == INV || t[3] == INV) continue;
+        if (o_orient4d(pt(&PP, t[0]), pt(&PP, t[1]), pt(&PP, t[2]), pt(&PP, t[3]), pt(&PP, v),
+                       t[0], t[1], t[2], t[3], v) < 0) {
+          stamp[nb] = pi;
+          cav[nc++] = nb;
+          queue[nq++] = nb;
+        }
+      }
+    }
+    /* step 3: boundary faces, cavity in ascending order (a BTreeSet) */
+    qsort(cav, (size_t)nc, 8, cmp_i64);
+    bd = (BFace *)realloc(bd, sizeof(BFace) * 4 * (size_t)nc);
+    int64_t nb_faces = 0;
+    for (int64_t i = 0; i < nc; ++i) {
+      int64_t ti = cav[i];
+      const uint32_t *t = T + 4 * ti;
+      for (int f = 0; f < 4; ++f) {
+        uint32_t pk = A[4 * ti + f];
+        BFace b;
+        for (int k = 0; k < 3; ++k) b.f[k] = t[fi[f][k]];
+        if (pk == INV) {
+          b.nt = -1;
+          b.nf = 0;
+        } else {
+          if (stamp[pk >> 5] == pi) continue;
+          b.nt = pk >> 5;
+          b.nf = pk & 31u;
+        }
+        bd[nb_faces++] = b;
+      }
+    }
+    /* step 4: new tets joining each boundary face to v */
+    NT = (uint32_t *)realloc(NT, 16 * (size_t)nb_faces);
+    NA = (uint32_t *)realloc(NA, 16 * (size_t)nb_faces);
+    for (int64_t i = 0; i < nb_faces; ++i) {
+      uint32_t a = bd[i].f[0], b = bd[i].f[1], c = bd[i].f[2];
+      int s = o_orient3d_sos(pt(&PP, a), pt(&PP, b), pt(&PP, c), pt(&PP, v), a, b, c, v);
+      uint32_t *q = NT + 4 * i;
+      if (s > 0) { q[0] = a; q[1] = b; } else { q[0] = b; q[1] = a; }
+      q[2] = c;
+      q[3] = v;
+      for (int k = 0; k < 4; ++k) NA[4 * i + k] = INV;
+    }
+    /* step 5: external links, then new tets linked pairwise */
+    int64_t base = len;
+    for (int64_t i = 0; i < nb_faces; ++i) {
+      uint32_t *q = NT + 4 * i;
+      int vp = 0;
+      while (q[vp] != v) ++vp;
+      if (bd[i].nt >= 0) {
+        NA[4 * i + vp] = ((uint32_t)bd[i].nt << 5) | bd[i].nf;
+        A[4 * bd[i].nt + bd[i].nf] = ((uint32_t)(base + i) << 5) | (uint32_t)vp;
+      }
+      for (int f = 0; f < 4; ++f) {
+        if (f == vp || NA[4 * i + f] != INV) continue;
+        uint32_t mf[3] = {q[fi[f][0]], q[fi[f][1]], q[fi[f][2]]};
+        sort3(mf);
+        for (int64_t j = i + 1; j < nb_faces; ++j) {
+          const uint32_t *r = NT + 4 * j;
+          for (int f2 = 0; f2 < 4; ++f2) {
+            uint32_t of[3] = {r[fi[f2][0]], r[fi[f2][1]], r[fi[f2][2]]};
+            sort3(of);
+            if (mf[0] == of[0] && mf[1] == of[1] && mf[2] == of[2]) {
+              NA[4 * i + f] = ((uint32_t)(base + j) << 5) | (uint32_t)f2;
+              NA[4 * j + f2] = ((uint32_t)(base + i) << 5) | (uint32_t)f;
+            }
+          }
+        }
+      }
+    }
+    /* step 6: cavity tets die, new tets are appended */
+    for (int64_t i = 0; i < nc; ++i)
+      for (int k = 0; k < 4; ++k) {
+        T[4 * cav[i] + k] = INV;
+        A[4 * cav[i] + k] = INV;
+      }
+    if (len + nb_faces > cap) {
+      while (len + nb_faces > cap) cap *= 2;
+      T = (uint32_t *)realloc(T, 16 * (size_t)cap);
+      A = (uint32_t *)realloc(A, 16 * (size_t)cap);
+      stamp = (int64_t *)realloc(stamp, 8 * (size_t)cap);
+    }
+    memcpy(T + 4 * len, NT, 16 * (size_t)nb_faces);
+    memcpy(A + 4 * len, NA, 16 * (size_t)nb_faces);
+    for (int64_t i = 0; i < nb_faces; ++i) stamp[len + i] = -1;
+    len += nb_faces;
+  }
+  int64_t cnt = 0;
+  for (int64_t t = 0; t < len; ++t) cnt += T[4 * t] != INV;
+  uint32_t *out = (uint32_t *)malloc(16 * (size_t)(cnt ? cnt : 1));
+  int64_t k = 0;
+  for (int64_t t = 0; t < len; ++t)
+    if (T[4 * t] != INV) memcpy(out + 4 * (k++), T + 4 * t, 16);
+  *tets_out = out;
+  free(P); free(T); free(A); free(stamp); free(cav); free(queue); free(bd); free(NT); free(NA);
+  return ok ? cnt : -1;
+}

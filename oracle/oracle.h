@@ -100,6 +100,15 @@ int64_t o_delaunay_bw(const float *pts, int64_t n, uint32_t **tets_out,
                       int64_t *stats);
 void o_free(void *p);
 
+/* full_bw_reconstruction as written (src/phase2.rs:383-606), the
+ * reference's CPU path: points inserted in input order, each seeded by a
+ * linear scan over every tet created so far (dead ones skipped) for the
+ * first whose circumsphere holds it, BFS cavity visited in ascending tet
+ * order, new tets appended (never reused), new-tet faces linked by pairwise
+ * search.  Same output contract as o_delaunay_bw.  O(N^2): the CPU
+ * baseline on small inputs only. */
+int64_t o_delaunay_bw_reference(const float *pts, int64_t n, uint32_t **tets_out);
+
 /* Keep tets whose 4 ids are all < n, sort each quad, sort the list.
  * Writes into out (capacity m*4), returns count. */
 int64_t o_canonical_real(const uint32_t *tets, int64_t m, uint32_t n,

@@ -49,6 +49,8 @@ def lib():
         L.o_stratified_order.argtypes = [i64, i64, P]
         L.o_delaunay_bw.argtypes = [P, i64, ctypes.POINTER(ctypes.POINTER(u32)), P]
         L.o_delaunay_bw.restype = i64
+        L.o_delaunay_bw_reference.argtypes = [P, i64, ctypes.POINTER(ctypes.POINTER(u32))]
+        L.o_delaunay_bw_reference.restype = i64
         L.o_free.argtypes = [P]
         L.o_canonical_real.argtypes = [P, i64, u32, P]
         L.o_canonical_real.restype = i64
@@ -172,6 +174,20 @@ def delaunay(points_normalized: np.ndarray, return_stats: bool = False):
     tets = np.ctypeslib.as_array(ptr, shape=(m * 4,)).copy().reshape(m, 4)
     lib().o_free(ctypes.cast(ptr, ctypes.c_void_p))
     return (tets, stats) if return_stats else tets
+
+
+def delaunay_reference(points_normalized: np.ndarray) -> np.ndarray:
+    """full_bw_reconstruction as written (src/phase2.rs:383-606): input
+    order, linear seed scan, append-only tets.  O(N^2); same contract as
+    ``delaunay``."""
+    pts = np.ascontiguousarray(points_normalized, dtype=np.float32).reshape(-1, 3)
+    ptr = ctypes.POINTER(ctypes.c_uint32)()
+    m = lib().o_delaunay_bw_reference(_p(pts), len(pts), ctypes.byref(ptr))
+    if m < 0:
+        raise RuntimeError("reference Bowyer-Watson restatement failed")
+    tets = np.ctypeslib.as_array(ptr, shape=(m * 4,)).copy().reshape(m, 4)
+    lib().o_free(ctypes.cast(ptr, ctypes.c_void_p))
+    return tets
 
 
 def canonical_real(tets: np.ndarray, n: int) -> np.ndarray:

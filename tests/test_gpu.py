@@ -177,17 +177,31 @@ def test_uniform_200k_matches_oracle(oracle, ctx):
     _check(oracle, ctx, W.uniform(200000, 11))
 
 
-def test_uniform_1m_properties(ctx):
-    """1,000,000 U[0,1)^3 f32 points, seed 1: no face fails the
-    insphere test and the counts are consistent (size-independent checks)."""
+def _assert_gold_set(oracle, tets, n, name):
+    """The canonical real-only set equals the oracle's, committed as count +
+    SHA-256 by tests/golden/make_fullsize_fixtures.py."""
+    g = GOLD["oracle_canonical_" + name]
+    can = oracle.canonical_real(tets, n)
+    assert len(can) == g["count"], (len(can), g["count"])
+    assert hashlib.sha256(can.astype("<u4").tobytes()).hexdigest() == g["sha256"]
+
+
+def test_uniform_1m_matches_oracle(oracle, ctx):
+    """1,000,000 U[0,1)^3 f32 points, seed 1: the canonical set equals the
+    oracle's (committed fixture); no face fails the insphere test and the
+    counts are consistent."""
     p = W.uniform(1000000, 1)
     assert W.sha256(p) == GOLD["uniform_1000000_seed1"]
     res = ctx.delaunay(p, wtet_amd.Config(collect_stats=True))
+    _assert_gold_set(oracle, res.tets, len(p), "uniform_1000000_seed1")
     v, interior = wtet_amd.check_quality(p, res)
     assert v == 0
     assert interior > 6 * len(p)
     ids = np.unique(res.tets)
     assert ids[0] == 0 and len(ids) == len(p) + 5  # every point, 4 super, infinity
+    # the exact-only path gives the same set
+    ex = ctx.delaunay(p, wtet_amd.Config(predicate_mode="exact"))
+    _assert_gold_set(oracle, ex.tets, len(p), "uniform_1000000_seed1")
 
 
 def test_context_reuse_and_smaller_second_call(oracle, ctx):
@@ -294,16 +308,17 @@ def test_resident_repair_on_neighbourhood(oracle, ctx, kind):
 
 
 @pytest.mark.parametrize("name", ["lattice_128", "sphere_1000000_seed2"])
-def test_degenerate_full_size_properties(ctx, name):
+def test_degenerate_full_size_matches_oracle(oracle, ctx, name):
     """Degenerate inputs at full size: the 128^3 lattice (2,097,152 points,
     co-spherical everywhere: exact stage and SoS) and 1M points on the unit
-    sphere.  Too large for the oracle in test time: the reference's own
-    audit (check_quality, src/lib.rs:25-78) must find no violation, every
-    point must be a vertex, and the adjacency must be closed."""
+    sphere.  The canonical set equals the oracle's (committed fixture); the
+    reference's own audit (check_quality, src/lib.rs:25-78) finds no
+    violation, every point is a vertex, and the adjacency is closed."""
     p = W.lattice(128) if name == "lattice_128" else W.sphere(1_000_000, 2)
     assert W.sha256(p) == GOLD[name]
     res = ctx.delaunay(p, wtet_amd.Config(collect_stats=True))
     st = res.stats
+    _assert_gold_set(oracle, res.tets, len(p), name)
     v, interior = wtet_amd.check_quality(p, res)
     assert v == 0, (v, st)
     assert interior > len(p)
@@ -397,3 +412,59 @@ def test_resident_repair_fetches_missing_rows(oracle, ctx, rings):
     assert np.array_equal(oracle.canonical_real(tets, len(p)), oracle.canonical_real(host.tets, len(p)))
     back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
     assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))
+
+
+@pytest.mark.parametrize("cap", [0, 1, 3])
+def test_insertion_cap_then_host_insertion(oracle, ctx, cap):
+    """max_insert_iterations stops the insertion rounds (src/phase1.rs:26;
+    the reference's own test runs max_insert_iterations=1,
+    src/tests.rs:1583-1595); the host phase inserts the points they left, so
+    the set still equals the oracle's, and failed_verts lists those points.
+    Host result, resident result and a batch all agree."""
+    p = W.uniform(4000, 81)
+    n = len(p)
+    res = _check(oracle, ctx, p, max_insert_iterations=cap)
+    st = res.stats
+    assert st["rounds"] == cap and st["points_left"] > 0, st
+    assert len(res.failed_verts) >= st["points_left"] - st["duplicates"]
+    assert np.all(np.diff(res.failed_verts.astype(np.int64)) > 0)
+    ref = oracle.canonical_real(res.tets, n)
+    d = ctx.upload(p)
+    try:
+        m, dt, da = ctx.delaunay_resident(d, n, wtet_amd.Config(max_insert_iterations=cap))
+        tets = ctx.download(dt, (m, 4), np.uint32)
+        adj = ctx.download(da, (m, 4), np.uint32)
+    finally:
+        ctx.free(d)
+    assert np.array_equal(oracle.canonical_real(tets, n), ref)
+    back = adj[(adj >> 5).ravel(), (adj & 3).ravel()]
+    assert np.array_equal(back, (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m))
+    q = W.clusters(2500, 82)
+    b = ctx.delaunay_batch([p, q], config=wtet_amd.Config(max_insert_iterations=cap))
+    assert np.array_equal(oracle.canonical_real(b[0].tets, n), ref)
+    qn = oracle.normalize_points(q)[0]
+    assert np.array_equal(oracle.canonical_real(b[1].tets, len(q)),
+                          oracle.canonical_real(oracle.delaunay(qn), len(q)))
+
+
+def test_insertion_cap_without_phase2_leaves_points_out(oracle, ctx):
+    """With enable_splaying off nothing finishes the mesh (src/lib.rs:227
+    skips phase 2): the points the rounds left are absent from the tets and
+    listed in failed_verts."""
+    p = W.uniform(3000, 83)
+    res = ctx.delaunay(p, wtet_amd.Config(max_insert_iterations=2, enable_splaying=False))
+    used = np.unique(res.tets[res.tets < len(p)])
+    absent = np.setdiff1d(np.arange(len(p)), used)
+    assert len(absent) == res.stats["points_left"] > 0
+    assert np.all(np.isin(absent, res.failed_verts))
+
+
+def test_no_flipping_overflows_audit_records(oracle, ctx):
+    """enable_flipping off: every non-Delaunay face the splits leave goes to
+    the host repair.  At 300k points these are more than the audit's first
+    record buffer (2^20 entries) holds; the audit runs again into a larger
+    one and the repair still reaches the oracle's set."""
+    p = W.uniform(300000, 84)
+    res = _check(oracle, ctx, p, enable_flipping=False)
+    assert res.stats["violations_gpu"] > (1 << 20), res.stats
+    assert res.stats["flip_iterations"] == 0

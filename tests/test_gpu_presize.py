@@ -107,3 +107,37 @@ def test_shipped_model_presizes_without_growing(oracle, ctx, family):
     assert sg["grow_events"] >= 2, sg
     if family == "sphere":
         assert sl["presize_slots"] < sf["presize_slots"], (sl["presize_slots"], sf["presize_slots"])
+
+
+def test_config5_anisotropic_5m_all_modes(oracle, ctx):
+    """The learned-presizing workload at full size: 5,000,000 points, 90%
+    from 32 anisotropic Gaussians + 10% uniform (seed 5).  Learned, fixed and
+    grow-and-retry presizing all give the oracle's canonical set (committed
+    fixture); the learned request covers the run without growing and asks
+    for at most 1.35x the slots used; grow-and-retry grows."""
+    import hashlib
+    import json
+    import os
+    if not os.path.exists(presize.MODEL_PATH):
+        pytest.skip("no trained model shipped yet")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    gold = json.load(open(os.path.join(root, "tests", "golden", "inputs_sha256.json")))
+    p = W.anisotropic(5_000_000, 5)
+    assert W.sha256(p) == gold["anisotropic_5000000_seed5"]
+    g = gold["oracle_canonical_anisotropic_5000000_seed5"]
+    st = {}
+    for mode in ("learned", "fixed", "grow"):
+        c = wtet_amd.Context(0)  # fresh buffers: growth is visible
+        try:
+            r = c.delaunay(p, wtet_amd.Config(presize=mode))
+            st[mode] = c.stats()
+        finally:
+            c.close()
+        can = oracle.canonical_real(r.tets, len(p))
+        del r
+        assert len(can) == g["count"], (mode, len(can))
+        assert hashlib.sha256(can.astype("<u4").tobytes()).hexdigest() == g["sha256"], mode
+    sl, sg = st["learned"], st["grow"]
+    assert sl["grow_events"] == 0, sl
+    assert sl["tet_slots"] <= sl["presize_slots"] <= 1.35 * sl["tet_slots"], sl
+    assert sg["grow_events"] >= 1, sg

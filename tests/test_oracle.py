@@ -355,3 +355,49 @@ def test_bw_agrees_with_qhull_on_uniform(oracle):
             q.append(np.vstack([ext[list(tt)], ext[s]]))
             idx.append(list(tt) + [s])
         assert np.any(oracle.orient4d(np.array(q), np.array(idx, np.uint32)) < 0)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "grid", "cospherical", "duplicates", "coplanar"])
+def test_reference_bw_restatement_matches_brute_force(oracle, kind):
+    """full_bw_reconstruction as written (input order, linear seed scan,
+    src/phase2.rs:383-606) gives the brute-force SoS Delaunay set on tiny
+    inputs, degenerate ones included."""
+    rng = np.random.default_rng(9)
+    if kind == "uniform":
+        p = rng.random((9, 3), dtype=np.float32)
+    elif kind == "grid":
+        p = _grid(3, 2, 2)
+    elif kind == "cospherical":
+        p = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                      [0.6, 0.8, 0], [0, 0.6, 0.8]], np.float32)
+    elif kind == "duplicates":
+        p = rng.random((6, 3), dtype=np.float32)
+        p = np.vstack([p, p[:3]])
+    else:
+        p = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0.5, 0.5, 0],
+                      [0.25, 0.75, 0], [0.5, 0.5, 1]], np.float32)
+    pn, _, _ = oracle.normalize_points(p)
+    got = sorted(tuple(sorted(int(v) for v in t)) for t in oracle.delaunay_reference(pn))
+    assert got == _brute_force_dt(oracle, pn)
+
+
+def test_reference_bw_restatement_matches_oracle_10k(oracle):
+    """The restated reference CPU path on the 10,000-point config-1 input
+    (seed 0) gives the oracle's committed canonical set, and on a 6^3
+    lattice (SoS everywhere) the oracle's set."""
+    import hashlib
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from wtet_amd import workloads as W
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "inputs_sha256.json")))
+    p = W.uniform(10000, 0)
+    assert W.sha256(p) == gold["uniform_10000_seed0"]
+    pn, _, _ = oracle.normalize_points(p)
+    can = oracle.canonical_real(oracle.delaunay_reference(pn), len(p))
+    g = gold["oracle_canonical_uniform_10000_seed0"]
+    assert len(can) == g["count"]
+    assert hashlib.sha256(can.astype("<u4").tobytes()).hexdigest() == g["sha256"]
+    q = _grid(6, 6, 6)
+    qn, _, _ = oracle.normalize_points(q)
+    assert np.array_equal(oracle.canonical_real(oracle.delaunay_reference(qn), len(q)),
+                          oracle.canonical_real(oracle.delaunay(qn), len(q)))

@@ -143,6 +143,11 @@ class _Stats(ctypes.Structure):
         ("repair_fetches", ctypes.c_uint32),
         ("repair_retries", ctypes.c_uint32),
         ("repair_max_region", ctypes.c_uint32),
+        ("points_left", ctypes.c_uint32),
+        ("flips_23", ctypes.c_uint64),
+        ("flips_32", ctypes.c_uint64),
+        ("small_checked", ctypes.c_uint64),
+        ("small_flips", ctypes.c_uint64),
     ]
 
 
@@ -744,9 +749,12 @@ class Config:
     """Triangulation options (src/python.rs:53-120; GDelConfig src/types.rs:32-61).
 
     ``predicate_mode`` ("filtered" | "exact"), ``collect_stats`` and
-    ``presize`` ("fixed": 7 tet slots per point; "learned": the presizing
-    network's prediction; "grow": grow-and-retry from 2 per point) are
-    additions of this library."""
+    ``presize`` ("fixed": the reference's max(8(N+5)+8, 12N, 64) tet slots,
+    src/gpu/mod.rs:101-108; "learned": the presizing network's prediction;
+    "grow": grow-and-retry from 2 per point) are additions of this library.
+    ``max_insert_iterations`` caps the insertion rounds as in
+    src/phase1.rs:26; the points they leave are inserted by the host phase
+    (when ``enable_splaying``) and listed in ``failed_verts``."""
 
     def __init__(self, insertion_rule: str = "circumcenter", enable_flipping: bool = True,
                  enable_sorting: bool = False, enable_hilbert_sorting: bool = False,

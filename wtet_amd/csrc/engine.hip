@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <atomic>
 #include <thread>
 #include <chrono>
@@ -312,6 +313,9 @@ void read_ctr(wtet_ctx* c) {
 struct Counters {
   uint64_t sumU = 0, sumA = 0, sumF = 0, peakA = 0;
   uint32_t rounds = 0, iters = 0;
+  uint32_t left = 0;  // points not inserted when max_insert_iterations stopped the rounds
+  uint64_t flips23 = 0;               // 2-3 flips of sumF
+  uint64_t smallA = 0, smallF = 0;    // of sumA, sumF: in one-workgroup iterations
 };
 
 void collect_times(wtet_ctx* c);
@@ -514,7 +518,7 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
   ensure(c, c->slow_p, (size_t)n * 4 + 16);
   ensure(c, c->ctr, C_COUNT * 4);
   ensure(c, c->gstats, 64);
-  ensure(c, c->isums, 64);
+  ensure(c, c->isums, S_COUNT * 8);
   ensure(c, c->rec, (1u << 20) * 4);
   ensure(c, c->blk, ((size_t)nblocks(n) + 2) * 4);
   if (R.learned) {
@@ -573,8 +577,8 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
     });
     HIP_CHECK(hipGetLastError());
   }
-  // tet slots for the whole call: fixed 7 per point, the learned
-  // prediction, or a small start that the grow-and-retry path extends
+  // tet slots for the whole call: the reference's fixed formula, the
+  // learned prediction, or a small start that the grow-and-retry path extends
   uint64_t slots;
   if (R.learned) {
     double sum = 0.0;
@@ -600,7 +604,7 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
   launch_start_tets(mesh_of(c, V, R.mode), CL, s);
   launch_set_ctr(P<uint32_t>(c->ctr), C_TETS, 5u * C, -1, 0u, s);
   HIP_CHECK(hipGetLastError());
-  launch_fill_u64(P<uint64_t>(c->isums), 0ull, 4, s);
+  launch_fill_u64(P<uint64_t>(c->isums), 0ull, S_COUNT, s);
   launch_set_ctr(P<uint32_t>(c->ctr), C_CAP, (uint32_t)c->cap_tets, -1, 0u, s);
 }
 
@@ -610,8 +614,7 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
 uint32_t dev_vote(wtet_ctx* c, Run& R) {
   hipStream_t s = c->stream;
   ++R.round;
-  if (R.round > std::max<uint32_t>(R.cfg.max_insert_iterations, 4096u))
-    throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
+  if (R.round > (1u << 16)) throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
   Mesh M = mesh_of(c, R.V, R.mode);
   const uint32_t U = R.U, cur = R.cur, round = R.round;
   launch_set_ctr(M.ctr, C_ULEN, U, C_ROUND, round, s);
@@ -723,6 +726,22 @@ void dev_compact(wtet_ctx* c, Run& R, uint32_t T, bool out, bool keep_lists) {
   HIP_CHECK(hipGetLastError());
 }
 
+// The audit once more into a record buffer that holds every violation the
+// first pass counted (C_FAILREC); leaves the counters in h_ctr.
+void reaudit(wtet_ctx* c, Run& R, uint32_t T) {
+  uint32_t need = c->h_ctr[C_FAILREC];
+  ensure(c, c->rec, ((size_t)need + need / 8 + 1024) * 4);
+  Mesh M = mesh_of(c, R.V, R.mode);
+  launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, c->stream);
+  const uint32_t rec_cap = (uint32_t)std::min<size_t>(c->rec.bytes / 4, 0xFFFFFFFFu);
+  timed(c, K_AUDIT, [&] {
+    launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, nullptr, P<uint32_t>(c->slow_t), c->stream);
+  });
+  HIP_CHECK(hipGetLastError());
+  read_ctr(c);
+  if (c->h_ctr[C_FAILREC] > rec_cap) throw Fail{WTET_ERR_INTERNAL, "audit records overflowed twice"};
+}
+
 // Audit of every finite face (gather.wgsl:68-151), compaction into rows,
 // and the copies the host needs.  Leaves the rows in prop/fout (cloud c
 // from cstart[c]), the violation records as rows, and -- unless resident --
@@ -731,12 +750,15 @@ void dev_output(wtet_ctx* c, Run& R, std::vector<uint32_t>& viol_rec, uint32_t& 
                 uint32_t& mout, std::vector<uint32_t>& cstart) {
   hipStream_t s = c->stream;
   const uint32_t C = R.C;
-  uint64_t sums[4] = {0, 0, 0, 0};
-  HIP_CHECK(hipMemcpyAsync(sums, c->isums.p, 32, hipMemcpyDeviceToHost, s));
+  uint64_t sums[S_COUNT] = {};
+  HIP_CHECK(hipMemcpyAsync(sums, c->isums.p, S_COUNT * 8, hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
-  R.K.sumA = sums[0];
-  R.K.sumF = sums[1];
-  R.K.iters = (uint32_t)sums[2];
+  R.K.sumA = sums[S_CHECKED];
+  R.K.sumF = sums[S_FLIPS];
+  R.K.iters = (uint32_t)sums[S_ITERS];
+  R.K.flips23 = sums[S_FLIPS23];
+  R.K.smallA = sums[S_SMALL_CHECKED];
+  R.K.smallF = sums[S_SMALL_FLIPS];
   // audit every finite face
   read_ctr(c);
   uint32_t T = c->h_ctr[C_TETS];
@@ -758,13 +780,18 @@ void dev_output(wtet_ctx* c, Run& R, std::vector<uint32_t>& viol_rec, uint32_t& 
   if (c->h_ctr[C_WALK_FAIL]) throw Fail{WTET_ERR_INTERNAL, "point relocation walk did not terminate"};
   uint32_t Mout = c->h_ctr[C_ALIVE];
   mout = Mout;
-  nviol = c->h_ctr[C_VIOL];
   if (cstart[C] != Mout) throw Fail{WTET_ERR_INTERNAL, "output row count mismatch"};
+  // more violations than records: a larger record buffer and the audit
+  // again (predicate counters already hold the first pass), so that the
+  // repair starts from every violated face
+  if (c->h_ctr[C_FAILREC] > rec_cap) reaudit(c, R, T);
+  nviol = c->h_ctr[C_VIOL];
   // the host needs every row only for a host result; a resident result's
   // repair downloads the neighbourhood it works on
   if (!resident) {
     // room for the repair to append (it rarely adds more than a few tets)
-    size_t cap = (size_t)Mout + Mout / 32 + 65536;
+    // and for the tets of the points the insertion cap left out
+    size_t cap = (size_t)Mout + Mout / 32 + 65536 + 8ull * R.K.left;
     size_t c1 = c->h_cap, c2 = c->h_cap;
     ensure_pinned(c->h_tets, c1, cap * 4);
     ensure_pinned(c->h_adj, c2, cap * 4);
@@ -772,7 +799,7 @@ void dev_output(wtet_ctx* c, Run& R, std::vector<uint32_t>& viol_rec, uint32_t& 
     HIP_CHECK(hipMemcpyAsync(c->h_tets, c->prop.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipMemcpyAsync(c->h_adj, c->fout.p, (size_t)Mout * 16, hipMemcpyDeviceToHost, s));
   }
-  uint32_t nrec = std::min(c->h_ctr[C_FAILREC], rec_cap);
+  uint32_t nrec = c->h_ctr[C_FAILREC];
   viol_rec.resize(nrec);
   if (nrec) {
     // records are slot-based; map them to rows on the device
@@ -794,7 +821,9 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   Run R;
   run_setup(c, R, off, cfg);
   dev_prepare(c, R, d_xyz);
-  while (R.U > 0) {
+  // at most max_insert_iterations rounds (src/phase1.rs:26); the host phase
+  // inserts what they leave
+  while (R.U > 0 && R.round < R.cfg.max_insert_iterations) {
     dev_vote(c, R);
     dev_split(c, R);
     dev_split_points(c, R);
@@ -805,6 +834,7 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
       launch_set_ctr(P<uint32_t>(c->ctr), C_A, 0u, -1, 0u, c->stream);
     }
   }
+  R.K.left = R.U;
   if (R.flipping) dev_finish_flips(c, R);
   dev_output(c, R, viol_rec, nviol, resident, mout, cstart);
   K = R.K;
@@ -976,8 +1006,8 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   return c->present[row] != 0;
 }
 
-void download_rows(wtet_ctx* c, uint32_t M) {
-  size_t cap = (size_t)M + M / 32 + 65536;
+void download_rows(wtet_ctx* c, uint32_t M, uint32_t left = 0) {
+  size_t cap = (size_t)M + M / 32 + 65536 + 8ull * left;
   size_t c1 = c->h_cap, c2 = c->h_cap;
   ensure_pinned(c->h_tets, c1, cap * 4);
   ensure_pinned(c->h_adj, c2, cap * 4);
@@ -1019,12 +1049,14 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
                    wtet_device_result* dout, std::chrono::steady_clock::time_point t0,
                    uint32_t region_cap) {
   const bool resident = dout != nullptr;
+  const uint32_t left = K.left;
   c->stats.violations_gpu = nviol;
+  c->stats.points_left = left;
   auto t1 = std::chrono::steady_clock::now();
   bool sparse = false;
-  if (resident && nviol > 0) {
-    if (rec.size() < nviol) {
-      download_rows(c, mout);  // records overflowed: repair needs all
+  if (resident && (nviol > 0 || left > 0)) {
+    if (left > 0 || rec.size() < nviol) {
+      download_rows(c, mout, left);  // point insertion needs every row
     } else {
       stage_neighbourhood(c, (uint32_t)rec.size(), mout);
       sparse = true;
@@ -1037,22 +1069,28 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     mb.known = mout;
     mb.fetch = [c, mout, &fetches](uint32_t row) { return fetch_rows(c, row, mout, fetches); };
   }
-  bool on_host = !resident || nviol > 0;
+  bool on_host = !resident || nviol > 0 || left > 0;
   std::vector<uint32_t> failed;
   if (on_host) failed = failed_vertices(mb.tets, mb.adj, rec, n);
+  // points the insertion cap left out (exact duplicates, never inserted,
+  // are among them until the host insertion skips them)
+  std::vector<uint32_t> missing;
+  if (left > 0) missing = missing_points(mb.tets, mb.count, n);
   bool repaired = false;
-  if (nviol > 0 && cfg.enable_splaying) {
+  if ((nviol > 0 || !missing.empty()) && cfg.enable_splaying) {
     // the repair needs the normalized coordinates of the input ids
     ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
     HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+  if (nviol > 0 && cfg.enable_splaying) {
     auto tc = std::chrono::steady_clock::now();
     c->stats.ms_repair_staging = std::chrono::duration<double, std::milli>(tc - t1).count();
     RepairReport rr;
     wtet_status rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr, region_cap);
     if (rs != WTET_OK && rr.needs_full && !rr.too_large) {
       // the neighbourhood was too small: every row, and again from the start
-      download_rows(c, mout);
+      download_rows(c, mout, left);
       mb = MeshBuf{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
       rr = RepairReport{};
       rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr, region_cap);
@@ -1067,11 +1105,33 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
     repaired = true;
   }
+  if (!missing.empty()) {
+    // phase 2 inserts what phase 1 left (src/lib.rs:227-307 rebuilds with
+    // every point); failed_verts lists those points
+    std::vector<uint32_t> added = missing;
+    if (cfg.enable_splaying) {
+      RepairReport ri;
+      wtet_status is = insert_points(c->h_xyzn, n, mb, missing, cfg.predicate_mode, &ri);
+      if (is != WTET_OK) throw Fail{is, "host point insertion failed: " + ri.message};
+      std::vector<uint32_t> still = missing_points(mb.tets, mb.count, n);
+      added.clear();
+      std::set_difference(missing.begin(), missing.end(), still.begin(), still.end(),
+                          std::back_inserter(added));
+      repaired = true;
+    }
+    std::vector<uint32_t> u;
+    std::set_union(failed.begin(), failed.end(), added.begin(), added.end(), std::back_inserter(u));
+    failed.swap(u);
+  }
   auto t2 = std::chrono::steady_clock::now();
   c->stats.ms_repair = std::chrono::duration<double, std::milli>(t2 - t1).count();
   c->stats.sum_uninserted = K.sumU;
   c->stats.sum_active = K.sumA;
   c->stats.sum_flips = K.sumF;
+  c->stats.flips_23 = K.flips23;
+  c->stats.flips_32 = K.sumF - K.flips23;
+  c->stats.small_checked = K.smallA;
+  c->stats.small_flips = K.smallF;
   c->stats.rounds = K.rounds;
   c->stats.flip_iterations = K.iters;
   c->stats.peak_active = K.peakA;
@@ -1090,8 +1150,9 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
           std::memcpy(q + 5, mb.adj + 4 * (size_t)sl, 16);
         }
         ensure(c, c->patch, np * 36);
-        if ((uint64_t)mb.count > c->prop.bytes / 16 || (uint64_t)mb.count > c->fout.bytes / 16)
-          throw Fail{WTET_ERR_INTERNAL, "repaired mesh larger than the device output buffers"};
+        // the repair and the host insertion may add rows
+        ensure(c, c->prop, (size_t)mb.count * 16, true);
+        ensure(c, c->fout, (size_t)mb.count * 16, true);
         HIP_CHECK(hipMemcpyAsync(c->patch.p, c->h_patch, np * 36, hipMemcpyHostToDevice, c->stream));
         launch_apply_patch(P<uint32_t>(c->patch), (uint32_t)np, P<uint32_t>(c->prop),
                            P<uint32_t>(c->fout), c->stream);
@@ -1205,6 +1266,7 @@ void batch_impl(void* a_) {
   bool resident = a.dout != nullptr;
   run_device(c, d_xyz, a.off, a.cfg, K, rec, nviol, resident, mout, cstart);
   c->stats.violations_gpu = nviol;
+  c->stats.points_left = K.left;
   auto t1 = std::chrono::steady_clock::now();
 
   // violation records (rows) per cloud, as cloud-local (row << 2) | face
@@ -1217,16 +1279,18 @@ void batch_impl(void* a_) {
   }
   std::sort(byc.begin(), byc.end());
   // where each cloud's rows sit in the pinned staging: the whole mesh for a
-  // host result; for a resident result only the clouds with violations,
-  // downloaded here one segment each
+  // host result; for a resident result only the clouds with violations (every
+  // cloud when the insertion cap left points), downloaded here one segment each
   std::vector<uint64_t> hoff(C, 0);
+  std::vector<uint8_t> on_host(C, resident ? 0 : 1);
+  for (auto& pr : byc) on_host[pr.first] = 1;
+  if (K.left > 0) on_host.assign(C, 1);
   if (!resident) {
     for (uint32_t cl = 0; cl < C; ++cl) hoff[cl] = cstart[cl];
-  } else if (!byc.empty()) {
+  } else if (!byc.empty() || K.left > 0) {
     uint64_t tot = 0;
-    for (size_t i = 0; i < byc.size(); ++i)
-      if (i == 0 || byc[i].first != byc[i - 1].first) {
-        uint32_t cl = byc[i].first;
+    for (uint32_t cl = 0; cl < C; ++cl)
+      if (on_host[cl]) {
         hoff[cl] = tot;
         tot += cstart[cl + 1] - cstart[cl];
       }
@@ -1234,9 +1298,8 @@ void batch_impl(void* a_) {
     ensure_pinned(c->h_tets, c1, (size_t)tot * 4 + 4);
     ensure_pinned(c->h_adj, c2, (size_t)tot * 4 + 4);
     c->h_cap = std::min(c1, c2);
-    for (size_t i = 0; i < byc.size(); ++i)
-      if (i == 0 || byc[i].first != byc[i - 1].first) {
-        uint32_t cl = byc[i].first;
+    for (uint32_t cl = 0; cl < C; ++cl)
+      if (on_host[cl]) {
         size_t cnt = cstart[cl + 1] - cstart[cl];
         HIP_CHECK(hipMemcpyAsync(c->h_tets + 4 * hoff[cl], P<uint32_t>(c->prop) + 4 * (size_t)cstart[cl],
                                  cnt * 16, hipMemcpyDeviceToHost, c->stream));
@@ -1263,9 +1326,16 @@ void batch_impl(void* a_) {
     std::sort(v.begin(), v.end());
     v.erase(std::unique(v.begin(), v.end()), v.end());
   }
-  // host repair of each cloud with violations, on a copy of its rows
+  // points the insertion cap left out, per cloud (cloud-local ids)
+  std::vector<std::vector<uint32_t>> missing(C);
+  if (K.left > 0)
+    for (uint32_t cl = 0; cl < C; ++cl)
+      missing[cl] = missing_points(c->h_tets + 4 * (size_t)hoff[cl], cstart[cl + 1] - cstart[cl],
+                                   a.off[cl + 1] - a.off[cl]);
+  // host repair of each cloud with violations, and insertion of the points
+  // the cap left, on a copy of its rows
   std::vector<CloudRows> rows(C);
-  if (nviol > 0 && a.cfg.enable_splaying) {
+  if ((nviol > 0 || K.left > 0) && a.cfg.enable_splaying) {
     ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
     HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1283,21 +1353,41 @@ void batch_impl(void* a_) {
       for (; i < byc.size() && byc[i].first == j.cl; ++i) j.viol.push_back(byc[i].second);
       jobs.push_back(std::move(j));
     }
+    {
+      std::vector<uint8_t> has(C, 0);
+      for (const Job& jb : jobs) has[jb.cl] = 1;
+      for (uint32_t cl = 0; cl < C; ++cl)
+        if (!has[cl] && !missing[cl].empty()) {
+          Job j;
+          j.cl = cl;
+          jobs.push_back(std::move(j));
+        }
+    }
     std::atomic<size_t> next{0};
     auto work = [&]() {
       for (size_t k; (k = next.fetch_add(1)) < jobs.size();) {
         Job& jb = jobs[k];
         uint32_t cl = jb.cl;
         size_t cnt = cstart[cl + 1] - cstart[cl];
-        size_t cap = cnt + cnt / 8 + 4096;
+        size_t cap = cnt + cnt / 8 + 4096 + 8 * missing[cl].size();
         CloudRows& R = rows[cl];
         R.tets.assign(c->h_tets + 4 * (size_t)hoff[cl], c->h_tets + 4 * (size_t)(hoff[cl] + cnt));
         R.adj.assign(c->h_adj + 4 * (size_t)hoff[cl], c->h_adj + 4 * (size_t)(hoff[cl] + cnt));
         R.tets.resize(4 * cap);
         R.adj.resize(4 * cap);
         MeshBuf mb{R.tets.data(), R.adj.data(), cnt, cap, {}};
-        jb.st = repair_delaunay(c->h_xyzn + 3 * (size_t)a.off[cl], a.off[cl + 1] - a.off[cl], mb,
-                                jb.viol, a.cfg.predicate_mode, &jb.rr);
+        const float* xc = c->h_xyzn + 3 * (size_t)a.off[cl];
+        const uint32_t nc = a.off[cl + 1] - a.off[cl];
+        if (!jb.viol.empty())
+          jb.st = repair_delaunay(xc, nc, mb, jb.viol, a.cfg.predicate_mode, &jb.rr);
+        if (jb.st == WTET_OK && !missing[cl].empty()) {
+          jb.st = insert_points(xc, nc, mb, missing[cl], a.cfg.predicate_mode, &jb.rr);
+          // failed_verts gets the points inserted here (not exact duplicates)
+          std::vector<uint32_t> still = missing_points(mb.tets, mb.count, nc), added;
+          std::set_difference(missing[cl].begin(), missing[cl].end(), still.begin(), still.end(),
+                              std::back_inserter(added));
+          missing[cl].swap(added);
+        }
         R.tets.resize(4 * mb.count);
         R.adj.resize(4 * mb.count);
         R.repaired = true;
@@ -1315,11 +1405,22 @@ void batch_impl(void* a_) {
       c->stats.repair_regions += jb.rr.regions;
     }
   }
+  for (uint32_t cl = 0; cl < C; ++cl)
+    if (!missing[cl].empty()) {
+      std::vector<uint32_t> u;
+      std::set_union(failed[cl].begin(), failed[cl].end(), missing[cl].begin(), missing[cl].end(),
+                     std::back_inserter(u));
+      failed[cl].swap(u);
+    }
   auto t2 = std::chrono::steady_clock::now();
   c->stats.ms_repair = std::chrono::duration<double, std::milli>(t2 - t1).count();
   c->stats.sum_uninserted = K.sumU;
   c->stats.sum_active = K.sumA;
   c->stats.sum_flips = K.sumF;
+  c->stats.flips_23 = K.flips23;
+  c->stats.flips_32 = K.sumF - K.flips23;
+  c->stats.small_checked = K.smallA;
+  c->stats.small_flips = K.smallF;
   c->stats.rounds = K.rounds;
   c->stats.flip_iterations = K.iters;
   c->stats.peak_active = K.peakA;
@@ -2304,10 +2405,10 @@ wtet_status wtet_stage_flip(wtet_ctx* c, int predicate_mode, wtet_stage_info* in
     read_ctr(c);
     if (c->h_ctr[C_STOP]) grow_for(c, (uint64_t)c->h_ctr[C_TETS] + c->h_ctr[C_A] + 64);
     uint32_t a = c->h_ctr[C_A];
-    uint64_t before[4] = {0, 0, 0, 0}, after[4] = {0, 0, 0, 0};
-    HIP_CHECK(hipMemcpyAsync(before, c->isums.p, 32, hipMemcpyDeviceToHost, c->stream));
+    uint64_t before[S_COUNT] = {}, after[S_COUNT] = {};
+    HIP_CHECK(hipMemcpyAsync(before, c->isums.p, S_COUNT * 8, hipMemcpyDeviceToHost, c->stream));
     if (a > 0) run_flip_batch(c, R.V, mode, R.st, 1, a, a / 8 + 1);
-    HIP_CHECK(hipMemcpyAsync(after, c->isums.p, 32, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipMemcpyAsync(after, c->isums.p, S_COUNT * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     S->checked = (uint32_t)(after[0] - before[0]);
     S->flips = (uint32_t)(after[1] - before[1]);
@@ -2346,8 +2447,9 @@ wtet_status wtet_stage_gather(wtet_ctx* c, wtet_stage_info* info) {
     });
     HIP_CHECK(hipGetLastError());
     read_ctr(c);
+    if (c->h_ctr[C_FAILREC] > rec_cap) reaudit(c, S->R, T);
     S->violations = c->h_ctr[C_VIOL];
-    S->nrec = std::min(c->h_ctr[C_FAILREC], rec_cap);
+    S->nrec = c->h_ctr[C_FAILREC];
     S->audit_valid = true;
     stage_info(c, S, info);
   });

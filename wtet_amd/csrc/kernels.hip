@@ -507,6 +507,7 @@ __device__ __forceinline__ void iter_begin(uint32_t* ctr) {
     ctr[C_ACT_NEXT] = 0u;
     ctr[C_FLIPS] = 0u;
     ctr[C_SLOW_T] = 0u;
+    ctr[C_TETS_IT] = ctr[C_TETS];
     if (A > ctr[C_PEAKA]) ctr[C_PEAKA] = A;
   }
 }
@@ -515,11 +516,15 @@ __global__ void k_iter_begin(uint32_t* ctr) {
   if (threadIdx.x == 0) iter_begin(ctr);
 }
 
+// sums (IterSum): checked tets, flips, iterations, 2-3 flips (a 2-3 flip
+// takes one fresh slot, a 3-2 none), then the checked tets and flips of the
+// iterations k_flip_small ran.
 __device__ __forceinline__ void iter_end(uint32_t* ctr, unsigned long long* sums) {
   if (!ctr[C_RUN]) return;
-  sums[0] += ctr[C_A];
-  sums[1] += ctr[C_FLIPS];
-  sums[2] += 1ull;
+  sums[S_CHECKED] += ctr[C_A];
+  sums[S_FLIPS] += ctr[C_FLIPS];
+  sums[S_ITERS] += 1ull;
+  sums[S_FLIPS23] += ctr[C_TETS] - ctr[C_TETS_IT];
   ctr[C_SLOW_SUM] += ctr[C_SLOW_T];
   ctr[C_A] = ctr[C_ACT_NEXT];
   ctr[C_PAR] ^= 1u;
@@ -654,6 +659,7 @@ __global__ void __launch_bounds__(kBlock) k_flip_small(Mesh M, FlipLists L, uint
           __atomic_store_n(&ctr[C_ACT_NEXT], 0u, __ATOMIC_RELAXED);
           __atomic_store_n(&ctr[C_FLIPS], 0u, __ATOMIC_RELAXED);
           __atomic_store_n(&ctr[C_SLOW_T], 0u, __ATOMIC_RELAXED);
+          __atomic_store_n(&ctr[C_TETS_IT], ld(C_TETS), __ATOMIC_RELAXED);
           if (A > ld(C_PEAKA)) __atomic_store_n(&ctr[C_PEAKA], A, __ATOMIC_RELAXED);
         }
       }
@@ -675,9 +681,12 @@ __global__ void __launch_bounds__(kBlock) k_flip_small(Mesh M, FlipLists L, uint
     for (uint32_t f = threadIdx.x; f < F; f += blockDim.x) flip_update_body(M, L.out, f, it);
     __syncthreads();
     if (threadIdx.x == 0) {
-      sums[0] += A;
-      sums[1] += F;
-      sums[2] += 1ull;
+      sums[S_CHECKED] += A;
+      sums[S_FLIPS] += F;
+      sums[S_ITERS] += 1ull;
+      sums[S_FLIPS23] += ld(C_TETS) - ld(C_TETS_IT);
+      sums[S_SMALL_CHECKED] += A;
+      sums[S_SMALL_FLIPS] += F;
       __atomic_store_n(&ctr[C_A], ld(C_ACT_NEXT), __ATOMIC_RELAXED);
       __atomic_store_n(&ctr[C_PAR], par ^ 1u, __ATOMIC_RELAXED);
     }

@@ -40,7 +40,19 @@ enum Ctr : int {
   C_SLOW_T = 22,   // tets deferred by the lean flip check this iteration
   C_SLOW_P = 23,   // points deferred by the lean point kernels
   C_SLOW_SUM = 24, // tets deferred by the lean flip check, summed over iterations
+  C_TETS_IT = 25,  // C_TETS when the current flip iteration began
   C_COUNT = 32
+};
+
+// per-call sums over flip iterations (u64, device)
+enum IterSum : int {
+  S_CHECKED = 0,        // tets checked
+  S_FLIPS = 1,          // flips executed
+  S_ITERS = 2,          // iterations
+  S_FLIPS23 = 3,        // 2-3 flips (the rest are 3-2)
+  S_SMALL_CHECKED = 4,  // of S_CHECKED: in k_flip_small
+  S_SMALL_FLIPS = 5,    // of S_FLIPS: in k_flip_small
+  S_COUNT = 8
 };
 
 // Adjacency of a tet slot in one 32-byte record: per face f, w[2f] = the

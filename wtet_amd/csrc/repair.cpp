@@ -533,9 +533,179 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
   return WTET_OK;
 }
 
-namespace {
-using namespace wtet;
-}  // namespace
+// ---------------------------------------------------------------------------
+// Bowyer-Watson insertion of the points the insertion rounds left out
+// ---------------------------------------------------------------------------
+wtet_status insert_points(const float* xyzn, uint32_t n, MeshBuf& mesh, std::vector<uint32_t> ids,
+                          int mode, RepairReport* rep) {
+  if (mesh.present) {
+    if (rep) rep->message = "point insertion needs every row on the host";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  PointSet S;
+  S.xyzn = xyzn;
+  S.n = n;
+  const float sv[4][3] = {{-100.f, -100.f, -100.f}, {400.f, -100.f, -100.f},
+                          {-100.f, 400.f, -100.f}, {-100.f, -100.f, 400.f}};
+  std::memcpy(S.sup, sv, sizeof(sv));
+  mode = mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
+  Mesh32 W{mesh, n + 4};
+  std::vector<uint32_t>& changed = mesh.changed;
+  // Morton order: each walk starts from the previous point's new tets
+  std::vector<std::pair<uint32_t, uint32_t>> order;
+  order.reserve(ids.size());
+  for (uint32_t q : ids) {
+    if (q >= n) return WTET_ERR_INVALID_ARGUMENT;
+    order.emplace_back(morton10(S.p(q)), q);
+  }
+  std::sort(order.begin(), order.end());
+  uint32_t last = kInvalid;
+  for (uint32_t t = 0; t < W.count() && last == kInvalid; ++t)
+    if (!W.dead(t) && !W.has_inf(t)) last = t;
+  if (last == kInvalid) {
+    if (rep) rep->message = "no finite tet to start from";
+    return WTET_ERR_INTERNAL;
+  }
+  auto conflict = [&](uint32_t t, uint32_t q) {
+    return !W.dead(t) && !W.has_inf(t) && o4(S, W.tv(t), q, mode) < 0;
+  };
+  std::vector<uint32_t> cav, stack;
+  std::unordered_set<uint32_t> incav;
+  struct BF {
+    uint32_t f[3];
+    uint32_t out;
+  };
+  std::vector<BF> bf;
+  std::unordered_map<uint64_t, uint32_t> edges;  // edge -> (slot << 2) | face
+  uint32_t rng = 0x9E3779B9u;
+  for (auto& pr : order) {
+    const uint32_t q = pr.second;
+    // visibility walk through finite tets: q lies inside the super tet, so
+    // a face it lies beyond is never a face of the super tet
+    uint32_t t = last;
+    bool found = false;
+    for (size_t step = 0; step < 4 * W.count() + 64 && !found; ++step) {
+      rng = rng * 1664525u + 1013904223u;
+      int off = (int)(rng >> 30);
+      bool moved = false;
+      for (int k = 0; k < 4 && !moved; ++k) {
+        int f = (k + off) & 3;
+        const uint32_t* tv = W.tv(t);
+        if (o3sos(S, tv[face_vi(f, 0)], tv[face_vi(f, 1)], tv[face_vi(f, 2)], q, mode) < 0) {
+          uint32_t e = W.ta(t, f);
+          if (e == kInvalid || W.has_inf(e >> 5)) {
+            if (rep) rep->message = "point walk left the super tetrahedron";
+            return WTET_ERR_INTERNAL;
+          }
+          t = e >> 5;
+          moved = true;
+        }
+      }
+      found = !moved;
+    }
+    if (!found || !conflict(t, q)) {
+      t = kInvalid;
+      for (uint32_t u = 0; u < W.count(); ++u)
+        if (conflict(u, q)) {
+          t = u;
+          break;
+        }
+      // no tet in conflict: an exact duplicate, redundant under the lifted
+      // SoS (the reference's Bowyer-Watson skips it, src/phase2.rs:440-443)
+      if (t == kInvalid) continue;
+    }
+    // cavity: the connected tets whose circumsphere holds q
+    cav.assign(1, t);
+    stack.assign(1, t);
+    incav.clear();
+    incav.insert(t);
+    std::unordered_set<uint32_t> outside;
+    while (!stack.empty()) {
+      uint32_t u = stack.back();
+      stack.pop_back();
+      for (int f = 0; f < 4; ++f) {
+        uint32_t e = W.ta(u, f);
+        if (e == kInvalid) continue;
+        uint32_t w = e >> 5;
+        if (incav.count(w) || outside.count(w)) continue;
+        if (conflict(w, q)) {
+          incav.insert(w);
+          cav.push_back(w);
+          stack.push_back(w);
+        } else {
+          outside.insert(w);
+        }
+      }
+    }
+    bf.clear();
+    for (uint32_t u : cav)
+      for (int f = 0; f < 4; ++f) {
+        uint32_t e = W.ta(u, f);
+        if (e != kInvalid && incav.count(e >> 5)) continue;
+        BF b;
+        const uint32_t* tv = W.tv(u);
+        for (int k = 0; k < 3; ++k) b.f[k] = tv[face_vi(f, k)];
+        b.out = e;
+        bf.push_back(b);
+      }
+    // a cavity of c tets has 2c + 2 boundary faces: its slots, then new ones
+    if (mesh.count + (bf.size() - cav.size()) > mesh.cap) {
+      if (rep) rep->message = "point insertion needs more room than the staging buffer has";
+      return WTET_ERR_OUT_OF_MEMORY;
+    }
+    edges.clear();
+    for (size_t i = 0; i < bf.size(); ++i) {
+      uint32_t slot = i < cav.size() ? cav[i] : (uint32_t)mesh.count++;
+      const BF& b = bf[i];
+      uint32_t x = b.f[0], y = b.f[1], z = b.f[2];
+      if (o3sos(S, x, y, z, q, mode) <= 0) std::swap(x, y);
+      uint32_t* tv = W.tv(slot);
+      tv[0] = x;
+      tv[1] = y;
+      tv[2] = z;
+      tv[3] = q;
+      W.ta(slot, 3) = b.out;
+      if (b.out != kInvalid) {
+        W.ta(b.out >> 5, (int)(b.out & 3u)) = (slot << 5) | 3u;
+        changed.push_back(b.out >> 5);
+      }
+      for (int f = 0; f < 3; ++f) {
+        uint32_t e0 = tv[(f + 1) % 3], e1 = tv[(f + 2) % 3];
+        uint64_t key = e0 < e1 ? ((uint64_t)e0 << 32 | e1) : ((uint64_t)e1 << 32 | e0);
+        auto it = edges.find(key);
+        if (it == edges.end()) {
+          edges.emplace(key, (slot << 2) | (uint32_t)f);
+        } else {
+          uint32_t o = it->second;
+          W.ta(slot, f) = ((o >> 2) << 5) | (o & 3u);
+          W.ta(o >> 2, (int)(o & 3u)) = (slot << 5) | (uint32_t)f;
+          edges.erase(it);
+        }
+      }
+      changed.push_back(slot);
+      last = slot;
+    }
+    if (!edges.empty()) {
+      if (rep) rep->message = "point cavity boundary is not closed";
+      return WTET_ERR_INTERNAL;
+    }
+    if (rep) rep->inserted++;
+  }
+  std::sort(changed.begin(), changed.end());
+  changed.erase(std::unique(changed.begin(), changed.end()), changed.end());
+  return WTET_OK;
+}
+
+// input ids 0..n-1 that no tet of rows [0, count) uses
+std::vector<uint32_t> missing_points(const uint32_t* tets, size_t count, uint32_t n) {
+  std::vector<uint8_t> seen(n, 0);
+  for (size_t i = 0; i < 4 * count; ++i)
+    if (tets[i] < n) seen[tets[i]] = 1;
+  std::vector<uint32_t> out;
+  for (uint32_t v = 0; v < n; ++v)
+    if (!seen[v]) out.push_back(v);
+  return out;
+}
 
 // ---------------------------------------------------------------------------
 // check_quality (src/python.rs:282-339 -> src/lib.rs:25-78)

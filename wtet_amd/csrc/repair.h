@@ -18,6 +18,7 @@ struct RepairReport {
   bool needs_full = false;   // the repair reached a row the sparse view lacks
   bool too_large = false;    // a region outgrew the cap the caller set
   uint32_t fetches = 0;      // rows fetched into the sparse view on demand
+  uint32_t inserted = 0;     // points inserted by insert_points
   std::string message;
 };
 
@@ -51,3 +52,15 @@ struct MeshBuf {
 wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
                             const std::vector<uint32_t>& viol, int mode, RepairReport* rep,
                             uint32_t region_cap = 0);
+
+// Inserts the input ids `ids` (points the insertion rounds did not insert,
+// src/phase1.rs:26 stops them at max_insert_iterations) into a mesh that is
+// already the exact Delaunay triangulation of the others, one Bowyer-Watson
+// cavity each (src/phase2.rs:383-606 inserts the same way); the mesh stays
+// Delaunay.  Every row must be on the host (no sparse view).  Exact
+// duplicates have no cavity and are skipped.
+wtet_status insert_points(const float* xyzn, uint32_t n, MeshBuf& mesh, std::vector<uint32_t> ids,
+                          int mode, RepairReport* rep);
+
+// input ids 0..n-1 used by no tet among `count` rows
+std::vector<uint32_t> missing_points(const uint32_t* tets, size_t count, uint32_t n);
