@@ -383,6 +383,9 @@ wtet_status wtet_stage_end(wtet_ctx *ctx);
  * full_bw_reconstruction -> splay, src/phase2.rs): turns a valid
  * triangulation of the n normalized points (ids 0..n-1, super vertices
  * n..n+3, optional infinity tets with id n+4) into the exact Delaunay one.
+ * Input ids that no tet uses are then inserted one Bowyer-Watson cavity at
+ * a time (the mesh of the others must contain at least one tet of real or
+ * super vertices), so the five start tets alone give the whole mesh.
  * *tets (num_tets*4) and *adjacency (same layout as wtet_result; may be
  * null: rebuilt by face matching) are read, then replaced by new malloc'd
  * arrays the caller frees with free().  regions (may be null) receives the

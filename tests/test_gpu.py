@@ -461,10 +461,10 @@ def test_insertion_cap_without_phase2_leaves_points_out(oracle, ctx):
 
 def test_no_flipping_overflows_audit_records(oracle, ctx):
     """enable_flipping off: every non-Delaunay face the splits leave goes to
-    the host repair.  At 300k points these are more than the audit's first
-    record buffer (2^20 entries) holds; the audit runs again into a larger
-    one and the repair still reaches the oracle's set."""
-    p = W.uniform(300000, 84)
+    the host repair, about one per point.  At 1.15M points these are more
+    than the audit's first record buffer (2^20 entries) holds; the audit runs
+    again into a larger one and the repair still reaches the oracle's set."""
+    p = W.uniform(1_150_000, 84)
     res = _check(oracle, ctx, p, enable_flipping=False)
     assert res.stats["violations_gpu"] > (1 << 20), res.stats
     assert res.stats["flip_iterations"] == 0

@@ -135,6 +135,37 @@ def test_repair_restores_delaunay_from_perturbed_mesh(oracle):
     assert oracle.count_violations(pn, out_t, oracle.rebuild_adjacency(out_t), only_real=False) == 0
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_repair_inserts_points_no_tet_uses(oracle, mode):
+    """Points absent from the mesh (the insertion rounds stopped at
+    max_insert_iterations, src/phase1.rs:26) are inserted by the host phase:
+    from the super tet alone, and from the exact triangulation of the first
+    half of the points.  In 3D a cavity may have fewer boundary faces than
+    tets; the slots it leaves must not survive as stale tets, so the tet set
+    and the adjacency are checked against the oracle's."""
+    rng = np.random.default_rng(11 + mode)
+    p = np.concatenate([rng.random((3000, 3), dtype=np.float32),
+                        (0.5 + 0.05 * rng.standard_normal((1000, 3))).astype(np.float32)])
+    pn, _, _ = oracle.normalize_points(p)
+    n = len(pn)
+    ref = oracle.canonical_real(oracle.delaunay(pn), n)
+    k = n // 2
+    half = oracle.delaunay(pn[:k]).astype(np.int64)
+    half[half >= k] += n - k  # super vertices k..k+3 -> n..n+3
+    for start in (np.array([[n, n + 1, n + 2, n + 3]], np.uint32), half.astype(np.uint32)):
+        out_t, out_a, _ = _core.repair(pn, start, None, mode=mode)
+        assert np.array_equal(oracle.canonical_real(out_t, n), ref)
+        m = len(out_t)
+        assert np.all(out_t < n + 4)
+        assert np.array_equal(np.sort(out_a.ravel()), np.sort(oracle.rebuild_adjacency(out_t).ravel()))
+        a = out_a.ravel()
+        own = (np.repeat(np.arange(m), 4) << 5) | np.tile(np.arange(4), m)
+        ok = a != 0xFFFFFFFF  # the super tet's outer faces have no neighbour
+        assert ok.sum() == 4 * m - 4
+        back = out_a[a[ok] >> 5, a[ok] & 3]
+        assert np.array_equal(back, own[ok])
+
+
 def test_check_quality_counts_like_reference(oracle):
     rng = np.random.default_rng(9)
     p = rng.random((300, 3), dtype=np.float32)

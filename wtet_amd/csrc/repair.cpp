@@ -278,6 +278,34 @@ struct Mesh32 {
   }
 };
 
+// Fills the dead slots `holes` with live tets moved from the end of the
+// rows (their neighbours' back-pointers follow), then drops dead tail rows,
+// so that rows [0, count) hold live tets only.
+void fill_holes(Mesh32& W, std::vector<uint32_t>& holes, std::vector<uint32_t>& changed) {
+  MeshBuf& mesh = W.m;
+  std::sort(holes.begin(), holes.end());
+  auto trim = [&]() {
+    while (mesh.count > 0 && W.dead((uint32_t)(mesh.count - 1))) --mesh.count;
+  };
+  for (uint32_t h : holes) {
+    trim();
+    if (h >= mesh.count) continue;
+    uint32_t last = (uint32_t)(mesh.count - 1);
+    std::memcpy(W.tv(h), W.tv(last), 16);
+    for (int f = 0; f < 4; ++f) {
+      uint32_t e = W.ta(last, f);
+      W.ta(h, f) = e;
+      if (e != kInvalid) {
+        W.ta(e >> 5, (int)(e & 3u)) = (h << 5) | (uint32_t)f;
+        changed.push_back(e >> 5);
+      }
+    }
+    changed.push_back(h);
+    --mesh.count;
+  }
+  trim();
+}
+
 bool face_bad(const PointSet& S, const Mesh32& W, uint32_t t, int f, int mode) {
   uint32_t e = W.ta(t, f);
   if (e == kInvalid) return false;
@@ -505,28 +533,7 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
     if (fail != WTET_OK) return fail;
     queue.pop_front();
   }
-  // fill holes with tets moved from the end
-  std::sort(holes.begin(), holes.end());
-  auto trim = [&]() {
-    while (mesh.count > 0 && W.dead((uint32_t)(mesh.count - 1))) --mesh.count;
-  };
-  for (uint32_t h : holes) {
-    trim();
-    if (h >= mesh.count) continue;
-    uint32_t last = (uint32_t)(mesh.count - 1);
-    std::memcpy(W.tv(h), W.tv(last), 16);
-    for (int f = 0; f < 4; ++f) {
-      uint32_t e = W.ta(last, f);
-      W.ta(h, f) = e;
-      if (e != kInvalid) {
-        W.ta(e >> 5, (int)(e & 3u)) = (h << 5) | (uint32_t)f;
-        changed.push_back(e >> 5);
-      }
-    }
-    changed.push_back(h);
-    --mesh.count;
-  }
-  trim();
+  fill_holes(W, holes, changed);
   std::sort(changed.begin(), changed.end());
   changed.erase(std::unique(changed.begin(), changed.end()), changed.end());
   while (!changed.empty() && changed.back() >= mesh.count) changed.pop_back();
@@ -577,6 +584,7 @@ wtet_status insert_points(const float* xyzn, uint32_t n, MeshBuf& mesh, std::vec
   };
   std::vector<BF> bf;
   std::unordered_map<uint64_t, uint32_t> edges;  // edge -> (slot << 2) | face
+  std::vector<uint32_t> holes;
   uint32_t rng = 0x9E3779B9u;
   for (auto& pr : order) {
     const uint32_t q = pr.second;
@@ -648,8 +656,10 @@ wtet_status insert_points(const float* xyzn, uint32_t n, MeshBuf& mesh, std::vec
         b.out = e;
         bf.push_back(b);
       }
-    // a cavity of c tets has 2c + 2 boundary faces: its slots, then new ones
-    if (mesh.count + (bf.size() - cav.size()) > mesh.cap) {
+    // the new tets take the cavity's slots, then appended ones; in 3D a
+    // cavity may have fewer boundary faces than tets, and its leftover
+    // slots become holes
+    if (bf.size() > cav.size() && mesh.count + (bf.size() - cav.size()) > mesh.cap) {
       if (rep) rep->message = "point insertion needs more room than the staging buffer has";
       return WTET_ERR_OUT_OF_MEMORY;
     }
@@ -685,14 +695,20 @@ wtet_status insert_points(const float* xyzn, uint32_t n, MeshBuf& mesh, std::vec
       changed.push_back(slot);
       last = slot;
     }
+    for (size_t k = bf.size(); k < cav.size(); ++k) {
+      W.tv(cav[k])[0] = kInvalid;
+      holes.push_back(cav[k]);
+    }
     if (!edges.empty()) {
       if (rep) rep->message = "point cavity boundary is not closed";
       return WTET_ERR_INTERNAL;
     }
     if (rep) rep->inserted++;
   }
+  fill_holes(W, holes, changed);
   std::sort(changed.begin(), changed.end());
   changed.erase(std::unique(changed.begin(), changed.end()), changed.end());
+  while (!changed.empty() && changed.back() >= mesh.count) changed.pop_back();
   return WTET_OK;
 }
 
@@ -815,12 +831,18 @@ extern "C" wtet_status wtet_repair(const float* xyzn, uint32_t n, uint32_t** tet
         if (o4(S, v, w[g], pm) < 0) viol.push_back(((uint32_t)t << 2) | (uint32_t)f);
       }
     }
+    // input ids no tet uses are inserted after the repair (phase 2
+    // rebuilds with every point, src/phase2.rs:383-606)
+    std::vector<uint32_t> missing = missing_points(tv.data(), M, n);
+    const size_t room = 65536 + 8 * missing.size();
     RepairReport rr;
     wtet_status s = WTET_OK;
-    tv.resize(tv.size() + 4 * 65536);
-    ta.resize(ta.size() + 4 * 65536);
-    MeshBuf mb{tv.data(), ta.data(), M, M + 65536, {}};
+    tv.resize(tv.size() + 4 * room);
+    ta.resize(ta.size() + 4 * room);
+    MeshBuf mb{tv.data(), ta.data(), M, M + room, {}};
     if (!viol.empty()) s = repair_delaunay(xyzn, n, mb, viol, mode, &rr);
+    if (s != WTET_OK) return s;
+    if (!missing.empty()) s = insert_points(xyzn, n, mb, missing, mode, &rr);
     if (s != WTET_OK) return s;
     tv.resize(4 * mb.count);
     ta.resize(4 * mb.count);
