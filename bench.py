@@ -24,11 +24,41 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# algorithmic bytes per unit (minimal SoA traffic):
-#   flip_check (the FP32-only check kernel): per active tet checked -- own
-#   quad + adjacency 32, four neighbour quads 64, eight vertex xyz 96, vote
-#   atomic 8
-BYTES_PER_UNIT = {"flip_check": 200}
+# Algorithmic bytes of each kernel group over one call, from the run's own
+# counters (minimal SoA traffic per unit):
+#   flip_check (FP32-only check): per tet checked -- own quad + adjacency 32,
+#     four neighbour quads 64, eight vertex xyz 96, vote atomic 8 = 200
+#   flip_decide_exec: per tet checked 24 (its vote word, the face's vote
+#     word, info) + per executed flip 208 (read/write <= 3 tets 192, flip
+#     record 16)
+#   flip_update: per executed flip 48 (six external back-pointers 24 and
+#     their messages 24); decide_exec + update = 24 per check + 256 per flip
+#   flip_small: the three above for the short-list iterations
+#   split: per insertion 224; audit: per tet slot 200; output: per tet 32
+def kernel_bytes(st):
+    a_big = st["sum_active"] - st["small_checked"]
+    f_big = st["sum_flips"] - st["small_flips"]
+    inserted = st["num_points"] - st["points_left"] - st["duplicates"]
+    return {
+        "flip_check": 200 * a_big,
+        "flip_decide_exec": 24 * a_big + 208 * f_big,
+        "flip_update": 48 * f_big,
+        "flip_small": 224 * st["small_checked"] + 256 * st["small_flips"],
+        "split": 224 * inserted,
+        "audit": 200 * st["tet_slots"],
+        "output": 32 * st["num_tets_out"],
+    }
+
+
+KERNEL_NAMES = {
+    "flip_check": "k_flip_check<LEAN> (FP32-only flip check)",
+    "flip_decide_exec": "k_flip_decide_exec (flip vote, claim and 2-3 / 3-2 execution)",
+    "flip_update": "k_flip_update (external adjacency through flip messages)",
+    "flip_small": "k_flip_small (one-workgroup iterations of short lists)",
+    "split": "k_split (1-4 split)",
+    "audit": "k_audit<LEAN> (exact audit of every finite face)",
+    "output": "k_output (compaction and id remap)",
+}
 
 
 def job_bytes(st):
@@ -38,6 +68,26 @@ def job_bytes(st):
     n = st["num_points"]
     return (12 * n + 128 * st["sum_uninserted"] + 224 * n + 200 * st["sum_active"]
             + 256 * st["sum_flips"] + 32 * st["num_tets_out"])
+
+
+def host_info():
+    """Core count, CPU model and the oracle's compiler flags of this host."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    flags = None
+    try:
+        for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+            if line.startswith("CFLAGS"):
+                flags = line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model, "compiler": "gcc", "cflags": flags}
 
 
 def main():
@@ -53,6 +103,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-cloud batched leg")
     ap.add_argument("--no-stress", action="store_true", help="skip the lattice / sphere leg")
     ap.add_argument("--no-presize", action="store_true", help="skip the learned-presizing leg")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the 10,000-point Hilbert exact-only leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default entries per thread instead of autotuning them")
     ap.add_argument("--no-graphs", action="store_true",
@@ -120,6 +172,20 @@ def main():
     print(f"filtered: {ms:.2f} ms/step, {value:.4g} points/s", file=sys.stderr, flush=True)
     st_t = ctx.stats()
 
+    # the same step with the flip iterations launched directly instead of
+    # replayed from HIP graphs (the form profiled under rocprofv3)
+    direct = None
+    if not args.no_graphs:
+        ctx.set_flip_graphs(False)
+        run(filt, 1)
+        wall, dev = [], []
+        for _ in range(3):
+            wall.append(run(filt, 1))
+            dev.append(ctx.stats()["ms_device"])
+        ctx.set_flip_graphs(True)
+        direct = {"ms_per_step_median": round(float(np.median(wall)), 3),
+                  "ms_device_median": round(float(np.median(dev)), 3)}
+
     # end to end: host points in, host tets and adjacency out (PCIe both ways)
     ctx.delaunay(pts, filt)
     t0 = time.perf_counter()
@@ -138,22 +204,41 @@ def main():
     kt = ctx.kernel_times()
     st_p = ctx.stats()
     ctx.set_profiling(False)
-    dom = "flip_check"
+    # roofline of the kernel group with the most measured time
+    kb = kernel_bytes(st_p)
+    per_kernel = {}
+    for name, (launches, tot_ms) in kt.items():
+        if not launches or name not in kb:
+            continue
+        gbs = kb[name] / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else 0.0
+        per_kernel[name] = {"launches": launches, "ms": round(tot_ms, 3),
+                            "bytes_per_launch": int(kb[name] / launches),
+                            "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
     launches, tot_ms = kt[dom]
-    bytes_dom = BYTES_PER_UNIT[dom] * st_p["sum_active"]
-    achieved = bytes_dom / (tot_ms / 1e3) / 1e9 if tot_ms > 0 else 0.0
-    # measured HBM bytes per checked tet (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    # copy-kernel calibrated) times this run's checked tets per launch
+    achieved = per_kernel[dom]["achieved_GBs"]
+    # measured HBM bytes per unit (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    # copy-kernel calibrated) times this run's units per launch
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "flip_check_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "flip_kernels_traffic.json")
     if os.path.exists(tpath):
-        per_tet = json.load(open(tpath))["traffic_bytes_per_checked_tet"]
-        traffic = int(per_tet * st_p["sum_active"] / max(launches, 1))
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        tj = json.load(open(tpath)).get(dom)
+        if tj:
+            units = {"flip_check": st_p["sum_active"] - st_p["small_checked"],
+                     "flip_decide_exec": st_p["sum_flips"] - st_p["small_flips"],
+                     "flip_update": st_p["sum_flips"] - st_p["small_flips"]}.get(dom)
+            if units is not None:
+                traffic = int(tj["bytes_per_unit"] * units / max(launches, 1))
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_flip_check<LEAN> (FP32-only flip check)", "launches": launches,
+                "kernel": KERNEL_NAMES[dom], "launches": launches,
                 "avg_launch_ms": round(tot_ms / max(launches, 1), 4),
-                "bytes_per_launch_avg": int(bytes_dom / max(launches, 1))}
+                "bytes_per_launch_avg": int(kb[dom] / max(launches, 1))}
+    flip_counts = {"sum_active": st_p["sum_active"], "sum_flips": st_p["sum_flips"],
+                   "flips_23": st_p["flips_23"], "flips_32": st_p["flips_32"],
+                   "small_checked": st_p["small_checked"], "small_flips": st_p["small_flips"],
+                   "per_point": {"checks": round(st_p["sum_active"] / args.n, 2),
+                                 "flips": round(st_p["sum_flips"] / args.n, 2)}}
 
     plain = None
     if not args.no_plain:
@@ -243,16 +328,53 @@ def main():
             c2.delaunay_resident(d_p, len(pp), cfg_p)
             ms_first = (time.perf_counter() - t0) * 1e3
             sp = c2.stats()
-            t0 = time.perf_counter()
-            c2.delaunay_resident(d_p, len(pp), cfg_p)
-            ms_again = (time.perf_counter() - t0) * 1e3
+            # later calls on the same context: device phase and host repair
+            # apart, median of three, so the presizing effect shows above the
+            # repair's call-to-call variation
+            again, dev, rep = [], [], []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                c2.delaunay_resident(d_p, len(pp), cfg_p)
+                again.append((time.perf_counter() - t0) * 1e3)
+                sa = c2.stats()
+                dev.append(sa["ms_device"])
+                rep.append(sa["ms_repair"])
             c2.free(d_p)
             c2.close()
-            presized[mode] = {"ms_first_call": round(ms_first, 3), "ms_second_call": round(ms_again, 3),
+            presized[mode] = {"ms_first_call": round(ms_first, 3),
+                              "ms_device_first_call": round(sp["ms_device"], 3),
+                              "ms_repair_first_call": round(sp["ms_repair"], 3),
+                              "ms_call_median": round(float(np.median(again)), 3),
+                              "ms_device_median": round(float(np.median(dev)), 3),
+                              "ms_repair_median": round(float(np.median(rep)), 3),
                               "value_first_call": round(len(pp) / (ms_first / 1e3), 1),
-                              "slots_requested": sp["presize_slots"], "slots_used": sp["tet_slots"],
+                              "slots_requested": sp["presize_slots"], "slots_allocated": sp["tet_capacity"],
+                              "slots_used": sp["tet_slots"],
                               "grow_events": sp["grow_events"], "peak_active": sp["peak_active"]}
-            print(f"presize {mode}: {ms_first:.1f} ms first call", file=sys.stderr, flush=True)
+            print(f"presize {mode}: {ms_first:.1f} ms first call, device {sp['ms_device']:.1f} ms, "
+                  f"later calls: device median {np.median(dev):.1f} ms", file=sys.stderr, flush=True)
+
+    # the reference's config 1: 10,000 points U[0,1)^3 seed 0, Hilbert order,
+    # exact predicates only, one instance (median of resident calls)
+    cfg1 = None
+    p1 = W.uniform(10_000, 0)
+    c1 = wtet_amd.Config(predicate_mode="exact", enable_hilbert_sorting=True)
+    if not args.no_config1:
+        d_1 = ctx.upload(p1)
+        for _ in range(3):
+            ctx.delaunay_resident(d_1, len(p1), c1)
+        t1 = []
+        for _ in range(20):
+            t0 = time.perf_counter()
+            ctx.delaunay_resident(d_1, len(p1), c1)
+            t1.append((time.perf_counter() - t0) * 1e3)
+        ctx.free(d_1)
+        ms1 = float(np.median(t1))
+        r1 = ctx.delaunay(p1, c1)
+        cfg1 = {"workload": "uniform_10000_seed0", "points": len(p1), "order": "hilbert",
+                "predicates": "exact only", "ms_median": round(ms1, 3), "calls": len(t1),
+                "value": round(len(p1) / (ms1 / 1e3), 1)}
+        print(f"config 1 (10k, Hilbert, exact only): {ms1:.2f} ms median", file=sys.stderr, flush=True)
 
     cpu = None
     parity = None
@@ -264,10 +386,26 @@ def main():
         dt = time.perf_counter() - t0
         cpu = {"value": args.n / dt, "unit": "points/s", "cores": 1, "kind": "port",
                "sample": f"the whole {args.n}-point input, serial Bowyer-Watson of the oracle "
-                         f"({dt:.1f} s, exact predicates + SoS)"}
+                         f"({dt:.1f} s, exact predicates + SoS, walk from the last new tet)",
+               "host": host_info(), "threads": 1}
         ref = O.canonical_real(tets, args.n)
         got = O.canonical_real(res.tets, args.n)
         parity = bool(ref.shape == got.shape and np.array_equal(ref, got))
+    if not args.no_cpu and cfg1 is not None:
+        # config 1 on the CPU: the same oracle, and full_bw_reconstruction as
+        # written (src/phase2.rs:383-606: input order, linear seed scan)
+        pn1, _, _ = O.normalize_points(p1)
+        t0 = time.perf_counter()
+        ref1 = O.canonical_real(O.delaunay(pn1), len(p1))
+        dt_walk = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref1b = O.canonical_real(O.delaunay_reference(pn1), len(p1))
+        dt_scan = time.perf_counter() - t0
+        cfg1["parity_with_oracle"] = bool(np.array_equal(O.canonical_real(r1.tets, len(p1)), ref1)
+                                          and np.array_equal(ref1, ref1b))
+        cfg1["cpu"] = {"oracle_walk_seed_ms": round(dt_walk * 1e3, 1),
+                       "reference_linear_seed_scan_ms": round(dt_scan * 1e3, 1), "cores": 1}
+        cpu["config1"] = cfg1["cpu"]
 
     kern = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in kt.items() if v[0]}
     out = {
@@ -287,8 +425,15 @@ def main():
                    "distribution": "i.i.d. U[0,1)^3 f32, numpy PCG64", "predicates": "f32->f64->exact",
                    "batched": False, "insertion_ring": args.ring, "autotuned_items_per_thread": items,
                    "flip_graphs": not args.no_graphs},
+        "value_definition": "GPU loop only: points resident in HBM before the timed region, tets and "
+                            "adjacency left in HBM (host repair included); end_to_end adds the "
+                            "host-to-device and device-to-host copies",
         "roofline": roofline,
+        "kernel_rooflines": per_kernel,
+        "flip_counts": flip_counts,
+        "direct_launches": direct,
         "cpu_baseline": cpu,
+        "config1": cfg1,
         "plain_exact": plain,
         "speedup_vs_plain": round(value / plain["value"], 3) if plain else None,
         "end_to_end": end_to_end,
@@ -299,6 +444,10 @@ def main():
         "stage_fractions": stages,
         "ms_device": round(st_t["ms_device"], 3),
         "ms_repair": round(st_t["ms_repair"], 3),
+        "ms_repair_staging": round(st_t["ms_repair_staging"], 3),
+        "ms_repair_compute": round(st_t["ms_repair_compute"], 3),
+        "repair_rows_staged": st_t["repair_rows_staged"],
+        "repair_fetches": st_t["repair_fetches"],
         "violations_gpu": st["violations_gpu"],
         "repair_regions": st["repair_regions"],
         "rounds": st["rounds"],

@@ -218,6 +218,16 @@ wtet_status wtet_device_free(wtet_ctx *ctx, void *d_ptr);
  * calibrated against. */
 wtet_status wtet_device_copy(wtet_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
+/* The library's stable radix sort (the point order of enable_sorting /
+ * enable_hilbert_sorting, which the reference sorts on the host,
+ * src/gpu/mod.rs:422-520): n (key, value) pairs in device memory, keys of
+ * key_bits 32 or 64 bits sorted on their low end_bit bits, equal keys in
+ * input order; inputs are read only and must not overlap the outputs.
+ * Synchronous. */
+wtet_status wtet_device_sort_pairs(wtet_ctx *ctx, const void *d_keys, const uint32_t *d_vals,
+                                   void *d_keys_out, uint32_t *d_vals_out, uint32_t n,
+                                   uint32_t key_bits, int end_bit);
+
 /* Points inserted in one round must lie in tets at least `ring`+1
  * face-steps apart (ring 0: one point per tet, as the reference's vote,
  * src/shaders/vote.wgsl:235-344; default 1).  A wider ring leaves fewer
@@ -242,6 +252,13 @@ wtet_status wtet_set_repair_rings(wtet_ctx *ctx, int rings);
  * has no limit.  The triangulation does not depend on it; the stats count
  * the reruns (repair_retries).  No counterpart in the reference. */
 wtet_status wtet_set_repair_region_cap(wtet_ctx *ctx, uint32_t tets);
+
+/* The audit of a call records at most `records` (>= 64, default 2^20)
+ * violated faces; when it finds more it runs again into a buffer as large
+ * as the count, so the host repair starts from every violated face.  No
+ * counterpart in the reference, whose gather clamps failed_verts to n
+ * (src/gpu/mod.rs:306-316). */
+wtet_status wtet_set_audit_records(wtet_ctx *ctx, uint32_t records);
 
 /* The flip iterations of a round run on the device without host reads
  * (counts, iteration tag and list parity live in device counters); they

@@ -461,10 +461,31 @@ def test_insertion_cap_without_phase2_leaves_points_out(oracle, ctx):
 
 def test_no_flipping_overflows_audit_records(oracle, ctx):
     """enable_flipping off: every non-Delaunay face the splits leave goes to
-    the host repair, about one per point.  At 1.15M points these are more
-    than the audit's first record buffer (2^20 entries) holds; the audit runs
-    again into a larger one and the repair still reaches the oracle's set."""
-    p = W.uniform(1_150_000, 84)
-    res = _check(oracle, ctx, p, enable_flipping=False)
-    assert res.stats["violations_gpu"] > (1 << 20), res.stats
-    assert res.stats["flip_iterations"] == 0
+    the host repair, about one per point.  With the first audit's record
+    buffer cut to 1,024 entries these overflow it; the audit runs again into
+    a larger one, the repair starts from every violated face and reaches the
+    oracle's set (a host result, a resident result and a batch)."""
+    p = W.uniform(20000, 84)
+    n = len(p)
+    ctx.set_audit_records(1024)
+    try:
+        res = _check(oracle, ctx, p, enable_flipping=False)
+        assert res.stats["violations_gpu"] > 1024, res.stats
+        assert res.stats["flip_iterations"] == 0
+        ref = oracle.canonical_real(res.tets, n)
+        d = ctx.upload(p)
+        try:
+            m, dt, _ = ctx.delaunay_resident(d, n, wtet_amd.Config(enable_flipping=False))
+            tets = ctx.download(dt, (m, 4), np.uint32)
+        finally:
+            ctx.free(d)
+        assert ctx.stats()["violations_gpu"] > 1024
+        assert np.array_equal(oracle.canonical_real(tets, n), ref)
+        q = W.clusters(3000, 85)
+        b = ctx.delaunay_batch([p, q], config=wtet_amd.Config(enable_flipping=False))
+        assert np.array_equal(oracle.canonical_real(b[0].tets, n), ref)
+        qn = oracle.normalize_points(q)[0]
+        assert np.array_equal(oracle.canonical_real(b[1].tets, len(q)),
+                              oracle.canonical_real(oracle.delaunay(qn), len(q)))
+    finally:
+        ctx.set_audit_records(1 << 20)

@@ -192,6 +192,8 @@ def lib():
             L.wtet_batch_result_free.argtypes = [ctypes.POINTER(_BatchResult)]
             L.wtet_get_stats.argtypes = [P, ctypes.POINTER(_Stats)]
             L.wtet_presize_features.argtypes = [P, P, P, u32, P]
+            L.wtet_device_sort_pairs.argtypes = [P, P, P, P, P, u32, u32, ctypes.c_int]
+            L.wtet_set_audit_records.argtypes = [P, u32]
             L.wtet_presize_predict.argtypes = [P, P, P, u32, P]
             L.wtet_set_presize_model.argtypes = [P, P]
             L.wtet_mlp_loss_grad.argtypes = [P, P, P, P, u32, P, P]
@@ -346,8 +348,25 @@ class Context:
         if lib().wtet_device_copy(self._h, ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes) != 0:
             raise RuntimeError(f"device copy failed: {self.error()}")
 
+    def sort_pairs(self, d_keys: int, d_vals: int, d_keys_out: int, d_vals_out: int, n: int,
+                   key_bits: int, end_bit: int):
+        """The library's stable radix sort on device arrays (synchronous)."""
+        st = lib().wtet_device_sort_pairs(self._h, ctypes.c_void_p(d_keys), ctypes.c_void_p(d_vals),
+                                          ctypes.c_void_p(d_keys_out), ctypes.c_void_p(d_vals_out),
+                                          int(n), int(key_bits), int(end_bit))
+        if st == 1:
+            raise ValueError(self.error())
+        if st != 0:
+            raise RuntimeError(f"device sort failed: {self.error()}")
+
     def free(self, ptr: int):
         lib().wtet_device_free(self._h, ctypes.c_void_p(ptr))
+
+    def set_audit_records(self, records: int):
+        """Violation records the first audit of a call keeps (default 2^20);
+        more violations run the audit again into a larger buffer."""
+        if lib().wtet_set_audit_records(self._h, int(records)) != 0:
+            raise ValueError("records must be at least 64")
 
     def set_insertion_ring(self, ring: int):
         """Insertion independence radius in face-steps (0, 1 or 2)."""

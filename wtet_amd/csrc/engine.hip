@@ -25,6 +25,7 @@
 #include "kernels.h"
 #include "mlp.h"
 #include "repair.h"
+#include "sort.h"
 
 using namespace wtet;
 
@@ -64,6 +65,7 @@ struct wtet_ctx {
   // neighbourhood of the host repair (sparse download)
   DevBuf nbr_mark, nbr_list, nbr_misc, nbr_pack;
   uint32_t items = 2;  // list entries per thread the persistent grids are sized for
+  uint32_t audit_records = 1u << 20;  // violation records the first audit of a call keeps
   bool use_graphs = true;
   struct GraphEntry {
     std::vector<uint64_t> key;
@@ -519,7 +521,7 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
   ensure(c, c->ctr, C_COUNT * 4);
   ensure(c, c->gstats, 64);
   ensure(c, c->isums, S_COUNT * 8);
-  ensure(c, c->rec, (1u << 20) * 4);
+  ensure(c, c->rec, (size_t)c->audit_records * 4);
   ensure(c, c->blk, ((size_t)nblocks(n) + 2) * 4);
   if (R.learned) {
     if (!c->presize_loaded) throw Fail{WTET_ERR_INVALID_ARGUMENT, "presize_mode 1 needs a model (wtet_set_presize_model)"};
@@ -547,8 +549,8 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
     // sort, gather into the vertex layout
     launch_prepare(d_xyz, n, CL, P<uint32_t>(c->mm6), P<float>(c->xyzn), P<uint64_t>(c->keys),
                    P<uint32_t>(c->ids), s);
-    sort_pairs(P<uint64_t>(c->keys), P<uint32_t>(c->ids), P<uint64_t>(c->keys2),
-               P<uint32_t>(c->ids2), n, R.end_bit, c->sort_tmp.p, c->sort_tmp.bytes, s);
+    HIP_CHECK(sort_pairs(P<uint64_t>(c->keys), P<uint32_t>(c->ids), P<uint64_t>(c->keys2),
+                         P<uint32_t>(c->ids2), n, R.end_bit, c->sort_tmp.p, c->sort_tmp.bytes, s));
     launch_gather_points(P<float>(c->xyzn), P<uint32_t>(c->ids2), P<uint64_t>(c->keys2), n, CL,
                          P<float4>(c->pts), s);
     // uninserted list: every point (minus redundant duplicates) in its
@@ -572,8 +574,9 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
   c->stats.duplicates = n - R.U;
   if (R.hilbert) {
     timed(c, K_PREP, [&] {
-      launch_stratify(P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), R.U, CL, P<uint64_t>(c->keys),
-                      P<uint64_t>(c->keys2), P<uint32_t>(c->ids2), c->sort_tmp.p, c->sort_tmp.bytes, s);
+      HIP_CHECK(launch_stratify(P<uint32_t>(c->uvert[0]), P<uint32_t>(c->utet[0]), R.U, CL,
+                                P<uint64_t>(c->keys), P<uint64_t>(c->keys2), P<uint32_t>(c->ids2),
+                                c->sort_tmp.p, c->sort_tmp.bytes, s));
     });
     HIP_CHECK(hipGetLastError());
   }
@@ -722,7 +725,7 @@ void dev_compact(wtet_ctx* c, Run& R, uint32_t T, bool out, bool keep_lists) {
     OB.out_adj = P<uint32_t>(c->fout);
   }
   Mesh M = mesh_of(c, R.V, R.mode);
-  timed(c, K_OUTPUT, [&] { launch_output(M, T, R.CL, OB, c->stream); });
+  timed(c, K_OUTPUT, [&] { HIP_CHECK(launch_output(M, T, R.CL, OB, c->stream)); });
   HIP_CHECK(hipGetLastError());
 }
 
@@ -764,7 +767,7 @@ void dev_output(wtet_ctx* c, Run& R, std::vector<uint32_t>& viol_rec, uint32_t& 
   uint32_t T = c->h_ctr[C_TETS];
   Mesh M = mesh_of(c, R.V, R.mode);
   launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, s);
-  const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
+  const uint32_t rec_cap = (uint32_t)std::min<size_t>(c->rec.bytes / 4, c->audit_records);
   timed(c, K_AUDIT, [&] { launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, R.st, P<uint32_t>(c->slow_t), s); });
   dev_compact(c, R, T, true, false);
   HIP_CHECK(hipEventRecord(c->ev1, s));
@@ -1006,8 +1009,8 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   return c->present[row] != 0;
 }
 
-void download_rows(wtet_ctx* c, uint32_t M, uint32_t left = 0) {
-  size_t cap = (size_t)M + M / 32 + 65536 + 8ull * left;
+void download_rows(wtet_ctx* c, uint32_t M, uint32_t left = 0, size_t min_cap = 0) {
+  size_t cap = std::max<size_t>((size_t)M + M / 32 + 65536 + 8ull * left, min_cap);
   size_t c1 = c->h_cap, c2 = c->h_cap;
   ensure_pinned(c->h_tets, c1, cap * 4);
   ensure_pinned(c->h_adj, c2, cap * 4);
@@ -1083,42 +1086,60 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
   }
-  if (nviol > 0 && cfg.enable_splaying) {
-    auto tc = std::chrono::steady_clock::now();
-    c->stats.ms_repair_staging = std::chrono::duration<double, std::milli>(tc - t1).count();
-    RepairReport rr;
-    wtet_status rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr, region_cap);
-    if (rs != WTET_OK && rr.needs_full && !rr.too_large) {
-      // the neighbourhood was too small: every row, and again from the start
-      download_rows(c, mout, left);
-      mb = MeshBuf{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
-      rr = RepairReport{};
+  // The repair and the insertion write into the staging rows.  The Delaunay
+  // mesh may need many more tets than the rows hold (a mesh from splits
+  // alone has about 3 per point, the Delaunay one about 6.7): when they run
+  // out of room, every row is downloaded again into staging twice as large
+  // and both start over.
+  std::vector<uint32_t> added = missing;
+  for (int attempt = 0;; ++attempt) {
+    wtet_status rs = WTET_OK;
+    std::string msg;
+    if (nviol > 0 && cfg.enable_splaying) {
+      auto tc = std::chrono::steady_clock::now();
+      c->stats.ms_repair_staging = std::chrono::duration<double, std::milli>(tc - t1).count();
+      RepairReport rr;
       rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr, region_cap);
-      c->stats.repair_full_downloads++;
-    }
-    c->stats.repair_max_region = rr.max_region;
-    if (rs != WTET_OK && rr.too_large) throw WiderRing{};
-    if (rs != WTET_OK) throw Fail{rs, "host repair failed: " + rr.message};
-    c->stats.repair_regions = rr.regions;
-    c->stats.repair_fetches = std::min(fetches, kMaxFetches);
-    c->stats.ms_repair_compute =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
-    repaired = true;
-  }
-  if (!missing.empty()) {
-    // phase 2 inserts what phase 1 left (src/lib.rs:227-307 rebuilds with
-    // every point); failed_verts lists those points
-    std::vector<uint32_t> added = missing;
-    if (cfg.enable_splaying) {
-      RepairReport ri;
-      wtet_status is = insert_points(c->h_xyzn, n, mb, missing, cfg.predicate_mode, &ri);
-      if (is != WTET_OK) throw Fail{is, "host point insertion failed: " + ri.message};
-      std::vector<uint32_t> still = missing_points(mb.tets, mb.count, n);
-      added.clear();
-      std::set_difference(missing.begin(), missing.end(), still.begin(), still.end(),
-                          std::back_inserter(added));
+      if (rs != WTET_OK && rr.needs_full && !rr.too_large) {
+        // the neighbourhood was too small: every row, and again from the start
+        download_rows(c, mout, left);
+        mb = MeshBuf{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
+        rr = RepairReport{};
+        rs = repair_delaunay(c->h_xyzn, n, mb, rec, cfg.predicate_mode, &rr, region_cap);
+        c->stats.repair_full_downloads++;
+      }
+      c->stats.repair_max_region = rr.max_region;
+      if (rs != WTET_OK && rr.too_large) throw WiderRing{};
+      c->stats.repair_regions = rr.regions;
+      c->stats.repair_fetches = std::min(fetches, kMaxFetches);
+      c->stats.ms_repair_compute =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
+      msg = "host repair failed: " + rr.message;
       repaired = true;
     }
+    if (rs == WTET_OK && !missing.empty() && cfg.enable_splaying) {
+      // phase 2 inserts what phase 1 left (src/lib.rs:227-307 rebuilds with
+      // every point); failed_verts lists those points
+      RepairReport ri;
+      rs = insert_points(c->h_xyzn, n, mb, missing, cfg.predicate_mode, &ri);
+      msg = "host point insertion failed: " + ri.message;
+      if (rs == WTET_OK) {
+        std::vector<uint32_t> still = missing_points(mb.tets, mb.count, n);
+        added.clear();
+        std::set_difference(missing.begin(), missing.end(), still.begin(), still.end(),
+                            std::back_inserter(added));
+      }
+      repaired = true;
+    }
+    if (rs == WTET_ERR_OUT_OF_MEMORY && attempt < 6) {
+      download_rows(c, mout, left, 2 * mb.cap);
+      mb = MeshBuf{c->h_tets, c->h_adj, mout, c->h_cap / 4, {}};
+      continue;
+    }
+    if (rs != WTET_OK) throw Fail{rs, msg};
+    break;
+  }
+  if (!missing.empty()) {
     std::vector<uint32_t> u;
     std::set_union(failed.begin(), failed.end(), added.begin(), added.end(), std::back_inserter(u));
     failed.swap(u);
@@ -1371,25 +1392,37 @@ void batch_impl(void* a_) {
         size_t cnt = cstart[cl + 1] - cstart[cl];
         size_t cap = cnt + cnt / 8 + 4096 + 8 * missing[cl].size();
         CloudRows& R = rows[cl];
-        R.tets.assign(c->h_tets + 4 * (size_t)hoff[cl], c->h_tets + 4 * (size_t)(hoff[cl] + cnt));
-        R.adj.assign(c->h_adj + 4 * (size_t)hoff[cl], c->h_adj + 4 * (size_t)(hoff[cl] + cnt));
-        R.tets.resize(4 * cap);
-        R.adj.resize(4 * cap);
-        MeshBuf mb{R.tets.data(), R.adj.data(), cnt, cap, {}};
         const float* xc = c->h_xyzn + 3 * (size_t)a.off[cl];
         const uint32_t nc = a.off[cl + 1] - a.off[cl];
-        if (!jb.viol.empty())
-          jb.st = repair_delaunay(xc, nc, mb, jb.viol, a.cfg.predicate_mode, &jb.rr);
-        if (jb.st == WTET_OK && !missing[cl].empty()) {
-          jb.st = insert_points(xc, nc, mb, missing[cl], a.cfg.predicate_mode, &jb.rr);
-          // failed_verts gets the points inserted here (not exact duplicates)
-          std::vector<uint32_t> still = missing_points(mb.tets, mb.count, nc), added;
-          std::set_difference(missing[cl].begin(), missing[cl].end(), still.begin(), still.end(),
-                              std::back_inserter(added));
-          missing[cl].swap(added);
+        // out of room (the Delaunay mesh may need far more tets than the
+        // rows hold): start over from the cloud's rows with twice the room
+        for (int attempt = 0;; ++attempt) {
+          R.tets.assign(c->h_tets + 4 * (size_t)hoff[cl], c->h_tets + 4 * (size_t)(hoff[cl] + cnt));
+          R.adj.assign(c->h_adj + 4 * (size_t)hoff[cl], c->h_adj + 4 * (size_t)(hoff[cl] + cnt));
+          R.tets.resize(4 * cap);
+          R.adj.resize(4 * cap);
+          MeshBuf mb{R.tets.data(), R.adj.data(), cnt, cap, {}};
+          jb.rr = RepairReport{};
+          jb.st = WTET_OK;
+          if (!jb.viol.empty())
+            jb.st = repair_delaunay(xc, nc, mb, jb.viol, a.cfg.predicate_mode, &jb.rr);
+          std::vector<uint32_t> added;
+          if (jb.st == WTET_OK && !missing[cl].empty()) {
+            jb.st = insert_points(xc, nc, mb, missing[cl], a.cfg.predicate_mode, &jb.rr);
+            // failed_verts gets the points inserted here (not exact duplicates)
+            std::vector<uint32_t> still = missing_points(mb.tets, mb.count, nc);
+            std::set_difference(missing[cl].begin(), missing[cl].end(), still.begin(), still.end(),
+                                std::back_inserter(added));
+          }
+          if (jb.st == WTET_ERR_OUT_OF_MEMORY && attempt < 6) {
+            cap *= 2;
+            continue;
+          }
+          if (jb.st == WTET_OK && !missing[cl].empty()) missing[cl].swap(added);
+          R.tets.resize(4 * mb.count);
+          R.adj.resize(4 * mb.count);
+          break;
         }
-        R.tets.resize(4 * mb.count);
-        R.adj.resize(4 * mb.count);
         R.repaired = true;
       }
     };
@@ -2077,6 +2110,50 @@ wtet_status wtet_device_copy(wtet_ctx* c, void* d_dst, const void* d_src, size_t
   return WTET_OK;
 }
 
+struct SortArgs {
+  wtet_ctx* c;
+  const void* keys;
+  const uint32_t* vals;
+  void* keys_out;
+  uint32_t* vals_out;
+  uint32_t n;
+  uint32_t key_bits;
+  int end_bit;
+};
+
+static void sort_impl(void* a_) {
+  SortArgs& a = *static_cast<SortArgs*>(a_);
+  wtet_ctx* c = a.c;
+  HIP_CHECK(hipSetDevice(c->device));
+  size_t tb = a.key_bits == 64 ? radix_temp_bytes_u64(a.n, a.end_bit) : radix_temp_bytes_u32(a.n, a.end_bit);
+  ensure(c, c->sort_tmp, std::max<size_t>(tb, 16));
+  if (a.key_bits == 64)
+    HIP_CHECK(radix_sort_u64(static_cast<const uint64_t*>(a.keys), a.vals, static_cast<uint64_t*>(a.keys_out),
+                             a.vals_out, a.n, a.end_bit, c->sort_tmp.p, c->sort_tmp.bytes, c->stream));
+  else
+    HIP_CHECK(radix_sort_u32(static_cast<const uint32_t*>(a.keys), a.vals, static_cast<uint32_t*>(a.keys_out),
+                             a.vals_out, a.n, a.end_bit, c->sort_tmp.p, c->sort_tmp.bytes, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+wtet_status wtet_device_sort_pairs(wtet_ctx* c, const void* d_keys, const uint32_t* d_vals, void* d_keys_out,
+                                   uint32_t* d_vals_out, uint32_t n, uint32_t key_bits, int end_bit) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  if ((key_bits != 32 && key_bits != 64) || end_bit < 0 || end_bit > (int)key_bits ||
+      (n && (!d_keys || !d_vals || !d_keys_out || !d_vals_out))) {
+    c->err = "wtet_device_sort_pairs: key_bits must be 32 or 64, end_bit in 0..key_bits";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (n == 0) return WTET_OK;
+  SortArgs a{c, d_keys, d_vals, d_keys_out, d_vals_out, n, key_bits, end_bit};
+  return run_guarded(c, sort_impl, &a);
+}
+
+wtet_status wtet_set_audit_records(wtet_ctx* c, uint32_t records) {
+  if (!c || records < 64) return WTET_ERR_INVALID_ARGUMENT;
+  c->audit_records = records;
+  return WTET_OK;
+}
 
 wtet_status wtet_set_flip_graphs(wtet_ctx* c, int on) {
   if (!c) return WTET_ERR_INVALID_ARGUMENT;
@@ -2441,7 +2518,7 @@ wtet_status wtet_stage_gather(wtet_ctx* c, wtet_stage_info* info) {
     uint32_t T = written_slots(c, S);
     Mesh M = mesh_of(c, S->R.V, S->R.mode);
     launch_set_ctr(M.ctr, C_VIOL, 0u, C_FAILREC, 0u, c->stream);
-    const uint32_t rec_cap = (uint32_t)(c->rec.bytes / 4);
+    const uint32_t rec_cap = (uint32_t)std::min<size_t>(c->rec.bytes / 4, c->audit_records);
     timed(c, K_AUDIT, [&] {
       launch_audit(M, T, P<uint32_t>(c->rec), rec_cap, S->R.st, P<uint32_t>(c->slow_t), c->stream);
     });

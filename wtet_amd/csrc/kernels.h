@@ -21,8 +21,9 @@ void launch_gather_points(const float* xyzn, const uint32_t* ids, const uint64_t
                           uint32_t n, const Clouds& K, float4* pts, hipStream_t s);
 // five start tets per cloud in slots 5c..5c+4
 void launch_start_tets(const Mesh& M, const Clouds& K, hipStream_t s);
-void sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp,
-                uint32_t n, int end_bit, void* temp, size_t temp_bytes, hipStream_t s);
+// stable radix sort of (key, id) pairs (sort.h): keys/ids in, keys_tmp/ids_tmp out
+hipError_t sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp,
+                      uint32_t n, int end_bit, void* temp, size_t temp_bytes, hipStream_t s);
 size_t sort_temp_bytes(uint32_t n, int end_bit);
 size_t sort_temp_bytes_u32(uint32_t n, int end_bit);
 // bits of an output sort key: clouds 0..C-1 and C for free slots
@@ -39,9 +40,9 @@ void launch_init_uninserted(const float4* pts, const uint64_t* keys, uint32_t n,
 
 // Hilbert mode: reorder the uninserted list into the reference's
 // stratified order, per cloud (scratch: key, key_tmp, val_tmp of U entries)
-void launch_stratify(uint32_t* uvert, uint32_t* utet, uint32_t U, const Clouds& K, uint64_t* key,
-                     uint64_t* key_tmp, uint32_t* val_tmp, void* temp, size_t temp_bytes,
-                     hipStream_t s);
+hipError_t launch_stratify(uint32_t* uvert, uint32_t* utet, uint32_t U, const Clouds& K,
+                           uint64_t* key, uint64_t* key_tmp, uint32_t* val_tmp, void* temp,
+                           size_t temp_bytes, hipStream_t s);
 
 // rounds
 void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
@@ -109,7 +110,7 @@ struct OutputBufs {
   uint32_t* out_tets;  // rows * 4 (null: only slot2pos / cstart)
   uint32_t* out_adj;
 };
-void launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s);
+hipError_t launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s);
 
 // violation records (slot << 2 | face) -> output indices
 void launch_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx, hipStream_t s);

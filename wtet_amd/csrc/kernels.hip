@@ -7,10 +7,10 @@
 #include <algorithm>
 #include <cstring>
 
-#include <rocprim/rocprim.hpp>
 
 #include "kernels.h"
 #include "prep.h"
+#include "sort.h"
 
 namespace wtet {
 
@@ -218,28 +218,18 @@ void launch_start_tets(const Mesh& M, const Clouds& K, hipStream_t s) {
   k_start_tets<<<nblocks(5u * K.C), kBlock, 0, s>>>(M, S, K);
 }
 
-size_t sort_temp_bytes(uint32_t n, int end_bit) {
-  size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs((void*)nullptr, bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
-  return bytes;
+size_t sort_temp_bytes(uint32_t n, int end_bit) { return radix_temp_bytes_u64(n, end_bit); }
+
+hipError_t sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp, uint32_t n,
+                      int end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
+  return radix_sort_u64(keys, ids, keys_tmp, ids_tmp, n, end_bit, temp, temp_bytes, s);
 }
 
-void sort_pairs(uint64_t* keys, uint32_t* ids, uint64_t* keys_tmp, uint32_t* ids_tmp, uint32_t n,
-                int end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
-  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, ids, ids_tmp, n, 0, end_bit, s);
-}
+size_t sort_temp_bytes_u32(uint32_t n, int end_bit) { return radix_temp_bytes_u32(n, end_bit); }
 
-size_t sort_temp_bytes_u32(uint32_t n, int end_bit) {
-  size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs((void*)nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr, n, 0, end_bit);
-  return bytes;
-}
-
-void sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
-                    uint32_t n, int end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
-  (void)rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_tmp, vals, vals_tmp, n, 0, end_bit, s);
+hipError_t sort_pairs_u32(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
+                          uint32_t n, int end_bit, void* temp, size_t temp_bytes, hipStream_t s) {
+  return radix_sort_u32(keys, vals, keys_tmp, vals_tmp, n, end_bit, temp, temp_bytes, s);
 }
 
 // sorted position j of cloud c: kept unless it is a redundant duplicate;
@@ -308,14 +298,16 @@ __global__ void k_list_from_sorted(const uint32_t* sv, uint32_t U, Clouds K, uin
   utet[r] = 5u * (K.C > 1 ? cloud_of_vertex(K.off, K.C, v) : 0u);
 }
 
-void launch_stratify(uint32_t* uvert, uint32_t* utet, uint32_t U, const Clouds& K, uint64_t* key,
-                     uint64_t* key_tmp, uint32_t* val_tmp, void* temp, size_t temp_bytes,
-                     hipStream_t s) {
-  if (!U) return;
+hipError_t launch_stratify(uint32_t* uvert, uint32_t* utet, uint32_t U, const Clouds& K, uint64_t* key,
+                           uint64_t* key_tmp, uint32_t* val_tmp, void* temp, size_t temp_bytes,
+                           hipStream_t s) {
+  if (!U) return hipSuccess;
   k_stratify_keys<<<nblocks(U), kBlock, 0, s>>>(uvert, U, K, key);
   int bits = 32 + (K.C > 1 ? cloud_key_bits(K.C) : 0);
-  (void)rocprim::radix_sort_pairs(temp, temp_bytes, key, key_tmp, uvert, val_tmp, U, 0, bits, s);
+  hipError_t e = radix_sort_u64(key, uvert, key_tmp, val_tmp, U, bits, temp, temp_bytes, s);
+  if (e != hipSuccess) return e;
   k_list_from_sorted<<<nblocks(U), kBlock, 0, s>>>(val_tmp, U, K, uvert, utet);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -847,7 +839,7 @@ __global__ void k_output_rows(Mesh M, uint32_t T, const uint32_t* olist, const u
   output_body(M, j, olist, okey, cstart, slot2pos, K, out_tets, out_adj);
 }
 
-void launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s) {
+hipError_t launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s) {
   uint32_t nb = nblocks(T);
   k_alive_count<<<nb, kBlock, 0, s>>>(M, T, B.blk);
   k_scan_blocks<<<1, 1024, 0, s>>>(B.blk, nb, M.ctr, 1);
@@ -860,7 +852,9 @@ void launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs&
     olist = B.olist;
   } else {
     k_slot_keys<<<nb, kBlock, 0, s>>>(M, T, K, B.okey, B.olist);
-    sort_pairs_u32(B.okey, B.olist, B.okey2, B.olist2, T, cloud_key_bits(K.C), B.temp, B.temp_bytes, s);
+    hipError_t e = sort_pairs_u32(B.okey, B.olist, B.okey2, B.olist2, T, cloud_key_bits(K.C), B.temp,
+                                  B.temp_bytes, s);
+    if (e != hipSuccess) return e;
     k_row_positions<<<nb, kBlock, 0, s>>>(B.okey2, B.olist2, T, K.C, B.slot2pos, B.cstart);
     k_cstart_tail<<<1, 64, 0, s>>>(B.cstart, K.C, M.ctr);
     olist = B.olist2;
@@ -869,6 +863,7 @@ void launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs&
   if (B.out_tets)
     k_output_rows<<<nb, kBlock, 0, s>>>(M, T, olist, okey, B.cstart, B.slot2pos, K, B.out_tets,
                                         B.out_adj);
+  return hipGetLastError();
 }
 
 __global__ void k_map_records(uint32_t* rec, uint32_t n, const uint32_t* slot2idx) {

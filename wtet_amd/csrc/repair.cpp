@@ -834,15 +834,29 @@ extern "C" wtet_status wtet_repair(const float* xyzn, uint32_t n, uint32_t** tet
     // input ids no tet uses are inserted after the repair (phase 2
     // rebuilds with every point, src/phase2.rs:383-606)
     std::vector<uint32_t> missing = missing_points(tv.data(), M, n);
-    const size_t room = 65536 + 8 * missing.size();
+    size_t room = 65536 + 8 * missing.size();
     RepairReport rr;
     wtet_status s = WTET_OK;
-    tv.resize(tv.size() + 4 * room);
-    ta.resize(ta.size() + 4 * room);
-    MeshBuf mb{tv.data(), ta.data(), M, M + room, {}};
-    if (!viol.empty()) s = repair_delaunay(xyzn, n, mb, viol, mode, &rr);
-    if (s != WTET_OK) return s;
-    if (!missing.empty()) s = insert_points(xyzn, n, mb, missing, mode, &rr);
+    const std::vector<uint32_t> tv0 = tv, ta0 = ta;
+    MeshBuf mb{nullptr, nullptr, 0, 0, {}};
+    // out of room (the Delaunay mesh may hold far more tets than the input):
+    // start over from the input with twice the room
+    for (int attempt = 0;; ++attempt) {
+      tv = tv0;
+      ta = ta0;
+      tv.resize(tv.size() + 4 * room);
+      ta.resize(ta.size() + 4 * room);
+      mb = MeshBuf{tv.data(), ta.data(), M, M + room, {}};
+      rr = RepairReport{};
+      s = WTET_OK;
+      if (!viol.empty()) s = repair_delaunay(xyzn, n, mb, viol, mode, &rr);
+      if (s == WTET_OK && !missing.empty()) s = insert_points(xyzn, n, mb, missing, mode, &rr);
+      if (s == WTET_ERR_OUT_OF_MEMORY && attempt < 6) {
+        room = 2 * (M + room);
+        continue;
+      }
+      break;
+    }
     if (s != WTET_OK) return s;
     tv.resize(4 * mb.count);
     ta.resize(4 * mb.count);

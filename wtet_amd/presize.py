@@ -84,35 +84,58 @@ def load_model(path: str | None = None) -> np.ndarray:
 FAMILIES = ("uniform", "clusters", "anisotropic", "sphere", "shell", "lattice", "flat", "mixed")
 
 
-def sample_cloud(rng: np.random.Generator, n: int):
-    """A synthetic cloud of about n points from a random family."""
+def draw_spec(rng: np.random.Generator, n: int) -> tuple:
+    """The random choices of one synthetic cloud of about n points (family,
+    seed and the family's parameters), drawn in a fixed order from rng."""
     fam = FAMILIES[int(rng.integers(len(FAMILIES)))]
     seed = int(rng.integers(1 << 31))
+    extra = ()
+    if fam == "anisotropic":
+        extra = (int(rng.integers(1, 33)), float(rng.uniform(0, 0.3)))
+    elif fam == "shell":
+        extra = (float(10 ** rng.uniform(-5, -1)),)
+    elif fam == "lattice":
+        extra = (bool(rng.random() < 0.5),)
+    elif fam == "flat":
+        extra = (float(10 ** rng.uniform(-4, -1)),)
+    return fam, n, seed, extra
+
+
+def make_cloud(spec: tuple):
+    """(family, points) of a spec from draw_spec; no shared random state, so
+    specs can be generated on several threads."""
+    fam, n, seed, extra = spec
     if fam == "uniform":
         p = W.uniform(n, seed)
     elif fam == "clusters":
         p = W.clusters(n, seed)
     elif fam == "anisotropic":
-        p = W.anisotropic(n, seed, k=int(rng.integers(1, 33)), uniform_share=float(rng.uniform(0, 0.3)))
+        p = W.anisotropic(n, seed, k=extra[0], uniform_share=extra[1])
     elif fam == "sphere":
         p = W.sphere(n, seed)
     elif fam == "shell":
         # thin shells above ~1e5 points send the host repair into whole-mesh
         # regions (minutes); the family is kept at sizes that finish quickly
-        p = W.shell(min(n, 100_000), seed, thickness=float(10 ** rng.uniform(-5, -1)))
+        p = W.shell(min(n, 100_000), seed, thickness=extra[0])
     elif fam == "lattice":
         k = max(2, int(round(n ** (1 / 3))))
         p = W.lattice(k)
-        if rng.random() < 0.5:
-            p = (p + rng.standard_normal(p.shape).astype(np.float32) * np.float32(1e-3 / k)).astype(np.float32)
+        if extra[0]:
+            jit = np.random.default_rng(seed).standard_normal(p.shape).astype(np.float32)
+            p = (p + jit * np.float32(1e-3 / k)).astype(np.float32)
     elif fam == "flat":
         p = W.uniform(n, seed)
-        p[:, 2] *= np.float32(10 ** rng.uniform(-4, -1))
+        p[:, 2] *= np.float32(extra[0])
     else:
         a = W.uniform(n // 2, seed)
         b = W.clusters(n - n // 2, seed + 1)
         p = np.concatenate([a, b])
     return fam, np.ascontiguousarray(p, np.float32)
+
+
+def sample_cloud(rng: np.random.Generator, n: int):
+    """A synthetic cloud of about n points from a random family."""
+    return make_cloud(draw_spec(rng, n))
 
 
 def targets_of(stats: dict, n: int) -> np.ndarray:
@@ -134,28 +157,37 @@ def collect(ctx: "_core.Context", runs: int, seed: int, nmin: int = 100, nmax: i
     N = np.zeros(runs, np.int64)
     fams = []
     cfg = _core.Config(presize="fixed")
+    specs = [draw_spec(rng, int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))) for _ in range(runs)]
     t0 = time.time()
     done = runs
-    for r in range(runs):
-        if budget_s and time.time() - t0 > budget_s:
-            done = r
-            break
-        n = int(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
-        fam, p = sample_cloud(rng, n)
-        d = ctx.upload(p)
-        try:
-            X[r] = ctx.presize_features(d, [0, len(p)])[0]
-            ctx.delaunay_resident(d, len(p), cfg)
-            Y[r] = targets_of(ctx.stats(), len(p))
-        finally:
-            ctx.free(d)
-        N[r] = len(p)
-        fams.append(fam)
-        if log:
-            log(f"collect: {r + 1}/{runs} runs, {time.time() - t0:.1f} s ({fam}, n={len(p)}, "
-                f"stuck={int(np.expm1(Y[r, 2]))})")
-        if save and (r + 1) % 50 == 0:
-            save(X[:r + 1], Y[:r + 1], N[:r + 1], np.array(fams))
+    # the host generates the next clouds while the GPU triangulates
+    from concurrent.futures import ThreadPoolExecutor
+    ahead = 6
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        futs = {r: pool.submit(make_cloud, specs[r]) for r in range(min(ahead, runs))}
+        for r in range(runs):
+            if budget_s and time.time() - t0 > budget_s:
+                done = r
+                break
+            fam, p = futs.pop(r).result()
+            if r + ahead < runs:
+                futs[r + ahead] = pool.submit(make_cloud, specs[r + ahead])
+            d = ctx.upload(p)
+            try:
+                X[r] = ctx.presize_features(d, [0, len(p)])[0]
+                ctx.delaunay_resident(d, len(p), cfg)
+                Y[r] = targets_of(ctx.stats(), len(p))
+            finally:
+                ctx.free(d)
+            N[r] = len(p)
+            fams.append(fam)
+            if log and (r < 20 or (r + 1) % 100 == 0):
+                log(f"collect: {r + 1}/{runs} runs, {time.time() - t0:.1f} s ({fam}, n={len(p)}, "
+                    f"stuck={int(np.expm1(Y[r, 2]))})")
+            if save and (r + 1) % 500 == 0:
+                save(X[:r + 1], Y[:r + 1], N[:r + 1], np.array(fams))
+        for f in futs.values():
+            f.cancel()
     return X[:done], Y[:done], N[:done], np.array(fams[:done])
 
 
