@@ -296,6 +296,19 @@ wtet_status wtet_autotune(wtet_ctx *ctx, const float *d_xyz, const uint64_t *off
  * wtet_delaunay_batch) into feat (host, num_clouds * 522) */
 wtet_status wtet_presize_features(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
                                   uint32_t num_clouds, float *feat);
+/* Same features, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  No counterpart in the
+ * reference.  It allocates nothing: the offsets, per-cloud bounding boxes,
+ * histogram accumulators and feature staging live in one static device
+ * array of WTET_NOALLOC_SCRATCH_BYTES per GPU (resolved by wtet_create,
+ * held until the process ends), so at most WTET_NOALLOC_MAX_CLOUDS clouds
+ * per call (more: WTET_ERR_INVALID_ARGUMENT).  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (two contexts, two threads)
+ * race.  Synchronous. */
+#define WTET_NOALLOC_MAX_CLOUDS 2048
+#define WTET_NOALLOC_SCRATCH_BYTES 8675840
+wtet_status wtet_presize_features_noalloc(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
+                                          uint32_t num_clouds, float *feat);
 /* GPU inference from host arrays: pred (num_clouds * 3) in log space,
  * margin included */
 wtet_status wtet_presize_predict(wtet_ctx *ctx, const float *params, const float *feat,

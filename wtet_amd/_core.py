@@ -192,6 +192,7 @@ def lib():
             L.wtet_batch_result_free.argtypes = [ctypes.POINTER(_BatchResult)]
             L.wtet_get_stats.argtypes = [P, ctypes.POINTER(_Stats)]
             L.wtet_presize_features.argtypes = [P, P, P, u32, P]
+            L.wtet_presize_features_noalloc.argtypes = [P, P, P, u32, P]
             L.wtet_device_sort_pairs.argtypes = [P, P, P, P, P, u32, u32, ctypes.c_int]
             L.wtet_set_audit_records.argtypes = [P, u32]
             L.wtet_presize_predict.argtypes = [P, P, P, u32, P]
@@ -289,6 +290,20 @@ class Context:
         st = lib().wtet_presize_features(self._h, ctypes.c_void_p(ptr), _p(off), C, _p(out))
         if st != 0:
             raise RuntimeError(f"wtet_presize_features failed: {self.error()}")
+        return out
+
+    def presize_features_noalloc(self, ptr: int, offsets) -> np.ndarray:
+        """presize_features without any device allocation: the library's
+        static per-GPU scratch array (at most NOALLOC_MAX_CLOUDS clouds per
+        call; calls that overlap on one GPU race)."""
+        off = _check_offsets(offsets)
+        C = len(off) - 1
+        out = np.empty((C, PRESIZE_FEATURES), np.float32)
+        st = lib().wtet_presize_features_noalloc(self._h, ctypes.c_void_p(ptr), _p(off), C, _p(out))
+        if st == 1:
+            raise ValueError(f"wtet_presize_features_noalloc: {self.error()}")
+        if st != 0:
+            raise RuntimeError(f"wtet_presize_features_noalloc failed: {self.error()}")
         return out
 
     def presize_predict(self, params: np.ndarray, feat: np.ndarray) -> np.ndarray:

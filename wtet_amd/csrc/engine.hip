@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <iterator>
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include <chrono>
 #include <cmath>
@@ -1542,6 +1543,8 @@ void batch_impl(void* a_) {
 
 extern "C" {
 
+static hipError_t resolve_scratch(int device);
+
 wtet_status wtet_create(int device, wtet_ctx** out) {
   if (!out) return WTET_ERR_INVALID_ARGUMENT;
   *out = nullptr;
@@ -1555,7 +1558,8 @@ wtet_status wtet_create(int device, wtet_ctx** out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipHostMalloc((void**)&c->h_ctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess ||
-      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      resolve_scratch(device) != hipSuccess) {
     (void)hipGetLastError();
     delete c;
     return WTET_ERR_DEVICE;
@@ -1856,6 +1860,91 @@ static void features_impl(void* a_) {
   HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------------------
+// wtet_presize_features_noalloc: the same kernels with every buffer beyond
+// the caller's points and features carved from one static device array.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoAllocMaxClouds = 2048;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaOffBytes = align256(((size_t)kNoAllocMaxClouds + 1) * 4);
+constexpr size_t kNaMmBytes = align256((size_t)kNoAllocMaxClouds * 24);
+constexpr size_t kNaAccBytes = align256((size_t)kNoAllocMaxClouds * (kHistBins / 2 + 9) * 8 + 64);
+constexpr size_t kNaFeatBytes = align256((size_t)kNoAllocMaxClouds * kFeat * 4);
+constexpr size_t kNoAllocScratchBytes = kNaOffBytes + kNaMmBytes + kNaAccBytes + kNaFeatBytes;
+static_assert(kNoAllocScratchBytes == WTET_NOALLOC_SCRATCH_BYTES, "scratch size in wtet.h");
+static_assert(kNoAllocMaxClouds == WTET_NOALLOC_MAX_CLOUDS, "cloud limit in wtet.h");
+
+// One copy per GPU, held until the process ends.  In host code this name is
+// a host-side stand-in, never a device address: kernels and copies use the
+// per-device pointer that hipGetSymbolAddress returns.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: resolves this GPU's copy of the array once.
+static hipError_t resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = ptr;
+  return hipSuccess;
+}
+
+struct FeatNoAllocArgs {
+  wtet_ctx* c;
+  const float* d_xyz;
+  const uint64_t* offsets;
+  uint32_t C;
+  float* feat;
+};
+
+static void features_noalloc_impl(void* a_) {
+  FeatNoAllocArgs& a = *static_cast<FeatNoAllocArgs*>(a_);
+  wtet_ctx* c = a.c;
+  HIP_CHECK(hipSetDevice(c->device));
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = static_cast<unsigned char*>(g_scratch_ptr[c->device]);
+  }
+  if (!base) throw Fail{WTET_ERR_DEVICE, "no-alloc scratch was not resolved for this device"};
+  uint32_t* doff = reinterpret_cast<uint32_t*>(base);
+  uint32_t* mm = reinterpret_cast<uint32_t*>(base + kNaOffBytes);
+  double* acc = reinterpret_cast<double*>(base + kNaOffBytes + kNaMmBytes);
+  float* feat = reinterpret_cast<float*>(base + kNaOffBytes + kNaMmBytes + kNaAccBytes);
+  const uint32_t C = a.C;
+  hipStream_t s = c->stream;
+  // the cloud offsets as u32 words, from a small stack buffer in chunks;
+  // every chunk's copy completes before the buffer is reused
+  uint32_t chunk[64];
+  for (uint32_t i = 0; i <= C; i += 64) {
+    uint32_t k = std::min<uint32_t>(64, C + 1 - i);
+    for (uint32_t j = 0; j < k; ++j) chunk[j] = (uint32_t)a.offsets[i + j];
+    HIP_CHECK(hipMemcpyAsync(doff + i, chunk, (size_t)k * 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+  }
+  Clouds CL;
+  CL.off = doff;
+  CL.C = C;
+  CL.V = (uint32_t)a.offsets[C] + 4 * C;
+  int kbits, cshift, end_bit;
+  key_layout(C, false, &kbits, &cshift, &end_bit);
+  CL.kbits = kbits;
+  CL.cshift = cshift;
+  CL.hilbert = 0;
+  const uint32_t n = (uint32_t)a.offsets[C];
+  launch_minmax(a.d_xyz, n, CL, mm, s);
+  launch_features(a.d_xyz, n, doff, C, mm, acc, feat, s);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(a.feat, feat, (size_t)C * kFeat * 4, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
 struct PredictArgs {
   wtet_ctx* c;
   const float* params;
@@ -2043,6 +2132,28 @@ wtet_status wtet_presize_features(wtet_ctx* c, const float* d_xyz, const uint64_
     if (offsets[i + 1] <= offsets[i]) return WTET_ERR_INVALID_ARGUMENT;
   FeatArgs a{c, d_xyz, offsets32(offsets, num_clouds), feat};
   return run_guarded(c, features_impl, &a);
+}
+
+wtet_status wtet_presize_features_noalloc(wtet_ctx* c, const float* d_xyz, const uint64_t* offsets,
+                                          uint32_t num_clouds, float* feat) {
+  if (!c) return WTET_ERR_INVALID_ARGUMENT;
+  if (!d_xyz || !offsets || !feat || num_clouds == 0 || offsets[0] != 0 ||
+      offsets[num_clouds] > (1ull << 26)) {
+    c->err = "wtet_presize_features_noalloc: null pointer, no cloud, offsets[0] != 0 or more than 2^26 points";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  if (num_clouds > kNoAllocMaxClouds) {
+    c->err = "wtet_presize_features_noalloc: at most " + std::to_string(kNoAllocMaxClouds) +
+             " clouds per call (got " + std::to_string(num_clouds) + ")";
+    return WTET_ERR_INVALID_ARGUMENT;
+  }
+  for (uint32_t i = 0; i < num_clouds; ++i)
+    if (offsets[i + 1] <= offsets[i]) {
+      c->err = "wtet_presize_features_noalloc: every cloud needs at least one point";
+      return WTET_ERR_INVALID_ARGUMENT;
+    }
+  FeatNoAllocArgs a{c, d_xyz, offsets, num_clouds, feat};
+  return run_guarded(c, features_noalloc_impl, &a);
 }
 
 wtet_status wtet_presize_predict(wtet_ctx* c, const float* params, const float* feat,

@@ -12,6 +12,9 @@ constexpr int kBlock = 256;  // 4 wavefronts of 64
 
 inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
+// per-cloud bounding box only (the first kernels of launch_prepare): mm
+// gets 6 order-preserving words per cloud (min xyz, max xyz)
+void launch_minmax(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, hipStream_t s);
 // point preparation for one or many clouds (layout: prep.h):
 // per-cloud normalization (src/lib.rs:313-337) and (cloud, Morton) keys
 void launch_prepare(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, float* xyzn,

@@ -197,13 +197,17 @@ __global__ void __launch_bounds__(kBlock) k_minmax_single(const float* xyz, uint
   }
 }
 
-void launch_prepare(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, float* xyzn,
-                    uint64_t* keys, uint32_t* ids, hipStream_t s) {
+void launch_minmax(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, hipStream_t s) {
   k_minmax_init<<<nblocks(6u * K.C), kBlock, 0, s>>>(mm, K.C);
   if (K.C <= 1)
     k_minmax_single<<<std::max(1u, std::min(nblocks(n) / 16u, 1024u)), kBlock, 0, s>>>(xyz, n, mm);
   else
     k_minmax<<<nblocks(n), kBlock, 0, s>>>(xyz, n, K, mm);
+}
+
+void launch_prepare(const float* xyz, uint32_t n, const Clouds& K, uint32_t* mm, float* xyzn,
+                    uint64_t* keys, uint32_t* ids, hipStream_t s) {
+  launch_minmax(xyz, n, K, mm, s);
   k_normalize<<<nblocks(n), kBlock, 0, s>>>(xyz, n, K, mm, xyzn, keys, ids);
 }
 
