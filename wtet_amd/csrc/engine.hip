@@ -51,7 +51,7 @@ struct wtet_ctx {
   // point preparation
   DevBuf xyz_in, xyzn, keys, ids, keys2, ids2, sort_tmp, mm6, pts, doff;
   // mesh (per tet slot)
-  DevBuf tet, adj_t, tinfo, tfwd, tvote, fvote, tsplit, tpoint, tmsg, fit, fround;
+  DevBuf tet, adj_t, tinfo, tfwd, tvote, fvote, tsplit, tpoint, tmsg, fstamp;
   // lists
   DevBuf uvert[2], utet[2], wlist, act[2], prop, fout, blk, slot2idx, rec;
   // output grouping for many clouds
@@ -115,7 +115,7 @@ struct wtet_ctx {
   wtet_ctx() {
     DevBuf* l[] = {&xyz_in, &xyzn,   &keys,   &ids,   &keys2,  &ids2,     &sort_tmp, &mm6,
                    &pts,    &doff,   &tet,    &adj_t,  &tinfo,  &tfwd,  &tvote,    &fvote,    &tsplit,  &tpoint,
-                   &tmsg,   &fit,    &fround, &uvert[0], &uvert[1], &utet[0], &utet[1],
+                   &tmsg,   &fstamp, &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
                    &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
                    &presize_params, &feat_acc, &feat, &pred, &isums, &slow_t, &slow_p,
@@ -233,8 +233,7 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
   ensure(c, c->tsplit, cap * 8, keep);
   ensure(c, c->tpoint, cap * 4, keep);
   ensure(c, c->tmsg, cap * sizeof(FlipMsg), keep);
-  ensure(c, c->fit, cap * 4, keep);
-  ensure(c, c->fround, cap * 4, keep);
+  ensure(c, c->fstamp, cap * 8, keep);
   ensure(c, c->act[0], cap * 4, keep);
   ensure(c, c->act[1], cap * 4, keep);
   ensure(c, c->slow_t, cap * 4, false);
@@ -300,8 +299,7 @@ Mesh mesh_of(wtet_ctx* c, uint32_t inf, int mode) {
   M.tsplit = P<uint2>(c->tsplit);
   M.tpoint = P<uint32_t>(c->tpoint);
   M.tmsg = P<FlipMsg>(c->tmsg);
-  M.fit = P<uint32_t>(c->fit);
-  M.fround = P<uint32_t>(c->fround);
+  M.fstamp = P<uint64_t>(c->fstamp);
   M.ctr = P<uint32_t>(c->ctr);
   M.inf = inf;
   M.mode = mode;
@@ -394,7 +392,7 @@ void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, ui
       (uint64_t)(uintptr_t)M.adj,   (uint64_t)(uintptr_t)M.tinfo,  (uint64_t)(uintptr_t)M.tfwd,
       (uint64_t)(uintptr_t)M.fvote, (uint64_t)(uintptr_t)M.tsplit, (uint64_t)(uintptr_t)M.tmsg,
       (uint64_t)(uintptr_t)M.tpoint,
-      (uint64_t)(uintptr_t)M.ctr,   (uint64_t)(uintptr_t)M.fit,    (uint64_t)(uintptr_t)M.fround,
+      (uint64_t)(uintptr_t)M.ctr,   (uint64_t)(uintptr_t)M.fstamp,
       (uint64_t)M.inf,              (uint64_t)M.mode,              (uint64_t)(uintptr_t)L.act[0],
       (uint64_t)(uintptr_t)L.act[1], (uint64_t)(uintptr_t)L.prop,  (uint64_t)(uintptr_t)L.out,
       (uint64_t)(uintptr_t)L.slow,  (uint64_t)(uintptr_t)st,       (uint64_t)(uintptr_t)c->isums.p,

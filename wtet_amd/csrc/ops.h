@@ -14,34 +14,36 @@ namespace wtet {
 
 // device-side counters (Mesh::ctr)
 enum Ctr : int {
-  C_TETS = 0,      // next free tet slot
-  C_ACT_NEXT = 1,  // length of the next active list
-  C_FLIPS = 2,     // winning flips this iteration
-  C_WALK_FAIL = 3, // relocation walks that hit the step limit
-  C_VIOL = 4,      // non-Delaunay faces found by the audit
-  C_WIN = 5,       // insertions this round
-  C_BASE = 6,      // first slot reserved for this round's splits
-  C_ALIVE = 7,     // alive tets (output compaction)
-  C_FAILREC = 8,   // recorded violation entries
-  C_BADTET = 9,    // points found in a tet touching infinity (must stay 0)
-  C_NPTS = 10,     // points to insert (input minus exact duplicates)
-  C_DUPSCAN = 11,  // duplicate scans that hit their length cap
-  C_ULEN = 12,     // uninserted list length at the start of the round
-  C_SURV = 13,     // points left after this round's insertions
-  // device-driven flip iterations (the host sets C_ROUND and C_CAP)
-  C_A = 14,        // length of the current active list
-  C_IT = 15,       // flip iteration tag (fvote keys, fit stamps)
-  C_PAR = 16,      // which of the two active lists is current
-  C_RUN = 17,      // 1 while this iteration runs
-  C_STOP = 18,     // set when an iteration could overflow the tet slots
-  C_ROUND = 19,    // insertion round (fround stamps)
-  C_CAP = 20,      // tet slot capacity
-  C_PEAKA = 21,    // longest active list seen
-  C_SLOW_T = 22,   // tets deferred by the lean flip check this iteration
-  C_SLOW_P = 23,   // points deferred by the lean point kernels
-  C_SLOW_SUM = 24, // tets deferred by the lean flip check, summed over iterations
-  C_TETS_IT = 25,  // C_TETS when the current flip iteration began
-  C_COUNT = 32
+  // read by every thread of the flip kernels, written between kernels: the
+  // first line.  Counters that take atomics at volume sit 1 KB apart, away
+  // from it and from each other.
+  C_A = 0,         // length of the current active list
+  C_IT = 1,        // flip iteration tag (fvote keys, flip stamps)
+  C_PAR = 2,       // which of the two active lists is current
+  C_RUN = 3,       // 1 while this iteration runs
+  C_STOP = 4,      // set when an iteration could overflow the tet slots
+  C_ROUND = 5,     // insertion round (flip stamps)
+  C_CAP = 6,       // tet slot capacity
+  C_PEAKA = 7,     // longest active list seen
+  C_TETS_IT = 8,   // C_TETS when the current flip iteration began
+  C_SLOW_SUM = 9,  // tets deferred by the lean flip check, summed over iterations
+  C_WALK_FAIL = 10, // relocation walks that hit the step limit
+  C_WIN = 11,      // insertions this round
+  C_BASE = 12,     // first slot reserved for this round's splits
+  C_ALIVE = 13,    // alive tets (output compaction)
+  C_BADTET = 14,   // points found in a tet touching infinity (must stay 0)
+  C_NPTS = 15,     // points to insert (input minus exact duplicates)
+  C_DUPSCAN = 16,  // duplicate scans that hit their length cap
+  C_ULEN = 17,     // uninserted list length at the start of the round
+  C_SURV = 18,     // points left after this round's insertions
+  C_TETS = 256,      // next free tet slot
+  C_ACT_NEXT = 512,  // length of the next active list
+  C_FLIPS = 768,     // winning flips this iteration
+  C_SLOW_T = 1024,   // tets deferred by the lean flip check this iteration
+  C_SLOW_P = 1280,   // points deferred by the lean point kernels
+  C_VIOL = 1536,     // non-Delaunay faces found by the audit
+  C_FAILREC = 1537,  // recorded violation entries
+  C_COUNT = 1792
 };
 
 // per-call sums over flip iterations (u64, device)
@@ -93,8 +95,7 @@ struct Mesh {
   uint2* tsplit;      // (round, first new slot) for tets split this round
   uint32_t* tpoint;   // the point inserted into a tet split this round
   FlipMsg* tmsg;      // per slot destroyed by a flip this iteration
-  uint32_t* fit;      // flip iteration that destroyed the tet held in the slot
-  uint32_t* fround;   // round of the last flip that touched the slot
+  uint64_t* fstamp;   // the last flip that wrote the slot: round << 32 | flip iteration
   uint32_t* ctr;      // Ctr counters
   uint32_t inf;       // id of the infinity vertex
   int mode;           // PredMode
@@ -115,8 +116,8 @@ WT_HD void set_adj(const Mesh& M, uint32_t t, uint4 n, uint4 ov) {
   M.adj[t].hi = make_uint4(n.z, ov.z, n.w, ov.w);
 }
 WT_HD uint32_t& INFO(const Mesh& M, uint32_t t) { return M.tinfo[t]; }
-WT_HD uint32_t& FIT(const Mesh& M, uint32_t t) { return M.fit[t]; }
-WT_HD uint32_t& FROUND(const Mesh& M, uint32_t t) { return M.fround[t]; }
+WT_HD uint64_t& FSTAMP(const Mesh& M, uint32_t t) { return M.fstamp[t]; }
+WT_HD uint64_t flip_stamp(uint32_t round, uint32_t it) { return (uint64_t)round << 32 | it; }
 WT_HD uint32_t& TFWD(const Mesh& M, uint32_t t) { return M.tfwd[t]; }
 
 WT_HD float3 P3(const Mesh& M, uint32_t v) {
@@ -303,8 +304,7 @@ WT_HD uint32_t split_body(const Mesh& M, const uint2* wlist, uint32_t k, uint32_
       INFO(M, c[j]) = kAlive | kQueued;
       M.fvote[c[j]] = kVoteNone;
       M.tsplit[c[j]] = make_uint2(0u, 0u);
-      FIT(M, c[j]) = 0u;
-      FROUND(M, c[j]) = 0u;
+      FSTAMP(M, c[j]) = 0ull;
     }
   }
   uint32_t old = at_or_u32(&INFO(M, t), kQueued);
@@ -507,15 +507,12 @@ WT_HD void flip_exec_loaded(const Mesh& M, const uint4 pr, const FlipTets& F, ui
     set4(mn, qy, enc_opp(Tzx, 3));
     set_msg(&M.tmsg[t], mt, o);
     set_msg(&M.tmsg[n], mn, a);
-    FIT(M, t) = it;
-    FIT(M, n) = it;
-    FROUND(M, t) = round;
-    FROUND(M, n) = round;
+    FSTAMP(M, t) = flip_stamp(round, it);
+    FSTAMP(M, n) = flip_stamp(round, it);
     M.tvote[s] = kVoteNone;
     M.fvote[s] = kVoteNone;
     M.tsplit[s] = make_uint2(0u, 0u);
-    FIT(M, s) = 0u;
-    FROUND(M, s) = 0u;
+    FSTAMP(M, s) = 0ull;
     INFO(M, s) = kAlive | kQueued;  // fresh slot: queued by the caller
     outk[0] = make_uint4(Txy, Tyz, Tzx, 0u);
     outk[1] = make_uint4(o, a, 0u, 0u);
@@ -547,12 +544,9 @@ WT_HD void flip_exec_loaded(const Mesh& M, const uint4 pr, const FlipTets& F, ui
     set_msg(&M.tmsg[t], mt, o);
     set_msg(&M.tmsg[n], mn, a);
     set_msg(&M.tmsg[m], mm, z);
-    FIT(M, t) = it;
-    FIT(M, n) = it;
-    FIT(M, m) = it;
-    FROUND(M, t) = round;
-    FROUND(M, n) = round;
-    FROUND(M, m) = round;
+    FSTAMP(M, t) = flip_stamp(round, it);
+    FSTAMP(M, n) = flip_stamp(round, it);
+    FSTAMP(M, m) = flip_stamp(round, it);
     INFO(M, m) = kDead;
     TFWD(M, m) = A;
     outk[0] = make_uint4(A, B, kInvalid, 1u);
@@ -661,7 +655,7 @@ WT_HD void flip_update_body(const Mesh& M, const uint4* out, uint32_t k, uint32_
   uint32_t ptr[6], fq[6];
   uint2 msg[6];
   for (int i = 0; i < 6; ++i) ptr[i] = *opp_word(M, T[i], E[i]);
-  for (int i = 0; i < 6; ++i) fq[i] = FIT(M, opp_tet(ptr[i]));
+  for (int i = 0; i < 6; ++i) fq[i] = (uint32_t)FSTAMP(M, opp_tet(ptr[i]));
   // every neighbour is a valid slot: its message is loaded unconditionally
   // (stale where it did not flip, then unused), so no branch splits the loads
   for (int i = 0; i < 6; ++i)
@@ -727,7 +721,7 @@ template <bool FAST>
 WT_HD bool relocate_body_t(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t i,
                            uint32_t round, PredStats* st) {
   uint32_t t = utet[i];
-  if (FROUND(M, t) != round) return false;
+  if ((uint32_t)(FSTAMP(M, t) >> 32) != round) return false;
   for (int guard = 0; guard < 64 && (INFO(M, t) & kDead); ++guard) t = TFWD(M, t);
   int state = 0;
   t = walk_to_t<FAST>(M, t, uvert[i], i * 0x85EBCA6Bu, st, &state);

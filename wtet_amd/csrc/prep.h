@@ -292,8 +292,7 @@ WT_HD void start_tet_body(const Mesh& M, const StartTets& S, uint32_t g, const C
   M.tvote[g] = kVoteNone;
   M.fvote[g] = kVoteNone;
   M.tsplit[g] = make_uint2(0u, 0u);
-  FIT(M, g) = 0u;
-  FROUND(M, g) = 0u;
+  FSTAMP(M, g) = 0ull;
 }
 
 // Exact duplicates (equal normalized coordinates in one cloud) share a key,
