@@ -248,6 +248,7 @@ def main():
         print(f"exact-only: {ms_plain:.2f} ms/step", file=sys.stderr, flush=True)
 
     batch = None
+    bp = boff = None
     if not args.no_batch:
         # 4,096 independent clouds (log-normal sizes, median 2,000, Gaussian
         # mixtures): one batched pass vs single-instance calls per cloud
@@ -406,6 +407,20 @@ def main():
         cfg1["cpu"] = {"oracle_walk_seed_ms": round(dt_walk * 1e3, 1),
                        "reference_linear_seed_scan_ms": round(dt_scan * 1e3, 1), "cores": 1}
         cpu["config1"] = cfg1["cpu"]
+    if not args.no_cpu and batch is not None:
+        # the batched workload on every host core: the oracle per cloud,
+        # clouds spread over OpenMP threads (each cloud normalized on its own)
+        threads = min(16, os.cpu_count() or 1)
+        pnb = np.concatenate([O.normalize_points(bp[int(boff[c]):int(boff[c + 1])])[0]
+                              for c in range(len(boff) - 1)])
+        t0 = time.perf_counter()
+        O.delaunay_many(pnb, boff, threads)
+        dt_b = time.perf_counter() - t0
+        batch["cpu_all_cores"] = {"value": round(len(bp) / dt_b, 1), "unit": "points/s", "threads": threads,
+                                  "kind": "port", "s": round(dt_b, 2),
+                                  "sample": "the whole 4,096-cloud batch, one oracle Bowyer-Watson per cloud"}
+        cpu["batch_all_cores"] = batch["cpu_all_cores"]
+        print(f"cpu batch on {threads} threads: {dt_b:.1f} s", file=sys.stderr, flush=True)
 
     kern = {k: {"launches": v[0], "ms": round(v[1], 3)} for k, v in kt.items() if v[0]}
     out = {

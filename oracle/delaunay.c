@@ -633,3 +633,24 @@ int64_t o_delaunay_bw_reference(const float *pts, int64_t n, uint32_t **tets_out
   free(P); free(T); free(A); free(stamp); free(cav); free(queue); free(bd); free(NT); free(NA);
   return ok ? cnt : -1;
 }
+
+/* Many independent clouds (normalized points, cloud c = rows
+ * [off[c], off[c+1])), each triangulated by o_delaunay_bw on its own, the
+ * clouds spread over `threads` OpenMP threads.  counts[c] receives cloud
+ * c's tet count (-1 on failure).  Returns the total, or -1.  The CPU
+ * baseline of the batched workload: the reference triangulates one cloud
+ * per delaunay_3d call (src/lib.rs:160-310). */
+int64_t o_delaunay_bw_many(const float *pts, const uint64_t *off, int64_t C, int threads,
+                           int64_t *counts) {
+  int64_t total = 0, bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads) reduction(+ : total, bad)
+  for (int64_t c = 0; c < C; ++c) {
+    uint32_t *t = NULL;
+    int64_t m = o_delaunay_bw(pts + 3 * off[c], (int64_t)(off[c + 1] - off[c]), &t, NULL);
+    counts[c] = m;
+    if (m < 0) bad++;
+    else total += m;
+    free(t);
+  }
+  return bad ? -1 : total;
+}

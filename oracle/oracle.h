@@ -108,6 +108,10 @@ void o_free(void *p);
  * search.  Same output contract as o_delaunay_bw.  O(N^2): the CPU
  * baseline on small inputs only. */
 int64_t o_delaunay_bw_reference(const float *pts, int64_t n, uint32_t **tets_out);
+/* Many independent normalized clouds on `threads` OpenMP threads, one
+ * o_delaunay_bw each: counts[c] = tets of cloud c; returns the total or -1. */
+int64_t o_delaunay_bw_many(const float *pts, const uint64_t *off, int64_t C, int threads,
+                           int64_t *counts);
 
 /* Keep tets whose 4 ids are all < n, sort each quad, sort the list.
  * Writes into out (capacity m*4), returns count. */

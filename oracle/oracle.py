@@ -51,6 +51,8 @@ def lib():
         L.o_delaunay_bw.restype = i64
         L.o_delaunay_bw_reference.argtypes = [P, i64, ctypes.POINTER(ctypes.POINTER(u32))]
         L.o_delaunay_bw_reference.restype = i64
+        L.o_delaunay_bw_many.argtypes = [P, P, i64, ctypes.c_int, P]
+        L.o_delaunay_bw_many.restype = i64
         L.o_free.argtypes = [P]
         L.o_canonical_real.argtypes = [P, i64, u32, P]
         L.o_canonical_real.restype = i64
@@ -188,6 +190,18 @@ def delaunay_reference(points_normalized: np.ndarray) -> np.ndarray:
     tets = np.ctypeslib.as_array(ptr, shape=(m * 4,)).copy().reshape(m, 4)
     lib().o_free(ctypes.cast(ptr, ctypes.c_void_p))
     return tets
+
+
+def delaunay_many(points_normalized: np.ndarray, offsets, threads: int) -> np.ndarray:
+    """``delaunay`` of each cloud [offsets[c], offsets[c+1]) of already
+    per-cloud-normalized points, the clouds spread over `threads` OpenMP
+    threads; returns the tet count of each cloud."""
+    pts = np.ascontiguousarray(points_normalized, dtype=np.float32).reshape(-1, 3)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    counts = np.zeros(len(off) - 1, np.int64)
+    if lib().o_delaunay_bw_many(_p(pts), _p(off), len(off) - 1, int(threads), _p(counts)) < 0:
+        raise RuntimeError("oracle Bowyer-Watson failed on a cloud")
+    return counts
 
 
 def canonical_real(tets: np.ndarray, n: int) -> np.ndarray:
