@@ -261,9 +261,13 @@ wtet_status wtet_set_repair_region_cap(wtet_ctx *ctx, uint32_t tets);
 wtet_status wtet_set_audit_records(wtet_ctx *ctx, uint32_t records);
 
 /* The flip iterations of a round run on the device without host reads
- * (counts, iteration tag and list parity live in device counters); they
- * are captured in HIP graphs and replayed every round (on by default; off:
- * direct launches).  The list kernels are persistent grid-stride kernels
+ * (counts, iteration tag and list parity live in device counters).  Each
+ * insertion round after its pick -- splits, points into the children, the
+ * round's flip iterations, relocation, and the next round's vote and pick
+ * -- is captured in a HIP graph keyed by its grid sizes (powers of two over
+ * the host's bounds; every kernel reads its counts from device counters)
+ * and replayed for every round and call that fits it, with one host read
+ * per round (on by default; off: the same kernels launched directly).  The list kernels are persistent grid-stride kernels
  * whose grid is sized per round from the host's bound on the first list,
  * for `items` entries per thread (default 2; powers of two from 16 to 4096
  * workgroups, one graph per size).  No counterpart in the reference, which

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <iterator>
 #include <atomic>
 #include <mutex>
@@ -51,7 +52,7 @@ struct wtet_ctx {
   // point preparation
   DevBuf xyz_in, xyzn, keys, ids, keys2, ids2, sort_tmp, mm6, pts, doff;
   // mesh (per tet slot)
-  DevBuf tet, adj_t, tinfo, tfwd, tvote, fvote, tsplit, tpoint, tmsg, fstamp;
+  DevBuf tet, adj_t, tinfo, tfwd, tvote, fvote, tsplit, tpoint, tmsg;
   // lists
   DevBuf uvert[2], utet[2], wlist, act[2], prop, fout, blk, slot2idx, rec;
   // output grouping for many clouds
@@ -72,7 +73,8 @@ struct wtet_ctx {
     std::vector<uint64_t> key;
     hipGraphExec_t exec;
   };
-  std::vector<GraphEntry> graphs;  // one per (buffers, grid, iterations)
+  std::vector<GraphEntry> graphs;        // flip iterations: one per (buffers, grid, iterations)
+  std::vector<GraphEntry> round_graphs;  // rounds: one per (buffers, grid buckets, variant)
   uint32_t graph_captures = 0;
   // stage-level session (wtet_stage_*), null when none
   wtet_stage_session* stage = nullptr;
@@ -115,7 +117,7 @@ struct wtet_ctx {
   wtet_ctx() {
     DevBuf* l[] = {&xyz_in, &xyzn,   &keys,   &ids,   &keys2,  &ids2,     &sort_tmp, &mm6,
                    &pts,    &doff,   &tet,    &adj_t,  &tinfo,  &tfwd,  &tvote,    &fvote,    &tsplit,  &tpoint,
-                   &tmsg,   &fstamp, &uvert[0], &uvert[1], &utet[0], &utet[1],
+                   &tmsg,   &uvert[0], &uvert[1], &utet[0], &utet[1],
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
                    &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
                    &presize_params, &feat_acc, &feat, &pred, &isums, &slow_t, &slow_p,
@@ -233,7 +235,6 @@ void ensure_tets(wtet_ctx* c, uint64_t cap, bool keep) {
   ensure(c, c->tsplit, cap * 8, keep);
   ensure(c, c->tpoint, cap * 4, keep);
   ensure(c, c->tmsg, cap * sizeof(FlipMsg), keep);
-  ensure(c, c->fstamp, cap * 8, keep);
   ensure(c, c->act[0], cap * 4, keep);
   ensure(c, c->act[1], cap * 4, keep);
   ensure(c, c->slow_t, cap * 4, false);
@@ -299,7 +300,6 @@ Mesh mesh_of(wtet_ctx* c, uint32_t inf, int mode) {
   M.tsplit = P<uint2>(c->tsplit);
   M.tpoint = P<uint32_t>(c->tpoint);
   M.tmsg = P<FlipMsg>(c->tmsg);
-  M.fstamp = P<uint64_t>(c->fstamp);
   M.ctr = P<uint32_t>(c->ctr);
   M.inf = inf;
   M.mode = mode;
@@ -368,6 +368,57 @@ uint32_t grid_for(const wtet_ctx* c, uint64_t a, uint32_t lo) {
 constexpr uint64_t kSmallList = 512;
 constexpr uint32_t kSmallLimit = 2048;
 
+// The device buffers a captured sequence reads: a graph of another buffer
+// set is stale.
+std::vector<uint64_t> buffer_key(wtet_ctx* c, const Mesh& M, const FlipLists& L, uint64_t* st) {
+  return {(uint64_t)(uintptr_t)M.pts,    (uint64_t)(uintptr_t)M.tet,    (uint64_t)(uintptr_t)M.tvote,
+          (uint64_t)(uintptr_t)M.adj,    (uint64_t)(uintptr_t)M.tinfo,  (uint64_t)(uintptr_t)M.tfwd,
+          (uint64_t)(uintptr_t)M.fvote,  (uint64_t)(uintptr_t)M.tsplit, (uint64_t)(uintptr_t)M.tmsg,
+          (uint64_t)(uintptr_t)M.tpoint, (uint64_t)(uintptr_t)M.ctr,
+          (uint64_t)M.inf,               (uint64_t)M.mode,              (uint64_t)(uintptr_t)L.act[0],
+          (uint64_t)(uintptr_t)L.act[1], (uint64_t)(uintptr_t)L.prop,   (uint64_t)(uintptr_t)L.out,
+          (uint64_t)(uintptr_t)L.slow,   (uint64_t)(uintptr_t)st,       (uint64_t)(uintptr_t)c->isums.p,
+          (uint64_t)(uintptr_t)c->uvert[0].p, (uint64_t)(uintptr_t)c->uvert[1].p,
+          (uint64_t)(uintptr_t)c->utet[0].p,  (uint64_t)(uintptr_t)c->utet[1].p,
+          (uint64_t)(uintptr_t)c->wlist.p,    (uint64_t)(uintptr_t)c->blk.p,
+          (uint64_t)(uintptr_t)c->slow_p.p};
+}
+
+// Replays the graph of (buffers, params), capturing `enqueue` into it on
+// first use.  Graphs of another buffer set in the same cache are destroyed
+// first: every call ends with a stream synchronization, and a growth frees
+// the old buffers (which synchronizes), so none of them is still running.
+void launch_cached_graph(wtet_ctx* c, std::vector<wtet_ctx::GraphEntry>& cache,
+                         const std::vector<uint64_t>& buffers, const std::vector<uint64_t>& params,
+                         const std::function<void()>& enqueue) {
+  std::vector<uint64_t> key = buffers;
+  key.insert(key.end(), params.begin(), params.end());
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : cache)
+    if (g.key == key) exec = g.exec;
+  if (!exec) {
+    for (size_t i = 0; i < cache.size();) {
+      const std::vector<uint64_t>& k = cache[i].key;
+      if (k.size() < buffers.size() || !std::equal(buffers.begin(), buffers.end(), k.begin())) {
+        HIP_CHECK(hipGraphExecDestroy(cache[i].exec));
+        cache.erase(cache.begin() + (long)i);
+      } else {
+        ++i;
+      }
+    }
+    hipGraph_t g = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    HIP_CHECK(hipStreamEndCapture(c->stream, &g));
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_CHECK(e);
+    cache.push_back({key, exec});
+    c->graph_captures++;
+  }
+  HIP_CHECK(hipGraphLaunch(exec, c->stream));
+}
+
 void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, uint64_t a_hint,
                     uint64_t d_hint) {
   Mesh M = mesh_of(c, inf, mode);
@@ -387,40 +438,8 @@ void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, ui
     HIP_CHECK(hipGetLastError());
     return;
   }
-  std::vector<uint64_t> key = {
-      (uint64_t)(uintptr_t)M.pts,   (uint64_t)(uintptr_t)M.tet,    (uint64_t)(uintptr_t)M.tvote,
-      (uint64_t)(uintptr_t)M.adj,   (uint64_t)(uintptr_t)M.tinfo,  (uint64_t)(uintptr_t)M.tfwd,
-      (uint64_t)(uintptr_t)M.fvote, (uint64_t)(uintptr_t)M.tsplit, (uint64_t)(uintptr_t)M.tmsg,
-      (uint64_t)(uintptr_t)M.tpoint,
-      (uint64_t)(uintptr_t)M.ctr,   (uint64_t)(uintptr_t)M.fstamp,
-      (uint64_t)M.inf,              (uint64_t)M.mode,              (uint64_t)(uintptr_t)L.act[0],
-      (uint64_t)(uintptr_t)L.act[1], (uint64_t)(uintptr_t)L.prop,  (uint64_t)(uintptr_t)L.out,
-      (uint64_t)(uintptr_t)L.slow,  (uint64_t)(uintptr_t)st,       (uint64_t)(uintptr_t)c->isums.p,
-      (uint64_t)dgrid,              (uint64_t)grid,                (uint64_t)k};
-  hipGraphExec_t exec = nullptr;
-  for (auto& g : c->graphs)
-    if (g.key == key) exec = g.exec;
-  if (!exec) {
-    // graphs of older buffer sets are stale: keep only this buffer set's
-    for (size_t i = 0; i < c->graphs.size();) {
-      if (!std::equal(key.begin(), key.end() - 3, c->graphs[i].key.begin())) {
-        HIP_CHECK(hipGraphExecDestroy(c->graphs[i].exec));
-        c->graphs.erase(c->graphs.begin() + (long)i);
-      } else {
-        ++i;
-      }
-    }
-    hipGraph_t g = nullptr;
-    HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_flip_iterations(c, M, L, st, k, grid, dgrid);
-    HIP_CHECK(hipStreamEndCapture(c->stream, &g));
-    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HIP_CHECK(e);
-    c->graphs.push_back({key, exec});
-    c->graph_captures++;
-  }
-  HIP_CHECK(hipGraphLaunch(exec, c->stream));
+  launch_cached_graph(c, c->graphs, buffer_key(c, M, L, st), {(uint64_t)dgrid, (uint64_t)grid, (uint64_t)k},
+                      [&] { enqueue_flip_iterations(c, M, L, st, k, grid, dgrid); });
 }
 
 // tet slots after growth by `need`; the flip iterations read the capacity
@@ -521,7 +540,7 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
   ensure(c, c->gstats, 64);
   ensure(c, c->isums, S_COUNT * 8);
   ensure(c, c->rec, (size_t)c->audit_records * 4);
-  ensure(c, c->blk, ((size_t)nblocks(n) + 2) * 4);
+  ensure(c, c->blk, ((size_t)round_grid(n) + 2) * 4);
   if (R.learned) {
     if (!c->presize_loaded) throw Fail{WTET_ERR_INVALID_ARGUMENT, "presize_mode 1 needs a model (wtet_set_presize_model)"};
     ensure(c, c->feat_acc, (size_t)C * (kHistBins / 2 + 9) * 8 + 64);
@@ -610,25 +629,11 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
   launch_set_ctr(P<uint32_t>(c->ctr), C_CAP, (uint32_t)c->cap_tets, -1, 0u, s);
 }
 
-// One insertion round's vote, winner pick and compaction of the survivors
-// (vote.wgsl:89-344, compact_vertex_arrays_*.wgsl): returns the round's
-// insertions; the survivors become the uninserted list.
-uint32_t dev_vote(wtet_ctx* c, Run& R) {
-  hipStream_t s = c->stream;
-  ++R.round;
-  if (R.round > (1u << 16)) throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
-  Mesh M = mesh_of(c, R.V, R.mode);
-  const uint32_t U = R.U, cur = R.cur, round = R.round;
-  launch_set_ctr(M.ctr, C_ULEN, U, C_ROUND, round, s);
-  timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
-  timed(c, K_PICK, [&] {
-    launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, round,
-                P<uint2>(c->wlist), P<uint32_t>(c->uvert[cur ^ 1]),
-                P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), round <= R.warmup ? 0 : c->ring, s);
-  });
-  HIP_CHECK(hipGetLastError());
-  // the one host read of a round: winners, reserved slots, and whether the
-  // previous round's flips stopped for capacity
+// After a round's pick, on the host: the one read of the round (winners,
+// reserved slots, and whether the previous round's flips stopped for
+// capacity), its checks and the growth of the tet slots.  U: the round's
+// uninserted count before the pick.
+void after_pick(wtet_ctx* c, Run& R, uint32_t U) {
   read_ctr(c);
   uint32_t W = c->h_ctr[C_WIN];
   if (c->h_ctr[C_BADTET]) throw Fail{WTET_ERR_INTERNAL, "a point was located outside the super tet"};
@@ -639,15 +644,35 @@ uint32_t dev_vote(wtet_ctx* c, Run& R) {
   R.K.rounds++;
   R.W = W;
   R.U = U - W;
-  R.cur = cur ^ 1;
-  return W;
+  R.cur ^= 1;
+}
+
+// One insertion round's vote, winner pick and compaction of the survivors
+// (vote.wgsl:89-344, compact_vertex_arrays_*.wgsl), launched one by one:
+// returns the round's insertions; the survivors become the uninserted list.
+uint32_t dev_vote(wtet_ctx* c, Run& R) {
+  hipStream_t s = c->stream;
+  ++R.round;
+  if (R.round > (1u << 16)) throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
+  Mesh M = mesh_of(c, R.V, R.mode);
+  const uint32_t U = R.U, cur = R.cur, round = R.round;
+  launch_set_ctr(M.ctr, C_ULEN, U, C_ROUND, round, s);
+  timed(c, K_VOTE, [&] { launch_vote(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, s); });
+  timed(c, K_PICK, [&] {
+    launch_pick(M, P<uint32_t>(c->uvert[cur]), P<uint32_t>(c->utet[cur]), U, P<uint2>(c->wlist),
+                P<uint32_t>(c->uvert[cur ^ 1]), P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk),
+                round <= R.warmup ? 0 : c->ring, s);
+  });
+  HIP_CHECK(hipGetLastError());
+  after_pick(c, R, U);
+  return R.W;
 }
 
 // 1 -> 4 splits of the round's winners (split.wgsl:104-327); the children
 // join the active list.
 void dev_split(wtet_ctx* c, Run& R) {
   Mesh M = mesh_of(c, R.V, R.mode);
-  timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), R.W, R.round, lists_of(c), c->stream); });
+  timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), R.W, lists_of(c), c->stream); });
   HIP_CHECK(hipGetLastError());
 }
 
@@ -655,8 +680,8 @@ void dev_split(wtet_ctx* c, Run& R) {
 void dev_split_points(wtet_ctx* c, Run& R) {
   Mesh M = mesh_of(c, R.V, R.mode);
   timed(c, K_SPLIT_POINTS, [&] {
-    launch_split_points(M, P<uint32_t>(c->uvert[R.cur]), P<uint32_t>(c->utet[R.cur]), R.U, R.round,
-                        R.st, P<uint32_t>(c->slow_p), c->stream);
+    launch_split_points(M, P<uint32_t>(c->uvert[R.cur]), P<uint32_t>(c->utet[R.cur]), R.U, R.st,
+                        P<uint32_t>(c->slow_p), c->stream);
   });
   HIP_CHECK(hipGetLastError());
 }
@@ -676,9 +701,70 @@ void dev_relocate(wtet_ctx* c, Run& R) {
   if (R.U == 0) return;
   Mesh M = mesh_of(c, R.V, R.mode);
   timed(c, K_RELOCATE, [&] {
-    launch_relocate(M, P<uint32_t>(c->uvert[R.cur]), P<uint32_t>(c->utet[R.cur]), R.U, R.round, R.st,
+    launch_relocate(M, P<uint32_t>(c->uvert[R.cur]), P<uint32_t>(c->utet[R.cur]), R.U, R.st,
                     P<uint32_t>(c->slow_p), c->stream);
   });
+  HIP_CHECK(hipGetLastError());
+}
+
+// One round after its pick as one captured graph: the 1-4 splits, the
+// points into the children, the round's flip iterations and the
+// relocation, then (unless it is the last round) the next round's vote and
+// pick.  Every count comes from ctr and every grid is a power of two over
+// the host's bound, so the graph of one (buffer set, grids, variant)
+// replays for each round and call that fits it; the host's one read per
+// round follows the pick.  When profiling, or with graphs off, the same
+// sequence is launched directly.
+void run_round(wtet_ctx* c, Run& R, bool last) {
+  hipStream_t s = c->stream;
+  Mesh M = mesh_of(c, R.V, R.mode);
+  FlipLists L = lists_of(c);
+  const uint32_t cur = R.cur, W = R.W, U = R.U;
+  const int ring_next = R.round + 1 <= R.warmup ? 0 : c->ring;
+  // the round's flips, sized as dev_round_flips sizes them
+  uint64_t dsum = c->h_ctr[C_SLOW_SUM];
+  uint64_t dper = (2 * (dsum - R.dsum_prev) + R.max_fi - 1) / (uint64_t)R.max_fi;
+  R.dsum_prev = dsum;
+  const uint64_t a_hint = (uint64_t)c->h_ctr[C_A] + 4ull * W;
+  const int variant = !R.flipping ? 0 : (a_hint <= kSmallList ? 1 : 2);
+  const uint32_t grid = variant == 2 ? grid_for(c, a_hint, 16) : 0u;
+  const uint32_t dgrid = variant == 2 ? std::min(grid_for(c, dper, 1), grid) : 0u;
+  const int k = R.max_fi;
+  uint32_t* uv = P<uint32_t>(c->uvert[cur]);
+  uint32_t* ut = P<uint32_t>(c->utet[cur]);
+  uint32_t* slow_p = P<uint32_t>(c->slow_p);
+  uint64_t* sums = P<uint64_t>(c->isums);
+  auto enqueue = [&] {
+    timed(c, K_SPLIT, [&] { launch_split(M, P<uint2>(c->wlist), W, L, s); });
+    timed(c, K_SPLIT_POINTS, [&] { launch_split_points(M, uv, ut, U, R.st, slow_p, s); });
+    if (variant == 0) {
+      launch_set_ctr(M.ctr, C_A, 0u, -1, 0u, s);
+    } else {
+      if (variant == 1)
+        timed(c, K_FLIP_SMALL, [&] { launch_flip_small(M, L, R.st, sums, k, kSmallLimit, s); });
+      else
+        enqueue_flip_iterations(c, M, L, R.st, k, grid, dgrid);
+      timed(c, K_RELOCATE, [&] { launch_relocate(M, uv, ut, U, R.st, slow_p, s); });
+    }
+    if (!last) {
+      launch_round_next(M.ctr, s);
+      timed(c, K_VOTE, [&] { launch_vote(M, uv, ut, U, s); });
+      timed(c, K_PICK, [&] {
+        launch_pick(M, uv, ut, U, P<uint2>(c->wlist), P<uint32_t>(c->uvert[cur ^ 1]),
+                    P<uint32_t>(c->utet[cur ^ 1]), P<uint32_t>(c->blk), ring_next, s);
+      });
+    }
+  };
+  if (!c->use_graphs || c->profiling) {
+    if (c->evused > 4096) collect_times(c);
+    enqueue();
+  } else {
+    launch_cached_graph(c, c->round_graphs, buffer_key(c, M, L, R.st),
+                        {(uint64_t)cur, (uint64_t)round_grid(W), (uint64_t)round_grid(U), (uint64_t)(U > 0),
+                         (uint64_t)variant, (uint64_t)grid, (uint64_t)dgrid, (uint64_t)k, (uint64_t)last,
+                         (uint64_t)ring_next},
+                        enqueue);
+  }
   HIP_CHECK(hipGetLastError());
 }
 
@@ -824,16 +910,18 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
   run_setup(c, R, off, cfg);
   dev_prepare(c, R, d_xyz);
   // at most max_insert_iterations rounds (src/phase1.rs:26); the host phase
-  // inserts what they leave
-  while (R.U > 0 && R.round < R.cfg.max_insert_iterations) {
+  // inserts what they leave.  The first round's vote and pick are launched
+  // one by one, every later round runs from a captured graph (run_round).
+  if (R.U > 0 && R.round < R.cfg.max_insert_iterations) {
     dev_vote(c, R);
-    dev_split(c, R);
-    dev_split_points(c, R);
-    if (R.flipping) {
-      dev_round_flips(c, R);
-      dev_relocate(c, R);
-    } else {
-      launch_set_ctr(P<uint32_t>(c->ctr), C_A, 0u, -1, 0u, c->stream);
+    for (;;) {
+      const bool last = R.U == 0 || R.round >= R.cfg.max_insert_iterations;
+      const uint32_t U = R.U;
+      run_round(c, R, last);
+      if (last) break;
+      ++R.round;  // the device counted it too (launch_round_next)
+      if (R.round > (1u << 16)) throw Fail{WTET_ERR_NOT_CONVERGED, "insertion did not finish"};
+      after_pick(c, R, U);
     }
   }
   R.K.left = R.U;
@@ -1572,6 +1660,7 @@ void wtet_destroy(wtet_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  for (auto& g : c->round_graphs) (void)hipGraphExecDestroy(g.exec);
   for (DevBuf* b : c->all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);

@@ -47,13 +47,18 @@ hipError_t launch_stratify(uint32_t* uvert, uint32_t* utet, uint32_t U, const Cl
                            uint64_t* key, uint64_t* key_tmp, uint32_t* val_tmp, void* temp,
                            size_t temp_bytes, hipStream_t s);
 
-// rounds
+// rounds.  The round kernels read their counts and the round number from
+// ctr (C_ULEN, C_WIN, C_SURV, C_ROUND); the count arguments only bound the
+// launch grids, a power of two of workgroups (round_grid), so a captured
+// round replays for any count up to its bound.
+uint32_t round_grid(uint64_t bound);
 void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                  hipStream_t s);
 // ring: insert only where no tet within `ring` face-steps has a smaller vote
+// (blk: round_grid(U) entries)
 void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
-                 uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
-                 uint32_t* blk, int ring, hipStream_t s);
+                 uint2* wlist, uint32_t* uvert2, uint32_t* utet2, uint32_t* blk, int ring,
+                 hipStream_t s);
 // the two active lists (current = act[ctr[C_PAR]]), flip proposals and the
 // executed flips of one iteration
 struct FlipLists {
@@ -64,12 +69,13 @@ struct FlipLists {
 };
 // appends the children to the current active list after its ctr[C_A]
 // entries and updates ctr[C_A]
-void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round,
-                  const FlipLists& L, hipStream_t s);
+void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, const FlipLists& L, hipStream_t s);
+// ctr[C_ULEN] = ctr[C_SURV], ctr[C_ROUND] += 1: the next round, on the device
+void launch_round_next(uint32_t* ctr, hipStream_t s);
 // filtered mode: an FP32-only kernel, then the full predicates on the points
 // it deferred (slow, U entries of scratch); exact mode: one full kernel
 void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                         uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s);
+                         uint64_t* st, uint32_t* slow, hipStream_t s);
 // one flip iteration = begin, check, decide+execute, update, end; counts,
 // tag, round and parity are read from ctr (device-driven, graph-capturable);
 // grid = persistent blocks of the three list kernels
@@ -91,7 +97,7 @@ void launch_iter_end(uint32_t* ctr, uint64_t* sums, hipStream_t s);
 // end of one iteration followed by the begin of the next
 void launch_iter_step(uint32_t* ctr, uint64_t* sums, hipStream_t s);
 void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                     uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s);
+                     uint64_t* st, uint32_t* slow, hipStream_t s);
 // filtered mode: an FP32-only pass, then the full predicates on the tets it
 // could not decide (slow: T entries of scratch); exact mode: one full pass
 void launch_audit(const Mesh& M, uint32_t T, uint32_t* rec, uint32_t cap, uint64_t* st,

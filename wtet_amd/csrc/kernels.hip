@@ -330,7 +330,8 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
 // their keys with a wave min first (a few passes), so early rounds -- when
 // most points still share a few tets -- issue one atomic per wave instead of
 // one per point; the remaining lanes vote directly.
-__global__ void __launch_bounds__(kBlock) k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U) {
+__global__ void __launch_bounds__(kBlock) k_vote(Mesh M, const uint32_t* uvert, const uint32_t* utet) {
+  const uint32_t U = M.ctr[C_ULEN];
   uint32_t i = gid();
   uint64_t key = kVoteNone;
   uint32_t t = 0;
@@ -357,7 +358,8 @@ __global__ void __launch_bounds__(kBlock) k_vote(Mesh M, const uint32_t* uvert, 
   if (pending) at_min_u64(&M.tvote[t], key);
 }
 
-__global__ void __launch_bounds__(kBlock) k_pick_count(Mesh M, const uint32_t* utet, uint32_t U, uint32_t* blk, int ring) {
+__global__ void __launch_bounds__(kBlock) k_pick_count(Mesh M, const uint32_t* utet, uint32_t* blk, int ring) {
+  const uint32_t U = M.ctr[C_ULEN];
   uint32_t i = gid();
   bool w = i < U && is_winner(M, utet, i, ring);
   uint32_t tot;
@@ -365,9 +367,10 @@ __global__ void __launch_bounds__(kBlock) k_pick_count(Mesh M, const uint32_t* u
   if (threadIdx.x == 0) blk[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(kBlock) k_pick_scatter(Mesh M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
-                               uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2,
-                               const uint32_t* blk, int ring) {
+__global__ void __launch_bounds__(kBlock) k_pick_scatter(Mesh M, const uint32_t* uvert, const uint32_t* utet,
+                               uint2* wlist, uint32_t* uvert2, uint32_t* utet2, const uint32_t* blk,
+                               int ring) {
+  const uint32_t U = M.ctr[C_ULEN], round = M.ctr[C_ROUND];
   uint32_t i = gid();
   bool w = i < U && is_winner(M, utet, i, ring);
   uint32_t tot;
@@ -384,9 +387,18 @@ __global__ void __launch_bounds__(kBlock) k_pick_scatter(Mesh M, const uint32_t*
   }
 }
 
+// Launch grids of the round kernels: a power of two of workgroups covering
+// the host's bound, so that a captured round can be replayed for any count
+// up to that bound (the kernels read the counts from ctr).
+uint32_t round_grid(uint64_t bound) {
+  uint32_t want = std::max<uint32_t>(1u, nblocks(bound)), g = 1;
+  while (g < want) g <<= 1;
+  return g;
+}
+
 void launch_vote(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
                  hipStream_t s) {
-  k_vote<<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U);
+  k_vote<<<round_grid(U), kBlock, 0, s>>>(M, uvert, utet);
 }
 
 // votes of tets whose winner was deferred must not survive into the next
@@ -397,12 +409,14 @@ __global__ void k_clear_votes(Mesh M, const uint32_t* utet2, const uint32_t* ctr
 }
 
 void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uint32_t U,
-                 uint32_t round, uint2* wlist, uint32_t* uvert2, uint32_t* utet2, uint32_t* blk,
-                 int ring, hipStream_t s) {
-  uint32_t nb = nblocks(U);
-  k_pick_count<<<nb, kBlock, 0, s>>>(M, utet, U, blk, ring);
+                 uint2* wlist, uint32_t* uvert2, uint32_t* utet2, uint32_t* blk, int ring,
+                 hipStream_t s) {
+  // blocks past the count write zero block totals, so the scan may cover
+  // the whole grid
+  uint32_t nb = round_grid(U);
+  k_pick_count<<<nb, kBlock, 0, s>>>(M, utet, blk, ring);
   k_scan_blocks<<<1, 1024, 0, s>>>(blk, nb, M.ctr, 0);
-  k_pick_scatter<<<nb, kBlock, 0, s>>>(M, uvert, utet, U, round, wlist, uvert2, utet2, blk, ring);
+  k_pick_scatter<<<nb, kBlock, 0, s>>>(M, uvert, utet, wlist, uvert2, utet2, blk, ring);
   if (ring > 0) k_clear_votes<<<nb, kBlock, 0, s>>>(M, utet2, M.ctr);
 }
 
@@ -412,7 +426,8 @@ void launch_pick(const Mesh& M, const uint32_t* uvert, const uint32_t* utet, uin
 // that follow are numbered in list order, and lower slot ids win flip
 // votes.  On co-spherical inputs a tet-major order left about four times
 // as many faces stuck (1M points on a sphere: 3,150 against 866).
-__global__ void __launch_bounds__(kBlock) k_split(Mesh M, const uint2* wlist, uint32_t W, uint32_t round, FlipLists L) {
+__global__ void __launch_bounds__(kBlock) k_split(Mesh M, const uint2* wlist, FlipLists L) {
+  const uint32_t W = M.ctr[C_WIN], round = M.ctr[C_ROUND];
   uint32_t k = gid();
   uint32_t c[4], m = 0u;
   if (k < W) m = split_body(M, wlist, k, round, c);
@@ -437,23 +452,33 @@ __global__ void k_ctr_copy(uint32_t* ctr, int dst, int src) {
   if (threadIdx.x == 0) ctr[dst] = ctr[src];
 }
 
-void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, uint32_t round, const FlipLists& L,
-                  hipStream_t s) {
+void launch_split(const Mesh& M, const uint2* wlist, uint32_t W, const FlipLists& L, hipStream_t s) {
   // the children are appended after the active entries carried over
   k_ctr_copy<<<1, 64, 0, s>>>(M.ctr, C_ACT_NEXT, C_A);
-  if (W) k_split<<<nblocks(W), kBlock, 0, s>>>(M, wlist, W, round, L);
+  k_split<<<round_grid(W), kBlock, 0, s>>>(M, wlist, L);
   k_ctr_copy<<<1, 64, 0, s>>>(M.ctr, C_A, C_ACT_NEXT);
 }
+
+// the next round's uninserted count and number, on the device
+__global__ void k_round_next(uint32_t* ctr) {
+  if (threadIdx.x == 0) {
+    ctr[C_ULEN] = ctr[C_SURV];
+    ctr[C_ROUND] += 1u;
+  }
+}
+
+void launch_round_next(uint32_t* ctr, hipStream_t s) { k_round_next<<<1, 64, 0, s>>>(ctr); }
 
 // Point kernels in three forms: LEAN runs the FP32 filter alone and defers
 // the points it cannot decide to `slow` (ctr[C_SLOW_P]); SLOW runs the full
 // staged predicates on those; FULL runs them on every point (exact mode).
 enum KForm : int { LEAN = 0, FULL = 1, SLOW = 2 };
 
+// The survivors of the round's pick (ctr[C_SURV]) in the round ctr[C_ROUND].
 template <int FORM>
 __global__ void __launch_bounds__(kBlock) k_split_points(Mesh M, const uint32_t* uvert,
-                                                         uint32_t* utet, uint32_t U, uint32_t round,
-                                                         uint64_t* st, uint32_t* slow) {
+                                                         uint32_t* utet, uint64_t* st, uint32_t* slow) {
+  const uint32_t U = M.ctr[C_SURV], round = M.ctr[C_ROUND];
   PredStats ls = {};
   if (FORM == SLOW) {
     uint32_t n = M.ctr[C_SLOW_P], stride = gridDim.x * blockDim.x;
@@ -469,14 +494,15 @@ __global__ void __launch_bounds__(kBlock) k_split_points(Mesh M, const uint32_t*
 }
 
 void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                         uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s) {
+                         uint64_t* st, uint32_t* slow, hipStream_t s) {
   if (!U) return;
+  uint32_t g = round_grid(U);
   if (M.mode == PRED_FILTERED) {
     launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
-    k_split_points<LEAN><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
-    k_split_points<SLOW><<<std::min(nblocks(U) / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_split_points<LEAN><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
+    k_split_points<SLOW><<<std::min(g / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, st, slow);
   } else {
-    k_split_points<FULL><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_split_points<FULL><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
   }
 }
 
@@ -729,10 +755,11 @@ void launch_flip_update(const Mesh& M, const FlipLists& L, uint32_t grid, hipStr
   k_flip_update<<<grid, kBlock, 0, s>>>(M, L);
 }
 
+// The uninserted points after the round's pick (ctr[C_SURV]), round ctr[C_ROUND].
 template <int FORM>
 __global__ void __launch_bounds__(kBlock) k_relocate(Mesh M, const uint32_t* uvert, uint32_t* utet,
-                                                     uint32_t U, uint32_t round, uint64_t* st,
-                                                     uint32_t* slow) {
+                                                     uint64_t* st, uint32_t* slow) {
+  const uint32_t U = M.ctr[C_SURV], round = M.ctr[C_ROUND];
   PredStats ls = {};
   if (FORM == SLOW) {
     uint32_t n = M.ctr[C_SLOW_P], stride = gridDim.x * blockDim.x;
@@ -748,14 +775,15 @@ __global__ void __launch_bounds__(kBlock) k_relocate(Mesh M, const uint32_t* uve
 }
 
 void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint32_t U,
-                     uint32_t round, uint64_t* st, uint32_t* slow, hipStream_t s) {
+                     uint64_t* st, uint32_t* slow, hipStream_t s) {
   if (!U) return;
+  uint32_t g = round_grid(U);
   if (M.mode == PRED_FILTERED) {
     launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
-    k_relocate<LEAN><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
-    k_relocate<SLOW><<<std::min(nblocks(U) / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_relocate<LEAN><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
+    k_relocate<SLOW><<<std::min(g / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, st, slow);
   } else {
-    k_relocate<FULL><<<nblocks(U), kBlock, 0, s>>>(M, uvert, utet, U, round, st, slow);
+    k_relocate<FULL><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
   }
 }
 

@@ -69,9 +69,11 @@ struct __attribute__((aligned(32))) AdjRec {
 // Where the faces of a tet destroyed by a flip went: for each external face
 // f, w[2f] = (new tet << 2) | face and w[2f + 1] = the new tet's vertex
 // opposite it (one 8-byte load per face in flip_update_body).
-struct __attribute__((aligned(32))) FlipMsg {
-  uint4 lo;  // faces 0, 1
-  uint4 hi;  // faces 2, 3
+struct __attribute__((aligned(64))) FlipMsg {
+  uint4 lo;        // faces 0, 1
+  uint4 hi;        // faces 2, 3
+  uint64_t stamp;  // the last flip that wrote the slot: round << 32 | flip iteration
+  uint64_t pad[3];
 };
 
 // message of a destroyed tet: new neighbour pointers e (kInvalid where
@@ -82,7 +84,7 @@ WT_HD void set_msg(FlipMsg* m, uint4 e, uint32_t v) {
 }
 
 static_assert(sizeof(AdjRec) == 32, "adjacency record layout");
-static_assert(sizeof(FlipMsg) == 32, "flip message layout");
+static_assert(sizeof(FlipMsg) == 64, "flip message layout: one 64-byte line per slot");
 
 struct Mesh {
   const float4* pts;  // n + 5 points (n real, 4 super, 1 infinity)
@@ -95,7 +97,6 @@ struct Mesh {
   uint2* tsplit;      // (round, first new slot) for tets split this round
   uint32_t* tpoint;   // the point inserted into a tet split this round
   FlipMsg* tmsg;      // per slot destroyed by a flip this iteration
-  uint64_t* fstamp;   // the last flip that wrote the slot: round << 32 | flip iteration
   uint32_t* ctr;      // Ctr counters
   uint32_t inf;       // id of the infinity vertex
   int mode;           // PredMode
@@ -116,7 +117,9 @@ WT_HD void set_adj(const Mesh& M, uint32_t t, uint4 n, uint4 ov) {
   M.adj[t].hi = make_uint4(n.z, ov.z, n.w, ov.w);
 }
 WT_HD uint32_t& INFO(const Mesh& M, uint32_t t) { return M.tinfo[t]; }
-WT_HD uint64_t& FSTAMP(const Mesh& M, uint32_t t) { return M.fstamp[t]; }
+// the stamp shares its slot's message line: the adjacency update reads a
+// flipped neighbour's stamp and message together
+WT_HD uint64_t& FSTAMP(const Mesh& M, uint32_t t) { return M.tmsg[t].stamp; }
 WT_HD uint64_t flip_stamp(uint32_t round, uint32_t it) { return (uint64_t)round << 32 | it; }
 WT_HD uint32_t& TFWD(const Mesh& M, uint32_t t) { return M.tfwd[t]; }
 

@@ -114,9 +114,9 @@ def make_cloud(spec: tuple):
     elif fam == "sphere":
         p = W.sphere(n, seed)
     elif fam == "shell":
-        # thin shells above ~1e5 points send the host repair into whole-mesh
-        # regions (minutes); the family is kept at sizes that finish quickly
-        p = W.shell(min(n, 100_000), seed, thickness=extra[0])
+        # thin shells send the host repair into whole-mesh regions (minutes
+        # above ~1e5 points); the family is kept at sizes that finish quickly
+        p = W.shell(min(n, 30_000), seed, thickness=extra[0])
     elif fam == "lattice":
         k = max(2, int(round(n ** (1 / 3))))
         p = W.lattice(k)
