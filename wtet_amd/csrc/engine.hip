@@ -808,6 +808,10 @@ void dev_compact(wtet_ctx* c, Run& R, uint32_t T, bool out, bool keep_lists) {
   if (out) {
     OB.out_tets = P<uint32_t>(c->prop);
     OB.out_adj = P<uint32_t>(c->fout);
+    // the sort keys are free by now: 2n words hold the n + 4C + 1 ranks
+    // (every cloud has at least 4 points)
+    ensure(c, c->keys, ((size_t)R.V + 1) * 4 + 16, false);
+    OB.vrank = P<uint32_t>(c->keys);
   }
   Mesh M = mesh_of(c, R.V, R.mode);
   timed(c, K_OUTPUT, [&] { HIP_CHECK(launch_output(M, T, R.CL, OB, c->stream)); });

@@ -118,6 +118,7 @@ struct OutputBufs {
   size_t temp_bytes;
   uint32_t* out_tets;  // rows * 4 (null: only slot2pos / cstart)
   uint32_t* out_adj;
+  uint32_t* vrank;     // scratch, one word per vertex (K.V + 1): each vertex's SoS rank
 };
 hipError_t launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s);
 

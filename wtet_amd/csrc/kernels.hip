@@ -864,11 +864,16 @@ __global__ void k_cstart_tail(uint32_t* cstart, uint32_t C, const uint32_t* ctr)
 }
 
 __global__ void k_output_rows(Mesh M, uint32_t T, const uint32_t* olist, const uint32_t* okey,
-                              const uint32_t* cstart, const uint32_t* slot2pos, Clouds K,
-                              uint32_t* out_tets, uint32_t* out_adj) {
+                              const uint32_t* cstart, const uint32_t* slot2pos, const uint32_t* vrank,
+                              Clouds K, uint32_t* out_tets, uint32_t* out_adj) {
   uint32_t j = gid();
   if (j >= T || j >= M.ctr[C_ALIVE]) return;
-  output_body(M, j, olist, okey, cstart, slot2pos, K, out_tets, out_adj);
+  output_body(M, j, olist, okey, cstart, slot2pos, vrank, K, out_tets, out_adj);
+}
+
+__global__ void k_vertex_ranks(const float4* pts, uint32_t nv, uint32_t* vrank) {
+  uint32_t v = gid();
+  if (v < nv) vrank[v] = pt_rank(pts[v]);
 }
 
 hipError_t launch_output(const Mesh& M, uint32_t T, const Clouds& K, const OutputBufs& B, hipStream_t s) {
@@ -892,9 +897,11 @@ hipError_t launch_output(const Mesh& M, uint32_t T, const Clouds& K, const Outpu
     olist = B.olist2;
     okey = B.okey2;
   }
-  if (B.out_tets)
-    k_output_rows<<<nb, kBlock, 0, s>>>(M, T, olist, okey, B.cstart, B.slot2pos, K, B.out_tets,
+  if (B.out_tets) {
+    k_vertex_ranks<<<nblocks((uint64_t)K.V + 1u), kBlock, 0, s>>>(M.pts, K.V + 1u, B.vrank);
+    k_output_rows<<<nb, kBlock, 0, s>>>(M, T, olist, okey, B.cstart, B.slot2pos, B.vrank, K, B.out_tets,
                                         B.out_adj);
+  }
   return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // Learned presizing kernels: per-cloud features, MLP inference, and the
 // forward / backward / Adam kernels that train the network on the GPU.
 // All FP32 except the feature moments (f64 accumulation).
+#include <algorithm>
 #include <cstring>
 
 #include "kernels.h"
@@ -31,14 +32,28 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 }  // namespace
 
-// histogram counts (u32) and coordinate moments (f64) per cloud
-__global__ void k_feat_accum(const float* xyz, uint32_t n, const uint32_t* off, uint32_t C,
-                             const uint32_t* mm, uint32_t* hist, double* mom) {
-  uint32_t i = gtid();
-  bool valid = i < n;
-  uint32_t c = (valid && C > 1) ? cloud_of_point(off, C, i) : 0u;
+// Histogram counts (u32) and coordinate moments (f64) per cloud.  Each
+// block accumulates the points of its first cloud in LDS (histogram and
+// moments) and adds them to the global arrays once at the end; points of
+// other clouds go straight to the global arrays.  One cloud: a grid-stride
+// loop over a few hundred blocks, so that the global adds are per block
+// and not per point (a global atomic per point on 512 words serializes at
+// the memory side).  Many clouds: one point per thread, so that a block's
+// points mostly share a cloud.
+__global__ void __launch_bounds__(kBlock) k_feat_accum(const float* xyz, uint32_t n, const uint32_t* off,
+                                                       uint32_t C, const uint32_t* mm, uint32_t* hist,
+                                                       double* mom) {
+  __shared__ uint32_t lh[kHistBins];
+  __shared__ double lm[kMom];
+  for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) lh[b] = 0u;
+  if (threadIdx.x < kMom) lm[threadIdx.x] = 0.0;
+  const uint32_t first = blockIdx.x * blockDim.x;
+  const uint32_t cb = (C > 1 && first < n) ? cloud_of_point(off, C, first) : 0u;
+  __syncthreads();
   double m[kMom] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (valid) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = gtid(); i < n; i += stride) {
+    uint32_t c = C > 1 ? cloud_of_point(off, C, i) : 0u;
     const uint32_t* b = mm + 6 * (size_t)c;
     float u[3];
     uint32_t q[3];
@@ -51,24 +66,27 @@ __global__ void k_feat_accum(const float* xyz, uint32_t n, const uint32_t* off, 
       q[k] = (uint32_t)(v * 1023.0f);
     }
     uint32_t code = (expand10(q[2]) << 2) | (expand10(q[1]) << 1) | expand10(q[0]);
-    atomicAdd(&hist[(size_t)c * kHistBins + (code >> 21)], 1u);
-    m[0] = u[0];
-    m[1] = u[1];
-    m[2] = u[2];
-    m[3] = (double)u[0] * u[0];
-    m[4] = (double)u[1] * u[1];
-    m[5] = (double)u[2] * u[2];
-    m[6] = (double)u[0] * u[1];
-    m[7] = (double)u[0] * u[2];
-    m[8] = (double)u[1] * u[2];
+    double pm[kMom] = {u[0], u[1], u[2], (double)u[0] * u[0], (double)u[1] * u[1], (double)u[2] * u[2],
+                       (double)u[0] * u[1], (double)u[0] * u[2], (double)u[1] * u[2]};
+    if (c == cb) {
+      atomicAdd(&lh[code >> 21], 1u);
+      for (int k = 0; k < kMom; ++k) m[k] += pm[k];
+    } else {
+      atomicAdd(&hist[(size_t)c * kHistBins + (code >> 21)], 1u);
+      for (int k = 0; k < kMom; ++k) atomicAdd(&mom[(size_t)c * kMom + k], pm[k]);
+    }
   }
-  // lanes in the cloud of lane 0 reduce over the wave; others add directly
-  uint32_t c0 = __shfl(c, 0);
-  bool same = c == c0;
+  // the block's own cloud: wave sums into LDS, then one global add each
   for (int k = 0; k < kMom; ++k) {
-    double s = wave_sum(same ? m[k] : 0.0);
-    if (__lane_id() == 0 && s != 0.0) atomicAdd(&mom[(size_t)c0 * kMom + k], s);
-    if (!same && valid) atomicAdd(&mom[(size_t)c * kMom + k], m[k]);
+    double w = wave_sum(m[k]);
+    if (__lane_id() == 0 && w != 0.0) atomicAdd(&lm[k], w);
+  }
+  __syncthreads();
+  if (first < n) {
+    for (int b = threadIdx.x; b < kHistBins; b += blockDim.x)
+      if (lh[b]) atomicAdd(&hist[(size_t)cb * kHistBins + b], lh[b]);
+    if (threadIdx.x < kMom && lm[threadIdx.x] != 0.0)
+      atomicAdd(&mom[(size_t)cb * kMom + threadIdx.x], lm[threadIdx.x]);
   }
 }
 
@@ -101,7 +119,8 @@ void launch_features(const float* xyz, uint32_t n, const uint32_t* off, uint32_t
   double* mom = acc + ((size_t)C * kHistBins + 1) / 2;
   launch_fill_u32(hist, 0u, (uint64_t)C * kHistBins, s);
   launch_fill_u64(reinterpret_cast<uint64_t*>(mom), 0ull, (uint64_t)C * kMom, s);
-  if (n) k_feat_accum<<<nblocks(n), kBlock, 0, s>>>(xyz, n, off, C, mm, hist, mom);
+  uint32_t grid = C > 1 ? nblocks(n) : std::min(nblocks(n), 512u);
+  if (n) k_feat_accum<<<grid, kBlock, 0, s>>>(xyz, n, off, C, mm, hist, mom);
   k_feat_finish<<<nblocks((uint64_t)C * kFeat), kBlock, 0, s>>>(off, C, hist, mom, feat);
 }
 

@@ -333,9 +333,11 @@ WT_HD uint32_t cloud_of_tet(const Mesh& M, uint32_t t, const Clouds& K) {
 // Output row j (cloud order): slot = olist[j] of cloud c = ocloud[j], rows of
 // cloud c start at cstart[c].  Vertex ids are cloud-local (rank; infinity
 // n_c + 4), adjacency (local row << 5) | face.  slot2pos: slot -> row.
+// vrank[v]: vertex v's rank (its input id within its cloud), one word per
+// vertex so that the four lookups of a row stay in a small array
 WT_HD void output_body(const Mesh& M, uint32_t j, const uint32_t* olist, const uint32_t* ocloud,
-                       const uint32_t* cstart, const uint32_t* slot2pos, const Clouds& K,
-                       uint32_t* out_tets, uint32_t* out_adj) {
+                       const uint32_t* cstart, const uint32_t* slot2pos, const uint32_t* vrank,
+                       const Clouds& K, uint32_t* out_tets, uint32_t* out_adj) {
   uint32_t t = olist[j];
   uint32_t c = K.C > 1 ? ocloud[j] : 0u;
   uint32_t base = cstart[c];
@@ -344,7 +346,7 @@ WT_HD void output_body(const Mesh& M, uint32_t j, const uint32_t* olist, const u
   uint4 ov, oa;
   for (int k = 0; k < 4; ++k) {
     uint32_t v = get4(tv, k);
-    set4(ov, k, v == K.V ? inf_id : pt_rank(M.pts[v]));
+    set4(ov, k, v == K.V ? inf_id : vrank[v]);
     uint32_t e = get4(to, k);
     uint32_t nr = slot2pos[opp_tet(e)];
     set4(oa, k, nr == kInvalid ? kInvalid : ((nr - base) << 5) | opp_face(e));

@@ -112,7 +112,9 @@ def make_cloud(spec: tuple):
     elif fam == "anisotropic":
         p = W.anisotropic(n, seed, k=extra[0], uniform_share=extra[1])
     elif fam == "sphere":
-        p = W.sphere(n, seed)
+        # co-spherical points leave many faces to the host repair (seconds to
+        # minutes above ~4e5 points); kept at sizes that finish quickly
+        p = W.sphere(min(n, 300_000), seed)
     elif fam == "shell":
         # thin shells send the host repair into whole-mesh regions (minutes
         # above ~1e5 points); the family is kept at sizes that finish quickly
@@ -172,6 +174,7 @@ def collect(ctx: "_core.Context", runs: int, seed: int, nmin: int = 100, nmax: i
             fam, p = futs.pop(r).result()
             if r + ahead < runs:
                 futs[r + ahead] = pool.submit(make_cloud, specs[r + ahead])
+            t_run = time.time()
             d = ctx.upload(p)
             try:
                 X[r] = ctx.presize_features(d, [0, len(p)])[0]
@@ -181,6 +184,8 @@ def collect(ctx: "_core.Context", runs: int, seed: int, nmin: int = 100, nmax: i
                 ctx.free(d)
             N[r] = len(p)
             fams.append(fam)
+            if log and time.time() - t_run > 1.5:
+                log(f"collect: run {r} took {time.time() - t_run:.1f} s: spec {specs[r]}")
             if log and (r < 20 or (r + 1) % 100 == 0):
                 log(f"collect: {r + 1}/{runs} runs, {time.time() - t0:.1f} s ({fam}, n={len(p)}, "
                     f"stuck={int(np.expm1(Y[r, 2]))})")
