@@ -207,7 +207,10 @@ void ensure_pinned(T*& p, size_t& cap, size_t need) {
   if (p && cap >= need) return;
   if (p) HIP_CHECK(hipHostFree(p));
   p = nullptr;
-  size_t nc = std::max(need, cap + cap / 2);
+  // slack: the rows of one input vary by a few hundred between calls (a
+  // non-Delaunay mesh of the same points has a different tet count), and a
+  // reallocation pre-faults hundreds of megabytes again
+  size_t nc = std::max(need + need / 8, cap + cap / 2);
   if (hipHostMalloc((void**)&p, nc * sizeof(T), hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     p = nullptr;
@@ -1081,8 +1084,9 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
 // region, so after kMaxFetches of them, or once a quarter of the mesh is on
 // the host, it returns false and the repair takes every row instead.  A
 // fetch costs a fraction of a millisecond at 1M points, the full download
-// and repair 10-15 ms.
-constexpr uint32_t kMaxFetches = 8;
+// and repair 10-15 ms at 1M and about 200 ms at 5M, where 5-6 fetches are
+// usual.
+constexpr uint32_t kMaxFetches = 32;
 
 bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   if (row >= M || c->present[row] || ++fetches > kMaxFetches) return false;
@@ -1092,7 +1096,9 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   uint32_t* seed = misc + kRepairRingsMax + 3;  // after the count and the ring bounds
   uint32_t key = row << 2;
   HIP_CHECK(hipMemcpyAsync(seed, &key, 4, hipMemcpyHostToDevice, c->stream));
-  launch_neighbourhood(seed, 1, M, 0, c->repair_rings, P<uint32_t>(c->fout), P<uint32_t>(c->nbr_mark),
+  // wider than the first staging: a region that left it is a large one
+  const int rings = std::min(c->repair_rings + 3, (int)kRepairRingsMax);
+  launch_neighbourhood(seed, 1, M, 0, rings, P<uint32_t>(c->fout), P<uint32_t>(c->nbr_mark),
                        P<uint32_t>(c->nbr_list), misc, misc + 1, true, c->stream);
   HIP_CHECK(hipGetLastError());
   uint32_t after = listed_rows(c, M);

@@ -14,10 +14,13 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <unordered_map>
 #include <unordered_set>
+#include <atomic>
+#include <thread>
 
 #include "common.h"
 
@@ -53,6 +56,49 @@ struct FaceHash {
     return (size_t)h;
   }
 };
+// The faces of a local triangulation in one open-addressing table (a
+// region's attempts reuse its storage): sorted vertex triple -> the two
+// sides, 4 * tet + face (-1: none yet).
+struct FaceTable {
+  std::vector<FaceKey> key;
+  std::vector<std::array<int64_t, 2>> side;
+  std::vector<uint8_t> used;
+  size_t mask = 0;
+  void reset(size_t faces) {
+    size_t cap = 64;
+    while (cap < 2 * faces) cap <<= 1;
+    if (key.size() < cap) {
+      key.resize(cap);
+      side.resize(cap);
+    }
+    used.assign(cap, 0);
+    mask = cap - 1;
+  }
+  size_t slot(const FaceKey& k) const {
+    uint64_t h = FaceHash()(k);
+    h ^= h >> 33;
+    h *= 0xFF51AFD7ED558CCDull;
+    h ^= h >> 33;
+    size_t i = (size_t)h & mask;
+    while (used[i] && !(key[i] == k)) i = (i + 1) & mask;
+    return i;
+  }
+  void add(const FaceKey& k, int64_t s) {
+    size_t i = slot(k);
+    if (!used[i]) {
+      used[i] = 1;
+      key[i] = k;
+      side[i] = {s, -1};
+    } else {
+      side[i][1] = s;
+    }
+  }
+  const std::array<int64_t, 2>* get(const FaceKey& k) const {
+    size_t i = slot(k);
+    return used[i] ? &side[i] : nullptr;
+  }
+};
+
 FaceKey face_key(uint32_t x, uint32_t y, uint32_t z) {
   uint32_t a[3] = {x, y, z};
   std::sort(a, a + 3);
@@ -68,6 +114,9 @@ struct Local {
   std::vector<uint8_t> alive;
   std::vector<int64_t> mark;
   std::vector<int64_t> freel;
+  int64_t last = 0;          // a tet of the last insertion (the next walk starts there)
+  int64_t stamp = 0;         // insertions so far (cavity marks)
+  uint32_t rng = 0x2545F491u;
 
   int64_t make() {
     if (!freel.empty()) {
@@ -97,8 +146,8 @@ uint32_t morton10(const float* p) {
   return (q(p[2]) << 2) | (q(p[1]) << 1) | q(p[0]);
 }
 
-bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Local& L,
-                    std::string& err) {
+// The Delaunay triangulation of the four super vertices alone.
+void local_init(const PointSet& S, int mode, Local& L) {
   uint32_t n = S.n;
   L = Local();
   int64_t t0 = L.make();
@@ -106,16 +155,21 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
   if (o3sos(S, s[0], s[1], s[2], s[3], mode) <= 0) std::swap(s[0], s[1]);
   std::memcpy(&L.v[0], s, 16);
   L.alive[t0] = 1;
-  // random insertion order (deterministic): a new point then mostly lies
-  // inside the hull of those already inserted, so its cavity stays small
-  std::sort(ids.begin(), ids.end());
-  uint64_t sh = 0x9E3779B97F4A7C15ull ^ ids.size();
-  for (size_t i = ids.size(); i > 1; --i) {
-    sh = sh * 6364136223846793005ull + 1442695040888963407ull;
-    std::swap(ids[i - 1], ids[(size_t)((sh >> 33) % i)]);
-  }
-  int64_t last = t0;
-  uint32_t rng = 0x2545F491u;
+  L.last = t0;
+}
+
+// Bowyer-Watson insertion of `ids` into L (initialised by local_init, maybe
+// holding earlier insertions: a region that grows inserts only its new
+// vertices).  Morton order: each visibility walk starts at the previous
+// point's new tets, a few steps away.
+bool local_insert(const PointSet& S, std::vector<uint32_t> ids, int mode, Local& L, std::string& err) {
+  std::vector<std::pair<uint32_t, uint32_t>> ord;
+  ord.reserve(ids.size());
+  for (uint32_t q : ids) ord.emplace_back(morton10(S.p(q)), q);
+  std::sort(ord.begin(), ord.end());
+  for (size_t i = 0; i < ord.size(); ++i) ids[i] = ord[i].second;
+  int64_t last = L.last;
+  uint32_t rng = L.rng;
   std::vector<int64_t> cav, stack;
   struct BF {
     uint32_t f[3];
@@ -164,7 +218,7 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
                             // (src/phase2.rs:440-443)
     }
     // cavity
-    int64_t stamp = (int64_t)it;
+    int64_t stamp = L.stamp++;
     cav.clear();
     stack.clear();
     L.mark[t] = stamp;
@@ -234,6 +288,8 @@ bool local_delaunay(const PointSet& S, std::vector<uint32_t> ids, int mode, Loca
       return false;
     }
   }
+  L.last = last;
+  L.rng = rng;
   return true;
 }
 
@@ -335,6 +391,262 @@ wtet_status repair_delaunay(const float* xyzn, uint32_t n, MeshBuf& mesh,
   }
 }
 
+namespace {
+
+// A region's replacement, computed from the mesh without writing to it, so
+// that the regions of distinct faces can be computed on several threads.
+enum : int { kPlanOk = 0, kPlanClean = 1, kPlanFetch = 2, kPlanFail = 3 };
+struct Plan {
+  uint32_t t = 0;
+  int f = 0;
+  int status = kPlanClean;
+  std::vector<uint32_t> R;      // the region's slots
+  Local L;                      // Delaunay triangulation of R's vertices and the super vertices
+  std::vector<int64_t> order;   // the tets of L inside R
+  std::unordered_map<FaceKey, uint32_t, FaceHash> bface;  // R's boundary faces -> outside neighbour
+  uint32_t missing_row = 0;     // kPlanFetch: a row the host lacks
+  wtet_status err = WTET_OK;    // kPlanFail
+  bool too_large = false;
+  std::string msg;
+};
+
+// The region of the face (t, f): R starts as t, its neighbour across f and
+// their neighbours, and grows by one ring of face-neighbours (and by the
+// tets across boundary faces that L lacks) until every boundary face of R
+// is a face of L; the plan is then the tets of L inside R.
+void plan_region(const PointSet& S, const Mesh32& W, uint32_t t, int f, int mode, uint32_t region_cap,
+                 Plan& P) {
+  P = Plan();
+  P.t = t;
+  P.f = f;
+  const uint32_t n = S.n;
+  try {
+    if (t >= W.count() || W.dead(t) || !face_bad(S, W, t, f, mode)) return;
+    std::unordered_set<uint32_t> inR;
+    std::vector<uint32_t>& R = P.R;
+    std::vector<uint32_t> ring;
+    auto add = [&](uint32_t x) {
+      if (x >= W.count() || W.dead(x) || W.has_inf(x) || inR.count(x)) return;
+      inR.insert(x);
+      R.push_back(x);
+    };
+    add(t);
+    add(W.ta(t, f) >> 5);
+    // the local triangulation grows with R: each attempt inserts only the
+    // vertices the previous attempts did not
+    Local& L = P.L;
+    local_init(S, mode, L);
+    std::vector<uint32_t> have, fresh, real;
+    FaceTable fmap;
+    for (;;) {
+      ring.clear();
+      for (uint32_t x : R)
+        for (int h = 0; h < 4; ++h)
+          if (W.ta(x, h) != kInvalid) ring.push_back(W.ta(x, h) >> 5);
+      for (uint32_t x : ring) add(x);
+      if (region_cap && R.size() > region_cap) {
+        P.status = kPlanFail;
+        P.err = WTET_ERR_NOT_CONVERGED;
+        P.too_large = true;
+        P.msg = "a repair region outgrew its cap";
+        return;
+      }
+      if (R.size() * 2 > W.count())
+        for (uint32_t x = 0; x < W.count(); ++x) add(x);
+      real.clear();
+      for (uint32_t x : R)
+        for (int k = 0; k < 4; ++k)
+          if (W.tv(x)[k] < n) real.push_back(W.tv(x)[k]);
+      std::sort(real.begin(), real.end());
+      real.erase(std::unique(real.begin(), real.end()), real.end());
+      fresh.clear();
+      std::set_difference(real.begin(), real.end(), have.begin(), have.end(), std::back_inserter(fresh));
+      have.swap(real);
+      std::string err;
+      if (!local_insert(S, fresh, mode, L, err)) {
+        P.status = kPlanFail;
+        P.err = WTET_ERR_INTERNAL;
+        P.msg = err;
+        return;
+      }
+      fmap.reset(2 * L.alive.size() + 16);
+      for (int64_t d = 0; d < (int64_t)L.alive.size(); ++d) {
+        if (!L.alive[d]) continue;
+        const uint32_t* dv = &L.v[4 * d];
+        for (int h = 0; h < 4; ++h)
+          fmap.add(face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]), 4 * d + h);
+      }
+      // boundary faces of R -> outside neighbour (t << 5 | f, or kInvalid)
+      auto& bface = P.bface;
+      bface.clear();
+      std::vector<int64_t> seeds;
+      std::vector<uint32_t> grow_with;
+      bool missing = false;
+      for (uint32_t x : R)
+        for (int h = 0; h < 4; ++h) {
+          uint32_t nb = W.ta(x, h);
+          if (nb != kInvalid && inR.count(nb >> 5)) continue;
+          if (nb != kInvalid) (void)W.dead(nb >> 5);  // written by the commit: must be present
+          const uint32_t* xv = W.tv(x);
+          uint32_t fa = xv[face_vi(h, 0)], fb = xv[face_vi(h, 1)], fc = xv[face_vi(h, 2)];
+          FaceKey k = face_key(fa, fb, fc);
+          const std::array<int64_t, 2>* it = fmap.get(k);
+          if (!it) {
+            missing = true;
+            if (nb != kInvalid) grow_with.push_back(nb >> 5);
+            continue;
+          }
+          bface.emplace(k, nb);
+          for (int s2 = 0; s2 < 2; ++s2) {
+            int64_t df = (*it)[s2];
+            if (df < 0) continue;
+            uint32_t apex = L.v[4 * (df >> 2) + (df & 3)];
+            if (o3sos(S, fa, fb, fc, apex, mode) > 0) seeds.push_back(df >> 2);
+          }
+        }
+      if (missing) {
+        for (uint32_t x : grow_with) add(x);
+        continue;
+      }
+      // inner tets of L: flood from the seeds, never across a boundary face
+      std::unordered_set<int64_t> inner;
+      std::vector<int64_t> st = seeds;
+      for (int64_t d : seeds)
+        if (inner.insert(d).second) P.order.push_back(d);
+      while (!st.empty()) {
+        int64_t d = st.back();
+        st.pop_back();
+        const uint32_t* dv = &L.v[4 * d];
+        for (int h = 0; h < 4; ++h) {
+          FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
+          if (bface.count(k)) continue;
+          int64_t nb = L.a[4 * d + h];
+          if (nb < 0) {
+            P.status = kPlanFail;
+            P.err = WTET_ERR_INTERNAL;
+            P.msg = "region flood reached the hull";
+            return;
+          }
+          if (inner.insert(nb >> 2).second) {
+            st.push_back(nb >> 2);
+            P.order.push_back(nb >> 2);
+          }
+        }
+      }
+      P.status = kPlanOk;
+      return;
+    }
+  } catch (const NotPresent& np) {
+    P.status = kPlanFetch;
+    P.missing_row = np.row;
+  }
+}
+
+// Writes a plan into the mesh: the tets of L inside R take R's slots, then
+// appended ones; R's leftover slots become holes.  Returns false when the
+// rows run out of room.  The boundary faces go to `recheck`.
+bool commit_plan(Mesh32& W, const Plan& P, std::vector<uint32_t>& holes, std::vector<uint32_t>& changed,
+                 std::vector<uint32_t>* touched, std::deque<std::pair<uint32_t, int>>& recheck,
+                 RepairReport* rep) {
+  MeshBuf& mesh = W.m;
+  const Local& L = P.L;
+  std::unordered_map<int64_t, uint32_t> slot;
+  slot.reserve(P.order.size());
+  size_t k = 0;
+  for (int64_t d : P.order) {
+    uint32_t sl;
+    if (k < P.R.size()) {
+      sl = P.R[k++];
+    } else {
+      if (mesh.count >= mesh.cap) return false;
+      sl = (uint32_t)mesh.count++;
+      for (int h = 0; h < 4; ++h) W.ta(sl, h) = kInvalid;
+    }
+    slot[d] = sl;
+    changed.push_back(sl);
+    if (touched) touched->push_back(sl);
+  }
+  for (; k < P.R.size(); ++k) {
+    W.tv(P.R[k])[0] = kInvalid;
+    holes.push_back(P.R[k]);
+    if (touched) touched->push_back(P.R[k]);
+  }
+  for (int64_t d : P.order) std::memcpy(W.tv(slot[d]), &L.v[4 * d], 16);
+  for (int64_t d : P.order) {
+    uint32_t sl = slot[d];
+    const uint32_t* dv = &L.v[4 * d];
+    for (int h = 0; h < 4; ++h) {
+      FaceKey fk = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
+      auto bf = P.bface.find(fk);
+      if (bf != P.bface.end()) {
+        uint32_t out = bf->second;
+        W.ta(sl, h) = out;
+        if (out != kInvalid) {
+          W.ta(out >> 5, (int)(out & 3u)) = (sl << 5) | (uint32_t)h;
+          changed.push_back(out >> 5);
+          if (touched) touched->push_back(out >> 5);
+        }
+        recheck.emplace_back(sl, h);
+      } else {
+        int64_t nb = L.a[4 * d + h];
+        W.ta(sl, h) = (slot.at(nb >> 2) << 5) | (uint32_t)(nb & 3);
+      }
+    }
+  }
+  if (rep) {
+    rep->regions++;
+    rep->max_region = std::max<uint32_t>(rep->max_region, (uint32_t)P.R.size());
+  }
+  return true;
+}
+
+// Centroids of the faces planned in a round, hashed on a grid of cells
+// `cell` wide (fixed per round); near(c, r): a planned face's centroid lies
+// within max(r, its own reach) of c, searched over the cells within two
+// cells of c (a face farther out than that is not found: it is planned
+// too, which costs a duplicate plan, not a wrong result).
+struct KnotGrid {
+  float cell = 0.f;
+  std::unordered_map<uint64_t, std::vector<std::array<float, 4>>> cells;
+  void clear() {
+    cells.clear();
+    cell = 0.f;
+  }
+  static int64_t ci(float x, float cell) { return (int64_t)std::floor(x / cell); }
+  static uint64_t key(int64_t x, int64_t y, int64_t z) {
+    return ((uint64_t)(x & 0x1FFFFF) << 42) | ((uint64_t)(y & 0x1FFFFF) << 21) | (uint64_t)(z & 0x1FFFFF);
+  }
+  bool near(const float* c, float r) const {
+    if (cells.empty() || !(cell > 0.f)) return false;
+    int64_t x = ci(c[0], cell), y = ci(c[1], cell), z = ci(c[2], cell);
+    for (int64_t dx = -2; dx <= 2; ++dx)
+      for (int64_t dy = -2; dy <= 2; ++dy)
+        for (int64_t dz = -2; dz <= 2; ++dz) {
+          auto it = cells.find(key(x + dx, y + dy, z + dz));
+          if (it == cells.end()) continue;
+          for (const auto& e : it->second) {
+            float d2 = (e[0] - c[0]) * (e[0] - c[0]) + (e[1] - c[1]) * (e[1] - c[1]) + (e[2] - c[2]) * (e[2] - c[2]);
+            float rr = std::max(r, e[3]);
+            if (d2 <= rr * rr) return true;
+          }
+        }
+    return false;
+  }
+  void insert(const float* c, float r) {
+    if (!(cell > 0.f)) cell = std::max(r, 1e-6f);
+    cells[key(ci(c[0], cell), ci(c[1], cell), ci(c[2], cell))].push_back({c[0], c[1], c[2], r});
+  }
+};
+
+// Host threads for the region plans of one round (the box gives a GPU's
+// process 16 cores)
+unsigned plan_threads(size_t jobs) {
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return (unsigned)std::min<size_t>(jobs, std::min(hw, 16u));
+}
+
+}  // namespace
+
 static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
                                const std::vector<uint32_t>& viol, int mode, RepairReport* rep,
                                uint32_t region_cap) {
@@ -347,191 +659,134 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
   mode = mode == 1 ? PRED_EXACT_ONLY : PRED_FILTERED;
   Mesh32 W{mesh, n + 4};
   std::vector<uint32_t>& changed = mesh.changed;
-
-  std::deque<std::pair<uint32_t, int>> queue;
-  for (uint32_t r : viol) queue.emplace_back(r >> 2, (int)(r & 3u));
   std::vector<uint32_t> holes;
-  std::unordered_set<uint32_t> inR;
-  std::vector<uint32_t> R, ring;
 
-  wtet_status fail = WTET_OK;
-  // one queued face: every row it reads is read before the first write, so
-  // a missing row can be fetched and the face processed again
-  auto process = [&](uint32_t t, int f) -> void {
-    if (t >= W.count() || W.dead(t) || !face_bad(S, W, t, f, mode)) return;
-    R.clear();
-    inR.clear();
-    auto add = [&](uint32_t x) {
-      if (x >= W.count() || W.dead(x) || W.has_inf(x) || inR.count(x)) return;
-      inR.insert(x);
-      R.push_back(x);
-    };
-    add(t);
-    add(W.ta(t, f) >> 5);
-    bool done = false;
-    while (!done) {
-      ring.clear();
-      for (uint32_t x : R)
-        for (int h = 0; h < 4; ++h)
-          if (W.ta(x, h) != kInvalid) ring.push_back(W.ta(x, h) >> 5);
-      for (uint32_t x : ring) add(x);
-      if (region_cap && R.size() > region_cap) {
-        if (rep) {
-          rep->too_large = true;
-          rep->message = "a repair region outgrew its cap";
-        }
-        fail = WTET_ERR_NOT_CONVERGED;
-        return;
+  auto failed = [&](const Plan& P) {
+    if (rep) {
+      rep->message = P.msg;
+      rep->too_large = rep->too_large || P.too_large;
+    }
+    return P.err;
+  };
+  // The serial path of one face: a plan, fetching the rows it lacks, then
+  // its commit.  Every row a plan reads is read before the commit writes.
+  std::deque<std::pair<uint32_t, int>> recheck;
+  Plan P1;
+  auto serial = [&](uint32_t t, int f, std::vector<uint32_t>* touched_out) -> wtet_status {
+    for (;;) {
+      plan_region(S, W, t, f, mode, region_cap, P1);
+      if (P1.status != kPlanFetch) break;
+      if (!mesh.fetch || !mesh.fetch(P1.missing_row)) throw NotPresent{P1.missing_row};
+      if (rep) rep->fetches++;
+    }
+    if (P1.status == kPlanFail) return failed(P1);
+    if (P1.status == kPlanOk && !commit_plan(W, P1, holes, changed, touched_out, recheck, rep)) {
+      if (rep) rep->message = "repair needs more room than the staging buffer has";
+      return WTET_ERR_OUT_OF_MEMORY;
+    }
+    return WTET_OK;
+  };
+
+  // Rounds: the queued faces that are still non-Delaunay, one per knot (a
+  // face within kKnotReach of its longest edge of a face already planned in
+  // the round, centroid to centroid, waits for the next round), are planned
+  // on several threads, then
+  // committed one by one in queue order.  A face whose rows the host lacks,
+  // and a plan whose region or outer neighbours an earlier commit of the
+  // round changed or that reached a missing row, go the serial way.  Each
+  // commit's boundary faces join the next round.
+  std::vector<std::pair<uint32_t, int>> cur;
+  for (uint32_t r : viol) cur.emplace_back(r >> 2, (int)(r & 3u));
+  std::vector<std::pair<uint32_t, int>> next, reps;
+  std::vector<Plan> plans;
+  std::unordered_set<uint32_t> touched;
+  std::vector<uint32_t> tl;
+  constexpr float kKnotReach = 6.f;
+  KnotGrid grid;
+  while (!cur.empty()) {
+    next.clear();
+    reps.clear();
+    grid.clear();
+    for (auto [t, f] : cur) {
+      bool bad = false, lacks = false;
+      try {
+        bad = t < W.count() && !W.dead(t) && face_bad(S, W, t, f, mode);
+      } catch (const NotPresent&) {
+        lacks = true;
       }
-      if (R.size() * 2 > W.count())
-        for (uint32_t x = 0; x < W.count(); ++x) add(x);
-      std::vector<uint32_t> real;
-      for (uint32_t x : R)
-        for (int k = 0; k < 4; ++k)
-          if (W.tv(x)[k] < n) real.push_back(W.tv(x)[k]);
-      std::sort(real.begin(), real.end());
-      real.erase(std::unique(real.begin(), real.end()), real.end());
-      Local L;
-      std::string err;
-      if (!local_delaunay(S, real, mode, L, err)) {
-        if (rep) rep->message = err;
-        fail = WTET_ERR_INTERNAL;
-        return;
-      }
-      std::unordered_map<FaceKey, std::array<int64_t, 2>, FaceHash> fmap;
-      fmap.reserve(4 * L.alive.size());
-      for (int64_t d = 0; d < (int64_t)L.alive.size(); ++d) {
-        if (!L.alive[d]) continue;
-        const uint32_t* dv = &L.v[4 * d];
-        for (int h = 0; h < 4; ++h) {
-          FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
-          auto it = fmap.find(k);
-          if (it == fmap.end()) fmap.emplace(k, std::array<int64_t, 2>{4 * d + h, -1});
-          else it->second[1] = 4 * d + h;
-        }
-      }
-      // boundary faces of R -> outside neighbour (t << 5 | f, or kInvalid)
-      std::unordered_map<FaceKey, uint32_t, FaceHash> bface;
-      std::vector<int64_t> seeds;
-      std::vector<uint32_t> grow_with;
-      bool missing = false;
-      for (uint32_t x : R)
-        for (int h = 0; h < 4; ++h) {
-          uint32_t nb = W.ta(x, h);
-          if (nb != kInvalid && inR.count(nb >> 5)) continue;
-          if (nb != kInvalid) (void)W.dead(nb >> 5);  // written below: must be present
-          const uint32_t* xv = W.tv(x);
-          uint32_t fa = xv[face_vi(h, 0)], fb = xv[face_vi(h, 1)], fc = xv[face_vi(h, 2)];
-          FaceKey k = face_key(fa, fb, fc);
-          auto it = fmap.find(k);
-          if (it == fmap.end()) {
-            missing = true;
-            if (nb != kInvalid) grow_with.push_back(nb >> 5);
-            continue;
-          }
-          bface.emplace(k, nb);
-          for (int s2 = 0; s2 < 2; ++s2) {
-            int64_t df = it->second[s2];
-            if (df < 0) continue;
-            uint32_t apex = L.v[4 * (df >> 2) + (df & 3)];
-            if (o3sos(S, fa, fb, fc, apex, mode) > 0) seeds.push_back(df >> 2);
-          }
-        }
-      if (missing) {
-        for (uint32_t x : grow_with) add(x);
+      if (lacks) {
+        // before this round's plans are computed, so they see its commit
+        wtet_status st = serial(t, f, nullptr);
+        if (st != WTET_OK) return st;
+        for (auto& q : recheck) next.push_back(q);
+        recheck.clear();
         continue;
       }
-      // inner tets of D
-      std::unordered_map<int64_t, uint32_t> inner;
-      std::vector<int64_t> order;
-      std::vector<int64_t> st = seeds;
-      for (int64_t d : seeds)
-        if (inner.emplace(d, kInvalid).second) order.push_back(d);
-      while (!st.empty()) {
-        int64_t d = st.back();
-        st.pop_back();
-        const uint32_t* dv = &L.v[4 * d];
-        for (int h = 0; h < 4; ++h) {
-          FaceKey k = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
-          if (bface.count(k)) continue;
-          int64_t nb = L.a[4 * d + h];
-          if (nb < 0) {
-            if (rep) rep->message = "region flood reached the hull";
-            fail = WTET_ERR_INTERNAL;
-            return;
-          }
-          if (inner.emplace(nb >> 2, kInvalid).second) {
-            st.push_back(nb >> 2);
-            order.push_back(nb >> 2);
-          }
-        }
+      if (!bad) continue;
+      // stuck faces come in knots a few tets across: a face near one
+      // already planned waits
+      const uint32_t* tv = W.tv(t);
+      const float* q[3] = {S.p(tv[face_vi(f, 0)]), S.p(tv[face_vi(f, 1)]), S.p(tv[face_vi(f, 2)])};
+      float c[3], len2 = 0.f;
+      for (int k = 0; k < 3; ++k) {
+        c[k] = (q[0][k] + q[1][k] + q[2][k]) / 3.f;
+        const float* a = q[k];
+        const float* b = q[(k + 1) % 3];
+        float d2 = (a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]);
+        len2 = std::max(len2, d2);
       }
-      // slots: R's slots first, then appended ones; leftovers become holes
-      size_t k = 0;
-      for (int64_t d : order) {
-        uint32_t slot;
-        if (k < R.size()) {
-          slot = R[k++];
-        } else {
-          if (mesh.count >= mesh.cap) {
-            if (rep) rep->message = "repair needs more room than the staging buffer has";
-            fail = WTET_ERR_OUT_OF_MEMORY;
-            return;
-          }
-          slot = (uint32_t)mesh.count++;
-          for (int h = 0; h < 4; ++h) W.ta(slot, h) = kInvalid;
-        }
-        inner[d] = slot;
-        changed.push_back(slot);
+      const float reach = kKnotReach * std::sqrt(len2);
+      if (grid.near(c, reach)) {
+        next.emplace_back(t, f);
+        continue;
       }
-      for (; k < R.size(); ++k) {
-        W.tv(R[k])[0] = kInvalid;
-        holes.push_back(R[k]);
-      }
-      for (int64_t d : order) {
-        uint32_t slot = inner[d];
-        std::memcpy(W.tv(slot), &L.v[4 * d], 16);
-      }
-      for (int64_t d : order) {
-        uint32_t slot = inner[d];
-        const uint32_t* dv = &L.v[4 * d];
-        for (int h = 0; h < 4; ++h) {
-          FaceKey fk = face_key(dv[face_vi(h, 0)], dv[face_vi(h, 1)], dv[face_vi(h, 2)]);
-          auto bf = bface.find(fk);
-          if (bf != bface.end()) {
-            uint32_t out = bf->second;
-            W.ta(slot, h) = out;
-            if (out != kInvalid) {
-              W.ta(out >> 5, (int)(out & 3u)) = (slot << 5) | (uint32_t)h;
-              changed.push_back(out >> 5);
-            }
-            queue.emplace_back(slot, h);
-          } else {
-            int64_t nb = L.a[4 * d + h];
-            W.ta(slot, h) = (inner.at(nb >> 2) << 5) | (uint32_t)(nb & 3);
-          }
-        }
-      }
-      if (rep) {
-        rep->regions++;
-        rep->max_region = std::max<uint32_t>(rep->max_region, (uint32_t)R.size());
-      }
-      done = true;
+      grid.insert(c, reach);
+      reps.emplace_back(t, f);
+
     }
-  };
-  while (!queue.empty()) {
-    auto [t, f] = queue.front();
-    for (;;) {
-      try {
-        process(t, f);
-        break;
-      } catch (const NotPresent& np) {
-        if (!mesh.fetch || !mesh.fetch(np.row)) throw;
-        if (rep) rep->fetches++;
-      }
+    plans.assign(reps.size(), Plan());
+    unsigned nt = plan_threads(reps.size());
+    if (nt <= 1) {
+      for (size_t i = 0; i < reps.size(); ++i)
+        plan_region(S, W, reps[i].first, reps[i].second, mode, region_cap, plans[i]);
+    } else {
+      std::atomic<size_t> nextjob{0};
+      std::vector<std::thread> th;
+      for (unsigned k = 0; k < nt; ++k)
+        th.emplace_back([&] {
+          for (size_t i; (i = nextjob.fetch_add(1)) < reps.size();)
+            plan_region(S, W, reps[i].first, reps[i].second, mode, region_cap, plans[i]);
+        });
+      for (auto& x : th) x.join();
     }
-    if (fail != WTET_OK) return fail;
-    queue.pop_front();
+    touched.clear();
+    for (size_t i = 0; i < plans.size(); ++i) {
+      const Plan& P = plans[i];
+      if (P.status == kPlanClean) continue;
+      if (P.status == kPlanFail) return failed(P);
+      bool stale = P.status == kPlanFetch;
+      for (size_t k = 0; !stale && k < P.R.size(); ++k) stale = touched.count(P.R[k]) > 0;
+      for (auto it = P.bface.begin(); !stale && it != P.bface.end(); ++it)
+        stale = it->second != kInvalid && touched.count(it->second >> 5) > 0;
+      if (stale) {
+        tl.clear();
+        wtet_status st = serial(P.t, P.f, &tl);
+        if (st != WTET_OK) return st;
+        touched.insert(tl.begin(), tl.end());
+        for (auto& [t2, f2] : recheck) next.emplace_back(t2, f2);
+        recheck.clear();
+        continue;
+      }
+      tl.clear();
+      if (!commit_plan(W, P, holes, changed, &tl, recheck, rep)) {
+        if (rep) rep->message = "repair needs more room than the staging buffer has";
+        return WTET_ERR_OUT_OF_MEMORY;
+      }
+      touched.insert(tl.begin(), tl.end());
+      for (auto& [t2, f2] : recheck) next.emplace_back(t2, f2);
+      recheck.clear();
+    }
+    cur.swap(next);
   }
   fill_holes(W, holes, changed);
   std::sort(changed.begin(), changed.end());
