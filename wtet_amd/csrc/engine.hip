@@ -85,6 +85,11 @@ struct wtet_ctx {
   uint32_t* h_tets = nullptr;
   uint32_t* h_adj = nullptr;
   float* h_xyzn = nullptr;
+  // the normalized points go to h_xyzn on copy_stream while the device
+  // phase runs (the host repair reads them); ev_xyzn marks the copy's end
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_norm = nullptr, ev_xyzn = nullptr;
+  uint32_t xyzn_copied = 0;  // points of the copy in flight or done (0: none)
   // per-cloud rows of the last resident batch result
   std::vector<uint64_t> bstart;
   std::vector<uint32_t> bcount;
@@ -552,6 +557,10 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
     c->h_pred.resize((size_t)C * kOut);
   }
   R.st = R.cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
+  // the previous call's copy may still run into the buffer being replaced
+  HIP_CHECK(hipStreamSynchronize(c->copy_stream));
+  c->xyzn_copied = 0;
+  ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
 
   HIP_CHECK(hipMemcpyAsync(c->doff.p, off.data(), ((size_t)C + 1) * 4, hipMemcpyHostToDevice, s));
   Clouds& CL = R.CL;
@@ -570,6 +579,13 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
     // sort, gather into the vertex layout
     launch_prepare(d_xyz, n, CL, P<uint32_t>(c->mm6), P<float>(c->xyzn), P<uint64_t>(c->keys),
                    P<uint32_t>(c->ids), s);
+    // the normalized points are final: their copy to the host overlaps
+    // the rest of the device phase
+    HIP_CHECK(hipEventRecord(c->ev_norm, s));
+    HIP_CHECK(hipStreamWaitEvent(c->copy_stream, c->ev_norm, 0));
+    HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->copy_stream));
+    HIP_CHECK(hipEventRecord(c->ev_xyzn, c->copy_stream));
+    c->xyzn_copied = n;
     HIP_CHECK(sort_pairs(P<uint64_t>(c->keys), P<uint32_t>(c->ids), P<uint64_t>(c->keys2),
                          P<uint32_t>(c->ids2), n, R.end_bit, c->sort_tmp.p, c->sort_tmp.bytes, s));
     launch_gather_points(P<float>(c->xyzn), P<uint32_t>(c->ids2), P<uint64_t>(c->keys2), n, CL,
@@ -1118,6 +1134,20 @@ void download_rows(wtet_ctx* c, uint32_t M, uint32_t left = 0, size_t min_cap = 
   c->stats.repair_rows_staged = M;
 }
 
+// the normalized points on the host (h_xyzn): the copy dev_prepare started,
+// or one now
+void host_points(wtet_ctx* c, uint32_t n) {
+  if (c->xyzn_copied == n) {
+    HIP_CHECK(hipEventSynchronize(c->ev_xyzn));
+    return;
+  }
+  HIP_CHECK(hipStreamSynchronize(c->copy_stream));
+  ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
+  HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->xyzn_copied = n;
+}
+
 // failed_verts (gather.wgsl:68-151): the vertex opposite each non-Delaunay
 // face between two real-only tets, from violation records (row << 2 | face)
 std::vector<uint32_t> failed_vertices(const uint32_t* tets, const uint32_t* adj,
@@ -1179,9 +1209,7 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
   bool repaired = false;
   if ((nviol > 0 || !missing.empty()) && cfg.enable_splaying) {
     // the repair needs the normalized coordinates of the input ids
-    ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
-    HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    host_points(c, n);
   }
   // The repair and the insertion write into the staging rows.  The Delaunay
   // mesh may need many more tets than the rows hold (a mesh from splits
@@ -1454,9 +1482,7 @@ void batch_impl(void* a_) {
   // the cap left, on a copy of its rows
   std::vector<CloudRows> rows(C);
   if ((nviol > 0 || K.left > 0) && a.cfg.enable_splaying) {
-    ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
-    HIP_CHECK(hipMemcpyAsync(c->h_xyzn, c->xyzn.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    host_points(c, n);
     // clouds are independent: repair them on several host threads
     struct Job {
       uint32_t cl;
@@ -1653,8 +1679,11 @@ wtet_status wtet_create(int device, wtet_ctx** out) {
   if (!c) return WTET_ERR_OUT_OF_MEMORY;
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipStreamCreate(&c->copy_stream) != hipSuccess ||
       hipHostMalloc((void**)&c->h_ctr, C_COUNT * 4, hipHostMallocDefault) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_norm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_xyzn, hipEventDisableTiming) != hipSuccess ||
       resolve_scratch(device) != hipSuccess) {
     (void)hipGetLastError();
     delete c;
@@ -1669,6 +1698,7 @@ void wtet_destroy(wtet_ctx* c) {
   end_stage(c);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   for (auto& g : c->round_graphs) (void)hipGraphExecDestroy(g.exec);
   for (DevBuf* b : c->all)
@@ -1684,6 +1714,9 @@ void wtet_destroy(wtet_ctx* c) {
   }
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_norm) (void)hipEventDestroy(c->ev_norm);
+  if (c->ev_xyzn) (void)hipEventDestroy(c->ev_xyzn);
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
