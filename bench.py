@@ -34,7 +34,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 #   flip_update: per executed flip 48 (six external back-pointers 24 and
 #     their messages 24); decide_exec + update = 24 per check + 256 per flip
 #   flip_small: the three above for the short-list iterations
-#   split: per insertion 224; audit: per tet slot 200; output: per tet 32
+#   split: per insertion 224; output: per tet 32
+#   audit: per live tet 124 (quad 16, adjacency 32, info 4, its four points
+#     48, the opposite points of the two faces it owns 24); per slot a 3-2
+#     flip freed, its info word 4
 def kernel_bytes(st):
     a_big = st["sum_active"] - st["small_checked"]
     f_big = st["sum_flips"] - st["small_flips"]
@@ -45,7 +48,7 @@ def kernel_bytes(st):
         "flip_update": 48 * f_big,
         "flip_small": 224 * st["small_checked"] + 256 * st["small_flips"],
         "split": 224 * inserted,
-        "audit": 200 * st["tet_slots"],
+        "audit": 124 * st["num_tets_out"] + 4 * max(st["tet_slots"] - st["num_tets_out"], 0),
         "output": 32 * st["num_tets_out"],
     }
 
@@ -111,6 +114,10 @@ def main():
                     help="launch the flip iterations directly instead of replaying HIP graphs "
                          "(the same kernels; used under rocprofv3)")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--config1-only", action="store_true",
+                    help="only the 10,000-point Hilbert workload, exact only then filtered: 2 warmup "
+                         "and 5 timed resident calls each, one JSON line per mode (the program "
+                         "profiled for that workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (5 launches), "
                          "the known-size reference for HBM byte counters")
@@ -136,6 +143,28 @@ def main():
         ctx.free(dst)
         print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(dt * 1e3, 3),
                           "GBs_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+        return
+
+    if args.config1_only:
+        ctx = wtet_amd.Context(0)
+        p1 = W.uniform(10_000, 0)
+        d_1 = ctx.upload(p1)
+        for mode in ("exact", "filtered"):
+            cfg = wtet_amd.Config(predicate_mode=mode, enable_hilbert_sorting=True, collect_stats=True)
+            for _ in range(2):
+                ctx.delaunay_resident(d_1, len(p1), cfg)
+            t = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                ctx.delaunay_resident(d_1, len(p1), cfg)
+                t.append((time.perf_counter() - t0) * 1e3)
+            s1 = ctx.stats()
+            print(json.dumps({"workload": "uniform_10000_seed0", "order": "hilbert", "predicates": mode,
+                              "ms": [round(x, 2) for x in t], "ms_device": round(s1["ms_device"], 3),
+                              "rounds": s1["rounds"], "flip_iterations": s1["flip_iterations"],
+                              "checked_tets": s1["sum_active"], "violations_gpu": s1["violations_gpu"]}),
+                  flush=True)
+        ctx.free(d_1)
         return
 
     pts = W.uniform(args.n, args.seed)
