@@ -423,8 +423,9 @@ class Context:
         return int(b[0]), int(b[1])
 
     def autotune(self, ptr: int, offsets, config: "Config | None" = None):
-        """Time 1, 2, 4 and 8 list entries per thread on these device points;
-        keeps the fastest.  Returns (entries per thread, device ms)."""
+        """Time 1, 2, 4 and 8 list entries per thread on these device points
+        (median of three calls each, after one that captures the graphs);
+        keeps the fastest.  Returns (entries per thread, median device ms)."""
         off = _check_offsets(offsets)
         self._prepare(config)
         c = config._c() if config is not None else Config()._c()
