@@ -528,6 +528,11 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
   hipStream_t s = c->stream;
   const uint32_t C = R.C, n = R.n, V = R.V;
   const std::vector<uint32_t>& off = R.off;
+  // the previous call's copy of the normalized points may still read xyzn
+  // (a call that needed no repair never waits for it) and write h_xyzn:
+  // both may be replaced below
+  HIP_CHECK(hipStreamSynchronize(c->copy_stream));
+  c->xyzn_copied = 0;
   ensure(c, c->xyzn, (size_t)n * 12);
   ensure(c, c->keys, (size_t)n * 8);
   ensure(c, c->ids, (size_t)n * 4);
@@ -557,9 +562,6 @@ void dev_prepare(wtet_ctx* c, Run& R, const float* d_xyz) {
     c->h_pred.resize((size_t)C * kOut);
   }
   R.st = R.cfg.collect_stats ? P<uint64_t>(c->gstats) : nullptr;
-  // the previous call's copy may still run into the buffer being replaced
-  HIP_CHECK(hipStreamSynchronize(c->copy_stream));
-  c->xyzn_copied = 0;
   ensure_pinned(c->h_xyzn, c->h_npts, (size_t)n * 3);
 
   HIP_CHECK(hipMemcpyAsync(c->doff.p, off.data(), ((size_t)C + 1) * 4, hipMemcpyHostToDevice, s));
