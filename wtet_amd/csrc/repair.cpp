@@ -20,6 +20,7 @@
 #include <unordered_map>
 #include <unordered_set>
 #include <atomic>
+#include <new>
 #include <thread>
 
 #include "common.h"
@@ -754,8 +755,21 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
       std::vector<std::thread> th;
       for (unsigned k = 0; k < nt; ++k)
         th.emplace_back([&] {
-          for (size_t i; (i = nextjob.fetch_add(1)) < reps.size();)
-            plan_region(S, W, reps[i].first, reps[i].second, mode, region_cap, plans[i]);
+          for (size_t i; (i = nextjob.fetch_add(1)) < reps.size();) {
+            // an exception must not leave a worker thread (it would end the
+            // process): it becomes the plan's failure
+            try {
+              plan_region(S, W, reps[i].first, reps[i].second, mode, region_cap, plans[i]);
+            } catch (const std::bad_alloc&) {
+              plans[i].status = kPlanFail;
+              plans[i].err = WTET_ERR_OUT_OF_MEMORY;
+              plans[i].msg = "host memory ran out while planning a repair region";
+            } catch (...) {
+              plans[i].status = kPlanFail;
+              plans[i].err = WTET_ERR_INTERNAL;
+              plans[i].msg = "a repair region's plan failed";
+            }
+          }
         });
       for (auto& x : th) x.join();
     }
