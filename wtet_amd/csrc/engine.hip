@@ -65,7 +65,7 @@ struct wtet_ctx {
   // persistent grid, and the captured graph of one round's iterations
   DevBuf isums, slow_t, slow_p;
   // neighbourhood of the host repair (sparse download)
-  DevBuf nbr_mark, nbr_list, nbr_misc, nbr_pack;
+  DevBuf nbr_mark, nbr_list, nbr_misc, nbr_pack, nbr_seed;
   uint32_t items = 2;  // list entries per thread the persistent grids are sized for
   uint32_t audit_records = 1u << 20;  // violation records the first audit of a call keeps
   bool use_graphs = true;
@@ -126,7 +126,7 @@ struct wtet_ctx {
                    &wlist,  &act[0], &act[1], &prop,  &fout,   &blk,      &slot2idx, &rec,
                    &ord[0], &ord[1], &ord_tmp, &cstart, &ctr,  &gstats,   &patch,
                    &presize_params, &feat_acc, &feat, &pred, &isums, &slow_t, &slow_p,
-                   &nbr_mark, &nbr_list, &nbr_misc, &nbr_pack};
+                   &nbr_mark, &nbr_list, &nbr_misc, &nbr_pack, &nbr_seed};
     all.assign(std::begin(l), std::end(l));
   }
 };
@@ -1124,6 +1124,31 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   return c->present[row] != 0;
 }
 
+// fetch_rows for every row of `rows` the host lacks, in one pass: one
+// neighbourhood launch seeded by all of them, one copy (one fetch of the
+// kMaxFetches)
+bool fetch_rows_many(wtet_ctx* c, const std::vector<uint32_t>& rows, uint32_t M, uint32_t& fetches) {
+  std::vector<uint32_t> seeds;
+  for (uint32_t r : rows)
+    if (r < M && !c->present[r]) seeds.push_back(r << 2);
+  if (seeds.empty()) return true;
+  if (++fetches > kMaxFetches) return false;
+  uint32_t before = listed_rows(c, M);
+  if (before > M / 4) return false;
+  ensure(c, c->nbr_seed, seeds.size() * 4 + 16);
+  HIP_CHECK(hipMemcpyAsync(c->nbr_seed.p, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, c->stream));
+  uint32_t* misc = P<uint32_t>(c->nbr_misc);
+  const int rings = std::min(c->repair_rings + 3, (int)kRepairRingsMax);
+  launch_neighbourhood(P<uint32_t>(c->nbr_seed), (uint32_t)seeds.size(), M, 0, rings, P<uint32_t>(c->fout),
+                       P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), misc, misc + 1, true, c->stream);
+  HIP_CHECK(hipGetLastError());
+  uint32_t after = listed_rows(c, M);  // synchronizes: seeds stay alive until then
+  scatter_rows(c, before, after);
+  for (uint32_t r : rows)
+    if (r < M && !c->present[r]) return false;
+  return true;
+}
+
 void download_rows(wtet_ctx* c, uint32_t M, uint32_t left = 0, size_t min_cap = 0) {
   size_t cap = std::max<size_t>((size_t)M + M / 32 + 65536 + 8ull * left, min_cap);
   size_t c1 = c->h_cap, c2 = c->h_cap;
@@ -1200,6 +1225,9 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     mb.present = c->present.data();
     mb.known = mout;
     mb.fetch = [c, mout, &fetches](uint32_t row) { return fetch_rows(c, row, mout, fetches); };
+    mb.fetch_many = [c, mout, &fetches](const std::vector<uint32_t>& rows) {
+      return fetch_rows_many(c, rows, mout, fetches);
+    };
   }
   bool on_host = !resident || nviol > 0 || left > 0;
   std::vector<uint32_t> failed;

@@ -692,28 +692,39 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
   // face within kKnotReach of its longest edge of a face already planned in
   // the round, centroid to centroid, waits for the next round), are planned
   // on several threads, then
-  // committed one by one in queue order.  A face whose rows the host lacks,
-  // and a plan whose region or outer neighbours an earlier commit of the
-  // round changed or that reached a missing row, go the serial way.  Each
-  // commit's boundary faces join the next round.
+  // committed one by one in queue order.  A plan whose region or outer
+  // neighbours an earlier commit of the round changed goes the serial way.
+  // A face whose rows the host lacks, and a plan that reached a missing
+  // row, wait for the next round; the rows they reached are fetched
+  // together after the round's commits (with fetch_many; without it they go
+  // the serial way, which fetches one row at a time).  Each commit's
+  // boundary faces join the next round.
   std::vector<std::pair<uint32_t, int>> cur;
   for (uint32_t r : viol) cur.emplace_back(r >> 2, (int)(r & 3u));
   std::vector<std::pair<uint32_t, int>> next, reps;
   std::vector<Plan> plans;
   std::unordered_set<uint32_t> touched;
-  std::vector<uint32_t> tl;
+  std::vector<uint32_t> tl, want;
   constexpr float kKnotReach = 6.f;
   KnotGrid grid;
   while (!cur.empty()) {
     next.clear();
     reps.clear();
     grid.clear();
+    want.clear();
     for (auto [t, f] : cur) {
       bool bad = false, lacks = false;
+      uint32_t row = 0;
       try {
         bad = t < W.count() && !W.dead(t) && face_bad(S, W, t, f, mode);
-      } catch (const NotPresent&) {
+      } catch (const NotPresent& np) {
         lacks = true;
+        row = np.row;
+      }
+      if (lacks && mesh.fetch_many) {
+        want.push_back(row);
+        next.emplace_back(t, f);
+        continue;
       }
       if (lacks) {
         // before this round's plans are computed, so they see its commit
@@ -778,6 +789,11 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
       const Plan& P = plans[i];
       if (P.status == kPlanClean) continue;
       if (P.status == kPlanFail) return failed(P);
+      if (P.status == kPlanFetch && mesh.fetch_many) {
+        want.push_back(P.missing_row);
+        next.emplace_back(P.t, P.f);
+        continue;
+      }
       bool stale = P.status == kPlanFetch;
       for (size_t k = 0; !stale && k < P.R.size(); ++k) stale = touched.count(P.R[k]) > 0;
       for (auto it = P.bface.begin(); !stale && it != P.bface.end(); ++it)
@@ -799,6 +815,12 @@ static wtet_status repair_impl(const float* xyzn, uint32_t n, MeshBuf& mesh,
       touched.insert(tl.begin(), tl.end());
       for (auto& [t2, f2] : recheck) next.emplace_back(t2, f2);
       recheck.clear();
+    }
+    if (!want.empty()) {
+      std::sort(want.begin(), want.end());
+      want.erase(std::unique(want.begin(), want.end()), want.end());
+      if (!mesh.fetch_many(want)) throw NotPresent{want[0]};
+      if (rep) rep->fetches++;
     }
     cur.swap(next);
   }

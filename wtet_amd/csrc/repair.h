@@ -42,6 +42,10 @@ struct MeshBuf {
   uint8_t* present = nullptr;
   size_t known = 0;
   std::function<bool(uint32_t)> fetch;
+  // optional: the rows around every listed row in one pass (one device
+  // round trip for all the rows a round of region plans reached); returns
+  // false when it cannot make them present
+  std::function<bool(const std::vector<uint32_t>&)> fetch_many;
 };
 
 // xyzn: n normalized points (input id order).  viol: (t << 2) | f of
