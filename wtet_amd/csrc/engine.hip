@@ -1115,7 +1115,7 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   uint32_t key = row << 2;
   HIP_CHECK(hipMemcpyAsync(seed, &key, 4, hipMemcpyHostToDevice, c->stream));
   // wider than the first staging: a region that left it is a large one
-  const int rings = std::min(c->repair_rings + 3, (int)kRepairRingsMax);
+  const int rings = std::min(c->repair_rings + 6, (int)kRepairRingsMax);
   launch_neighbourhood(seed, 1, M, 0, rings, P<uint32_t>(c->fout), P<uint32_t>(c->nbr_mark),
                        P<uint32_t>(c->nbr_list), misc, misc + 1, true, c->stream);
   HIP_CHECK(hipGetLastError());
@@ -1138,7 +1138,7 @@ bool fetch_rows_many(wtet_ctx* c, const std::vector<uint32_t>& rows, uint32_t M,
   ensure(c, c->nbr_seed, seeds.size() * 4 + 16);
   HIP_CHECK(hipMemcpyAsync(c->nbr_seed.p, seeds.data(), seeds.size() * 4, hipMemcpyHostToDevice, c->stream));
   uint32_t* misc = P<uint32_t>(c->nbr_misc);
-  const int rings = std::min(c->repair_rings + 3, (int)kRepairRingsMax);
+  const int rings = std::min(c->repair_rings + 6, (int)kRepairRingsMax);
   launch_neighbourhood(P<uint32_t>(c->nbr_seed), (uint32_t)seeds.size(), M, 0, rings, P<uint32_t>(c->fout),
                        P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), misc, misc + 1, true, c->stream);
   HIP_CHECK(hipGetLastError());
