@@ -106,10 +106,12 @@ struct wtet_ctx {
   // the largest cloud (warmup_rounds)
   uint32_t ring0_rounds = 0xFFFFFFFFu;
   int repair_rings = 7;       // face-steps of rows staged around each stuck face
-  // rows of the host mirror that hold staged rows (sparse repair); kept
-  // across calls and cleared entry by entry (present_rows lists the set ones)
+  // rows of the host mirror that hold staged rows (sparse repair): those
+  // with present[r] == present_mark.  Each call takes the next mark instead
+  // of clearing the rows the previous one staged; the array is cleared once
+  // every 255 calls, when the mark wraps.
   std::vector<uint8_t> present;
-  std::vector<uint32_t> present_rows;
+  uint8_t present_mark = 0;
   // a single-cloud repair region above this many tets stops the repair and
   // reruns the device phase with a wider insertion ring (0: never)
   uint32_t region_cap = 1u << 14;
@@ -1041,13 +1043,14 @@ void scatter_rows(wtet_ctx* c, uint32_t from, uint32_t to) {
   uint32_t* ht = c->h_tets;
   uint32_t* ha = c->h_adj;
   uint8_t* pres = c->present.data();
-  auto part = [hp, ht, ha, pres](size_t a, size_t b) {
+  const uint8_t mark = c->present_mark;
+  auto part = [hp, ht, ha, pres, mark](size_t a, size_t b) {
     for (size_t i = a; i < b; ++i) {
       const uint32_t* q = hp + 9 * i;
       uint32_t r = q[0];
       std::memcpy(ht + 4 * (size_t)r, q + 1, 16);
       std::memcpy(ha + 4 * (size_t)r, q + 5, 16);
-      pres[r] = 1;
+      pres[r] = mark;
     }
   };
   unsigned nt = n >= (1u << 15) ? 8u : 1u;
@@ -1060,7 +1063,6 @@ void scatter_rows(wtet_ctx* c, uint32_t from, uint32_t to) {
       th.emplace_back(part, std::min<size_t>(n, per * k), std::min<size_t>(n, per * (k + 1)));
     for (auto& t : th) t.join();
   }
-  for (uint32_t i = 0; i < n; ++i) c->present_rows.push_back(hp[9 * (size_t)i]);
   c->stats.repair_rows_staged += n;
 }
 
@@ -1089,10 +1091,11 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
                        P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), cnt, cnt + 1, false, s);
   HIP_CHECK(hipGetLastError());
   uint32_t n = listed_rows(c, M);
-  // clear what the previous call marked instead of zeroing M bytes afresh
-  for (uint32_t r : c->present_rows) c->present[r] = 0;
-  c->present_rows.clear();
   if (c->present.size() < M) c->present.resize(M, 0);
+  if (++c->present_mark == 0) {
+    std::fill(c->present.begin(), c->present.end(), 0);
+    c->present_mark = 1;
+  }
   c->stats.repair_rows_staged = 0;
   scatter_rows(c, 0, n);
 }
@@ -1107,7 +1110,7 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
 constexpr uint32_t kMaxFetches = 32;
 
 bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
-  if (row >= M || c->present[row] || ++fetches > kMaxFetches) return false;
+  if (row >= M || c->present[row] == c->present_mark || ++fetches > kMaxFetches) return false;
   uint32_t before = listed_rows(c, M);
   if (before > M / 4) return false;
   uint32_t* misc = P<uint32_t>(c->nbr_misc);
@@ -1121,7 +1124,7 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
   HIP_CHECK(hipGetLastError());
   uint32_t after = listed_rows(c, M);
   scatter_rows(c, before, after);
-  return c->present[row] != 0;
+  return c->present[row] == c->present_mark;
 }
 
 // fetch_rows for every row of `rows` the host lacks, in one pass: one
@@ -1130,7 +1133,7 @@ bool fetch_rows(wtet_ctx* c, uint32_t row, uint32_t M, uint32_t& fetches) {
 bool fetch_rows_many(wtet_ctx* c, const std::vector<uint32_t>& rows, uint32_t M, uint32_t& fetches) {
   std::vector<uint32_t> seeds;
   for (uint32_t r : rows)
-    if (r < M && !c->present[r]) seeds.push_back(r << 2);
+    if (r < M && c->present[r] != c->present_mark) seeds.push_back(r << 2);
   if (seeds.empty()) return true;
   if (++fetches > kMaxFetches) return false;
   uint32_t before = listed_rows(c, M);
@@ -1145,7 +1148,7 @@ bool fetch_rows_many(wtet_ctx* c, const std::vector<uint32_t>& rows, uint32_t M,
   uint32_t after = listed_rows(c, M);  // synchronizes: seeds stay alive until then
   scatter_rows(c, before, after);
   for (uint32_t r : rows)
-    if (r < M && !c->present[r]) return false;
+    if (r < M && c->present[r] != c->present_mark) return false;
   return true;
 }
 
@@ -1223,6 +1226,7 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
   uint32_t fetches = 0;
   if (sparse) {
     mb.present = c->present.data();
+    mb.present_mark = c->present_mark;
     mb.known = mout;
     mb.fetch = [c, mout, &fetches](uint32_t row) { return fetch_rows(c, row, mout, fetches); };
     mb.fetch_many = [c, mout, &fetches](const std::vector<uint32_t>& rows) {

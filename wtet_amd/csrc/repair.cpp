@@ -305,7 +305,7 @@ struct Mesh32 {
   MeshBuf& m;
   uint32_t inf;
   void need(uint32_t t) const {
-    if (m.present && t < m.known && !m.present[t]) throw NotPresent{t};
+    if (m.present && t < m.known && m.present[t] != m.present_mark) throw NotPresent{t};
   }
   size_t count() const { return m.count; }
   bool dead(uint32_t t) const {

@@ -33,13 +33,14 @@ struct MeshBuf {
   size_t count;
   size_t cap;
   std::vector<uint32_t> changed;
-  // sparse view: when set, only rows r < known with present[r] != 0 hold
-  // data (rows >= known are the repair's own).  Reaching another row calls
+  // sparse view: when set, only rows r < known with present[r] ==
+  // present_mark hold data (rows >= known are the repair's own).  Reaching another row calls
   // fetch(row), which copies rows around it in and marks them present, and
   // retries; without fetch, or when it returns false, the repair stops with
   // needs_full.  Rows the repair has written are never fetched again: every
   // neighbour of a written row is read before the write.
   uint8_t* present = nullptr;
+  uint8_t present_mark = 1;
   size_t known = 0;
   std::function<bool(uint32_t)> fetch;
   // optional: the rows around every listed row in one pass (one device
