@@ -95,6 +95,10 @@ struct wtet_ctx {
   std::vector<uint32_t> bcount;
   uint32_t* h_patch = nullptr;
   size_t h_cap = 0, h_npts = 0, h_patch_cap = 0;
+  // allocates h_tets/h_adj while a single cloud's device phase runs (the
+  // first call at a new size: pinning and first-touching hundreds of MB);
+  // joined before anything reads them
+  std::thread pin_thread;
   uint64_t cap_tets = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -230,6 +234,38 @@ void ensure_pinned(T*& p, size_t& cap, size_t need) {
   // Large buffers are touched by several threads (page faults run in
   // parallel).
   prefault(p, nc * sizeof(T));
+}
+
+// wait for the mirror allocation pin_ahead started, if any
+void pin_join(wtet_ctx* c) {
+  if (c->pin_thread.joinable()) c->pin_thread.join();
+}
+
+// Start allocating the host side of the repair for a mesh of about `rows`
+// rows on a thread of its own, so that it overlaps the device phase: the
+// mirror (h_tets, h_adj), the row flags (present) and the staging of
+// scattered rows (h_patch, room for a sixteenth of the rows).  A failure is
+// left for the synchronous allocation at staging to report.
+void pin_ahead(wtet_ctx* c, size_t rows) {
+  pin_join(c);
+  const size_t need = rows * 4, patch = (rows / 16) * 9 + 9;
+  if (c->h_tets && c->h_adj && c->h_cap >= need && c->h_patch && c->h_patch_cap >= patch &&
+      c->present.size() >= rows)
+    return;
+  const int dev = c->device;
+  c->pin_thread = std::thread([c, rows, need, patch, dev] {
+    try {
+      if (hipSetDevice(dev) != hipSuccess) return;
+      size_t c1 = c->h_cap, c2 = c->h_cap;
+      ensure_pinned(c->h_tets, c1, need);
+      ensure_pinned(c->h_adj, c2, need);
+      c->h_cap = std::min(c1, c2);
+      if (c->present.size() < rows) c->present.assign(rows, 0);
+      ensure_pinned(c->h_patch, c->h_patch_cap, patch);
+    } catch (...) {
+      (void)hipGetLastError();
+    }
+  });
 }
 
 // per-tet-slot arrays sized for `cap` slots; keep=true preserves contents
@@ -906,6 +942,7 @@ void dev_output(wtet_ctx* c, Run& R, std::vector<uint32_t>& viol_rec, uint32_t& 
   if (!resident) {
     // room for the repair to append (it rarely adds more than a few tets)
     // and for the tets of the points the insertion cap left out
+    pin_join(c);
     size_t cap = (size_t)Mout + Mout / 32 + 65536 + 8ull * R.K.left;
     size_t c1 = c->h_cap, c2 = c->h_cap;
     ensure_pinned(c->h_tets, c1, cap * 4);
@@ -933,6 +970,14 @@ void run_device(wtet_ctx* c, const float* d_xyz, const std::vector<uint32_t>& of
                 const wtet_config& cfg, Counters& K, std::vector<uint32_t>& viol_rec,
                 uint32_t& nviol, bool resident, uint32_t& mout, std::vector<uint32_t>& cstart) {
   end_stage(c);  // the call reuses the buffers a stage session holds
+  pin_join(c);
+  // a single cloud's host work needs a mirror of about every row (the
+  // Delaunay mesh of n points in general position has about 6.7n tets):
+  // on a first call at this size it is allocated while the device works
+  if (off.size() == 2 && 7ull * off[1] >= (1u << 22)) {
+    const size_t rows = 7ull * off[1];
+    pin_ahead(c, rows + rows / 32 + 65536);
+  }
   Run R;
   run_setup(c, R, off, cfg);
   dev_prepare(c, R, d_xyz);
@@ -1091,7 +1136,9 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
                        P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), cnt, cnt + 1, false, s);
   HIP_CHECK(hipGetLastError());
   uint32_t n = listed_rows(c, M);
-  if (c->present.size() < M) c->present.resize(M, 0);
+  // a few rows more than the last call: the flags are allocated again
+  // (all clear) with room to spare, not copied to a vector just as long
+  if (c->present.size() < M) c->present.assign((size_t)M + M / 16, 0);
   if (++c->present_mark == 0) {
     std::fill(c->present.begin(), c->present.end(), 0);
     c->present_mark = 1;
@@ -1210,6 +1257,7 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
                    uint32_t region_cap) {
   const bool resident = dout != nullptr;
   const uint32_t left = K.left;
+  pin_join(c);
   c->stats.violations_gpu = nviol;
   c->stats.points_left = left;
   auto t1 = std::chrono::steady_clock::now();
@@ -1730,6 +1778,7 @@ wtet_status wtet_create(int device, wtet_ctx** out) {
 void wtet_destroy(wtet_ctx* c) {
   if (!c) return;
   end_stage(c);
+  pin_join(c);
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
