@@ -401,6 +401,16 @@ uint32_t grid_for(const wtet_ctx* c, uint64_t a, uint32_t lo) {
   return g;
 }
 
+// The deferred check's grid: threads for four times the deferred tets an
+// iteration is expected to hold (the first iterations of a round hold more
+// than the average, and each deferred tet keeps its thread busy for
+// microseconds), up to four times the check's grid.  Degenerate input
+// defers most tets: a 128^3 lattice's deferred checks took 188 ms at one
+// thread per expected tet, 165 ms at four.
+uint32_t deferred_grid(const wtet_ctx* c, uint64_t dper, uint32_t grid) {
+  return std::min(grid_for(c, 4 * dper, 1), std::min(grid * 4, 4096u));
+}
+
 // k flip iterations without the host: replays a captured graph (one per
 // buffer set, grid and k; captured on first use), or launches directly when
 // profiling (per-kernel events) or when graphs are off.
@@ -477,7 +487,7 @@ void run_flip_batch(wtet_ctx* c, uint32_t inf, int mode, uint64_t* st, int k, ui
     return;
   }
   uint32_t grid = grid_for(c, a_hint, 16);
-  uint32_t dgrid = std::min(grid_for(c, d_hint, 1), grid);
+  uint32_t dgrid = deferred_grid(c, d_hint, grid);
   if (!c->use_graphs || c->profiling) {
     if (c->evused > 4096) collect_times(c);
     enqueue_flip_iterations(c, M, L, st, k, grid, dgrid);
@@ -787,7 +797,7 @@ void run_round(wtet_ctx* c, Run& R, bool last) {
   const uint64_t a_hint = (uint64_t)c->h_ctr[C_A] + 4ull * W;
   const int variant = !R.flipping ? 0 : (a_hint <= kSmallList ? 1 : 2);
   const uint32_t grid = variant == 2 ? grid_for(c, a_hint, 16) : 0u;
-  const uint32_t dgrid = variant == 2 ? std::min(grid_for(c, dper, 1), grid) : 0u;
+  const uint32_t dgrid = variant == 2 ? deferred_grid(c, dper, grid) : 0u;
   const int k = R.max_fi;
   uint32_t* uv = P<uint32_t>(c->uvert[cur]);
   uint32_t* ut = P<uint32_t>(c->utet[cur]);

@@ -11,8 +11,10 @@
 //   3. exact stage: every coordinate is an integer multiple of 2^E, where E
 //      is the lowest set-bit exponent over the coordinates involved, so the
 //      determinant is evaluated exactly in fixed-width two's-complement
-//      integers (32-bit limbs; 64-bit limb pairs when the coordinate span is
-//      <= 60 bits, 192-bit limbs otherwise).
+//      integers (32-bit limbs): one limb per coordinate difference when the
+//      differences to the last point are exact in f64 and span <= 30 bits,
+//      else 64-bit limb pairs per coordinate when the coordinates span
+//      <= 60 bits, 192-bit limbs otherwise.
 // When the exact determinant is zero, Simulation of Simplicity decides, with
 // the cascades of the reference (src/predicates.rs:106-208 for orient3d,
 // :478-675 for insphere): Edelsbrunner-Muecke perturbations ranked by point
@@ -210,15 +212,93 @@ WT_HD BI<D> to_fixed(float v, int E) {
 }
 
 // ---------------------------------------------------------------------------
+// Short differences.  orient3d and insphere depend only on the differences
+// of their points to the last one.  Each difference of two f32 coordinates
+// is computed in f64 and kept when exact (checked with Knuth's two-sum
+// error term); when every one is exact and their set bits span at most
+// kRelSpan bits, they are integers below 2^30 in units of their lowest set
+// bit, and the determinant runs on single 32-bit limbs.  Nearby points have
+// short differences even where the coordinates themselves span many bits
+// (a 128^3 lattice: coordinates span up to 31 bits, neighbour differences
+// 24), and the limb products fall about fourfold.
+// ---------------------------------------------------------------------------
+constexpr int kRelSpan = 30;
+
+struct F64Parts {
+  uint64_t m;  // significand (0 for zero)
+  int e;       // value = +-m * 2^e
+  bool neg;
+};
+
+WT_HD F64Parts f64_parts(double v) {
+  uint64_t u;
+  __builtin_memcpy(&u, &v, 8);
+  F64Parts p;
+  p.neg = (u >> 63) != 0u;
+  uint32_t be = (uint32_t)((u >> 52) & 0x7FFu);
+  uint64_t fr = u & ((1ull << 52) - 1ull);
+  if (be == 0u) {
+    p.m = fr;
+    p.e = -1074;
+  } else {
+    p.m = fr | (1ull << 52);
+    p.e = (int)be - 1075;
+  }
+  return p;
+}
+
+// a - b in f64; false when the rounded difference is not the exact one
+WT_HD bool exact_sub(double a, double b, double* d) {
+  double s = a - b;
+  double bv = s - a;
+  double err = (a - (s - bv)) + (-b - bv);
+  *d = s;
+  return err == 0.0;
+}
+
+// the R points before the last as differences to it, 1-limb integers in
+// units of 2^E (E the lowest set bit over them); false when the tier does
+// not apply (a difference not exact in f64, or a span over kRelSpan)
+template <int R>
+WT_HD bool rel_fixed1(const float* P, BI<1> (*d)[3]) {
+  double v[R][3];
+  int L = 100000, H = -100000;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (!exact_sub((double)P[r * 3 + k], (double)P[R * 3 + k], &v[r][k])) return false;
+      F64Parts p = f64_parts(v[r][k]);
+      if (p.m == 0u) continue;
+      int l = p.e + __builtin_ctzll(p.m);
+      int h = p.e + 63 - __builtin_clzll(p.m);
+      L = l < L ? l : L;
+      H = h > H ? h : H;
+    }
+  if (L != 100000 && H - L + 1 > kRelSpan) return false;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      F64Parts p = f64_parts(v[r][k]);
+      BI<1> x;
+      x.w[0] = 0u;
+      if (p.m != 0u) {
+        int tz = __builtin_ctzll(p.m);
+        x.w[0] = (uint32_t)((p.m >> tz) << (p.e + tz - L));  // < 2^30
+        if (p.neg) x = bi_neg(x);
+      }
+      d[r][k] = x;
+    }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // exact determinants (templated on the coordinate width D in limbs)
 // ---------------------------------------------------------------------------
+// det of the 3x3 differences d (rows a-d, b-d, c-d)
 template <int D>
-WT_HD int exact_orient3d_T(const float* P, int E) {
-  // P: 4 points x 3 coords; det[a-d; b-d; c-d]
-  BI<D> d[3][3];
-  for (int r = 0; r < 3; ++r)
-    for (int k = 0; k < 3; ++k)
-      d[r][k] = bi_sub(to_fixed<D>(P[r * 3 + k], E), to_fixed<D>(P[9 + k], E));
+WT_HD int orient3d_core(const BI<D> (*d)[3]) {
   BI<2 * D> bc = bi_sub(bi_mul(d[1][0], d[2][1]), bi_mul(d[2][0], d[1][1]));
   BI<2 * D> ca = bi_sub(bi_mul(d[2][0], d[0][1]), bi_mul(d[0][0], d[2][1]));
   BI<2 * D> ab = bi_sub(bi_mul(d[0][0], d[1][1]), bi_mul(d[1][0], d[0][1]));
@@ -228,14 +308,24 @@ WT_HD int exact_orient3d_T(const float* P, int E) {
 }
 
 template <int D>
-WT_HD int exact_insphere_T(const float* P, int E) {
-  // P: 5 points; Shewchuk's insphere with a..d relative to e
+WT_HD int exact_orient3d_T(const float* P, int E) {
+  // P: 4 points x 3 coords; det[a-d; b-d; c-d]
+  BI<D> d[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      d[r][k] = bi_sub(to_fixed<D>(P[r * 3 + k], E), to_fixed<D>(P[9 + k], E));
+  return orient3d_core<D>(d);
+}
+
+// Shewchuk's insphere on the differences d (rows a-e .. d-e)
+template <int D>
+WT_HD int insphere_core(const BI<D> (*d)[3]) {
   BI<D> x[4], y[4], z[4];
   BI<2 * D> l[4];
   for (int r = 0; r < 4; ++r) {
-    x[r] = bi_sub(to_fixed<D>(P[r * 3 + 0], E), to_fixed<D>(P[12], E));
-    y[r] = bi_sub(to_fixed<D>(P[r * 3 + 1], E), to_fixed<D>(P[13], E));
-    z[r] = bi_sub(to_fixed<D>(P[r * 3 + 2], E), to_fixed<D>(P[14], E));
+    x[r] = d[r][0];
+    y[r] = d[r][1];
+    z[r] = d[r][2];
     l[r] = bi_add(bi_add(bi_mul(x[r], x[r]), bi_mul(y[r], y[r])), bi_mul(z[r], z[r]));
   }
   // 2x2 minors on (x, y)
@@ -253,6 +343,16 @@ WT_HD int exact_insphere_T(const float* P, int E) {
   BI<5 * D> det = bi_add(bi_sub(bi_mul(l[3], abc), bi_mul(l[2], dab)),
                          bi_sub(bi_mul(l[1], cda), bi_mul(l[0], bcd)));
   return bi_sign(det);
+}
+
+template <int D>
+WT_HD int exact_insphere_T(const float* P, int E) {
+  // P: 5 points; Shewchuk's insphere with a..d relative to e
+  BI<D> d[4][3];
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 3; ++k)
+      d[r][k] = bi_sub(to_fixed<D>(P[r * 3 + k], E), to_fixed<D>(P[12 + k], E));
+  return insphere_core<D>(d);
 }
 
 // det [[p0 p1 |p|^2 1]] over 4 points (coordinates already permuted)
@@ -277,6 +377,22 @@ WT_HD int exact_orient3d_lifted_T(const float* P, int E) {
   BI<2 * D> ca = bi_sub(bi_mul(dx[2], dy[0]), bi_mul(dx[0], dy[2]));
   BI<2 * D> ab = bi_sub(bi_mul(dx[0], dy[1]), bi_mul(dx[1], dy[0]));
   BI<4 * D> det = bi_add(bi_add(bi_mul(dw[0], bc), bi_mul(dw[1], ca)), bi_mul(dw[2], ab));
+  return bi_sign(det);
+}
+
+// det [[p0 p1 |p|^2 1]] over 4 COPLANAR points from their differences to
+// the last one: det[[u_i - u_3, v_i - v_3, |p_i - p_3|^2]].  Translating
+// the points by t changes the lifted determinant by -2 t_w det[[p0 p1 p2 1]]
+// (t_w the coordinate left out of the first two columns), which vanishes
+// when the four points are coplanar; the sign is then the absolute form's.
+WT_HD int orient3d_lifted_rel1(const BI<1> (*d)[3]) {
+  BI<2> w[3];
+  for (int r = 0; r < 3; ++r)
+    w[r] = bi_add(bi_add(bi_mul(d[r][0], d[r][0]), bi_mul(d[r][1], d[r][1])), bi_mul(d[r][2], d[r][2]));
+  BI<2> bc = bi_sub(bi_mul(d[1][0], d[2][1]), bi_mul(d[2][0], d[1][1]));
+  BI<2> ca = bi_sub(bi_mul(d[2][0], d[0][1]), bi_mul(d[0][0], d[2][1]));
+  BI<2> ab = bi_sub(bi_mul(d[0][0], d[1][1]), bi_mul(d[1][0], d[0][1]));
+  BI<4> det = bi_add(bi_add(bi_mul(w[0], bc), bi_mul(w[1], ca)), bi_mul(w[2], ab));
   return bi_sign(det);
 }
 
@@ -317,9 +433,24 @@ constexpr int kNarrowSpan = 60;
   if (hi_ - lo_ + 1 <= kNarrowSpan) return fn<2>(P, lo_); \
   return fn<6>(P, lo_);
 
-WT_HDN int exact_orient3d(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_T, P, 12) }
-WT_HDN int exact_insphere(const float* P) { WT_EXACT_DISPATCH(exact_insphere_T, P, 15) }
-WT_HDN int exact_orient3d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_lifted_T, P, 12) }
+WT_HDN int exact_orient3d(const float* P) {
+  BI<1> d[3][3];
+  if (rel_fixed1<3>(P, d)) return orient3d_core<1>(d);
+  WT_EXACT_DISPATCH(exact_orient3d_T, P, 12)
+}
+WT_HDN int exact_insphere(const float* P) {
+  BI<1> d[4][3];
+  if (rel_fixed1<4>(P, d)) return insphere_core<1>(d);
+  WT_EXACT_DISPATCH(exact_insphere_T, P, 15)
+}
+// the lifted orient3d of four points known to be coplanar (the SoS cascade
+// reaches a lifted 4-point step only after the plain orient3d of the same
+// four points vanished): short differences when they apply
+WT_HDN int exact_orient3d_lifted_coplanar(const float* P) {
+  BI<1> d[3][3];
+  if (rel_fixed1<3>(P, d)) return orient3d_lifted_rel1(d);
+  WT_EXACT_DISPATCH(exact_orient3d_lifted_T, P, 12)
+}
 WT_HDN int exact_orient2d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient2d_lifted_T, P, 9) }
 WT_HDN int exact_orient1d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient1d_lifted_T, P, 6) }
 
@@ -754,7 +885,10 @@ WT_HDN int orient4d_sos_only(const float* a, const float* b, const float* c, con
     if (s.kind == 3) {
       // a plain orient3d sign goes through the filters first: the same
       // exact sign, almost always from the FP32 stage
-      r = s.lifted ? exact_orient3d_lifted(Q) : orient3d_raw(Q, Q + 3, Q + 6, Q + 9, PRED_FILTERED, nullptr);
+      // a lifted step's four points had their plain orient3d vanish at an
+      // earlier depth (depths 1, 5, 15 for the lifted 2-4, 9/12/14, 25/31/34)
+      r = s.lifted ? exact_orient3d_lifted_coplanar(Q)
+                   : orient3d_raw(Q, Q + 3, Q + 6, Q + 9, PRED_FILTERED, nullptr);
     } else if (s.kind == 2) {
       r = s.lifted ? exact_orient2d_lifted(Q)
                    : dsign(orient2d_plain(Q[0], Q[1], Q[3], Q[4], Q[6], Q[7]));
