@@ -433,23 +433,39 @@ constexpr int kNarrowSpan = 60;
   if (hi_ - lo_ + 1 <= kNarrowSpan) return fn<2>(P, lo_); \
   return fn<6>(P, lo_);
 
-WT_HDN int exact_orient3d(const float* P) {
+// The short-difference tier as small leaf functions (few registers, so a
+// call saves none: no scratch traffic on the path degenerate inputs take
+// millions of times), returning 2 when the tier does not apply; the wide
+// forms are separate calls for the rest.
+WT_HDN int exact_orient3d_short(const float* P) {
   BI<1> d[3][3];
-  if (rel_fixed1<3>(P, d)) return orient3d_core<1>(d);
-  WT_EXACT_DISPATCH(exact_orient3d_T, P, 12)
+  return rel_fixed1<3>(P, d) ? orient3d_core<1>(d) : 2;
 }
-WT_HDN int exact_insphere(const float* P) {
+WT_HDN int exact_insphere_short(const float* P) {
   BI<1> d[4][3];
-  if (rel_fixed1<4>(P, d)) return insphere_core<1>(d);
-  WT_EXACT_DISPATCH(exact_insphere_T, P, 15)
+  return rel_fixed1<4>(P, d) ? insphere_core<1>(d) : 2;
+}
+WT_HDN int exact_orient3d_wide(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_T, P, 12) }
+WT_HDN int exact_insphere_wide(const float* P) { WT_EXACT_DISPATCH(exact_insphere_T, P, 15) }
+WT_HD int exact_orient3d(const float* P) {
+  int s = exact_orient3d_short(P);
+  return s != 2 ? s : exact_orient3d_wide(P);
+}
+WT_HD int exact_insphere(const float* P) {
+  int s = exact_insphere_short(P);
+  return s != 2 ? s : exact_insphere_wide(P);
 }
 // the lifted orient3d of four points known to be coplanar (the SoS cascade
 // reaches a lifted 4-point step only after the plain orient3d of the same
 // four points vanished): short differences when they apply
-WT_HDN int exact_orient3d_lifted_coplanar(const float* P) {
+WT_HDN int exact_orient3d_lifted_coplanar_short(const float* P) {
   BI<1> d[3][3];
-  if (rel_fixed1<3>(P, d)) return orient3d_lifted_rel1(d);
-  WT_EXACT_DISPATCH(exact_orient3d_lifted_T, P, 12)
+  return rel_fixed1<3>(P, d) ? orient3d_lifted_rel1(d) : 2;
+}
+WT_HDN int exact_orient3d_lifted_wide(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_lifted_T, P, 12) }
+WT_HD int exact_orient3d_lifted_coplanar(const float* P) {
+  int s = exact_orient3d_lifted_coplanar_short(P);
+  return s != 2 ? s : exact_orient3d_lifted_wide(P);
 }
 WT_HDN int exact_orient2d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient2d_lifted_T, P, 9) }
 WT_HDN int exact_orient1d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient1d_lifted_T, P, 6) }
@@ -809,20 +825,23 @@ struct SosStep {
   uint8_t kind, lifted, perm, r0, r1, r2, r3;
 };
 
+// packed: kind<<24 | lifted<<20 | perm<<16 | r0<<12 | r1<<8 | r2<<4 | r3
+// (a table in constant memory: as a local array it was built on the stack
+// at every call)
+constexpr uint32_t kSos4Steps[50] = {
+    0u,
+    0x3001234u, 0x3101234u, 0x3111234u, 0x3121234u, 0x3000234u, 0x2002340u,
+    0x2012340u, 0x2022340u, 0x3100234u, 0x2102340u, 0x2132340u, 0x3110234u,
+    0x2142340u, 0x3120234u, 0x3000134u, 0x2001340u, 0x2011340u, 0x2021340u,
+    0x2000340u, 0x1003400u, 0x1033400u, 0x2010340u, 0x1043400u, 0x2020340u,
+    0x3100134u, 0x2101340u, 0x2131340u, 0x2100340u, 0x1103400u, 0x2130340u,
+    0x3110134u, 0x2141340u, 0x2140340u, 0x3120134u, 0x3000124u, 0x2001240u,
+    0x2011240u, 0x2021240u, 0x2000240u, 0x1002400u, 0x1032400u, 0x2010240u,
+    0x1042400u, 0x2020240u, 0x2000140u, 0x1001400u, 0x1031400u, 0x1000400u,
+    0u};
+
 WT_HD SosStep sos4_step(int depth) {
-  // packed: kind<<24 | lifted<<20 | perm<<16 | r0<<12 | r1<<8 | r2<<4 | r3
-  const uint32_t T[50] = {
-      0u,
-      0x3001234u, 0x3101234u, 0x3111234u, 0x3121234u, 0x3000234u, 0x2002340u,
-      0x2012340u, 0x2022340u, 0x3100234u, 0x2102340u, 0x2132340u, 0x3110234u,
-      0x2142340u, 0x3120234u, 0x3000134u, 0x2001340u, 0x2011340u, 0x2021340u,
-      0x2000340u, 0x1003400u, 0x1033400u, 0x2010340u, 0x1043400u, 0x2020340u,
-      0x3100134u, 0x2101340u, 0x2131340u, 0x2100340u, 0x1103400u, 0x2130340u,
-      0x3110134u, 0x2141340u, 0x2140340u, 0x3120134u, 0x3000124u, 0x2001240u,
-      0x2011240u, 0x2021240u, 0x2000240u, 0x1002400u, 0x1032400u, 0x2010240u,
-      0x1042400u, 0x2020240u, 0x2000140u, 0x1001400u, 0x1031400u, 0x1000400u,
-      0u};
-  uint32_t x = T[depth];
+  uint32_t x = kSos4Steps[depth];
   SosStep s;
   s.kind = (uint8_t)(x >> 24);
   s.lifted = (uint8_t)((x >> 20) & 0xF);
