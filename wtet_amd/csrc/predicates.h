@@ -347,12 +347,30 @@ WT_HD int insphere_core(const BI<D> (*d)[3]) {
 
 template <int D>
 WT_HD int exact_insphere_T(const float* P, int E) {
-  // P: 5 points; Shewchuk's insphere with a..d relative to e
-  BI<D> d[4][3];
-  for (int r = 0; r < 4; ++r)
-    for (int k = 0; k < 3; ++k)
-      d[r][k] = bi_sub(to_fixed<D>(P[r * 3 + k], E), to_fixed<D>(P[12 + k], E));
-  return insphere_core<D>(d);
+  // P: 5 points; Shewchuk's insphere with a..d relative to e.  The same
+  // expression as insphere_core, written out on its own: built through the
+  // core's array argument, the wide forms spilled more.
+  BI<D> x[4], y[4], z[4];
+  BI<2 * D> l[4];
+  for (int r = 0; r < 4; ++r) {
+    x[r] = bi_sub(to_fixed<D>(P[r * 3 + 0], E), to_fixed<D>(P[12], E));
+    y[r] = bi_sub(to_fixed<D>(P[r * 3 + 1], E), to_fixed<D>(P[13], E));
+    z[r] = bi_sub(to_fixed<D>(P[r * 3 + 2], E), to_fixed<D>(P[14], E));
+    l[r] = bi_add(bi_add(bi_mul(x[r], x[r]), bi_mul(y[r], y[r])), bi_mul(z[r], z[r]));
+  }
+  BI<2 * D> ab = bi_sub(bi_mul(x[0], y[1]), bi_mul(x[1], y[0]));
+  BI<2 * D> bc = bi_sub(bi_mul(x[1], y[2]), bi_mul(x[2], y[1]));
+  BI<2 * D> cd = bi_sub(bi_mul(x[2], y[3]), bi_mul(x[3], y[2]));
+  BI<2 * D> da = bi_sub(bi_mul(x[3], y[0]), bi_mul(x[0], y[3]));
+  BI<2 * D> ac = bi_sub(bi_mul(x[0], y[2]), bi_mul(x[2], y[0]));
+  BI<2 * D> bd = bi_sub(bi_mul(x[1], y[3]), bi_mul(x[3], y[1]));
+  BI<3 * D> abc = bi_add(bi_sub(bi_mul(z[0], bc), bi_mul(z[1], ac)), bi_mul(z[2], ab));
+  BI<3 * D> bcd = bi_add(bi_sub(bi_mul(z[1], cd), bi_mul(z[2], bd)), bi_mul(z[3], bc));
+  BI<3 * D> cda = bi_add(bi_add(bi_mul(z[2], da), bi_mul(z[3], ac)), bi_mul(z[0], cd));
+  BI<3 * D> dab = bi_add(bi_add(bi_mul(z[3], ab), bi_mul(z[0], bd)), bi_mul(z[1], da));
+  BI<5 * D> det = bi_add(bi_sub(bi_mul(l[3], abc), bi_mul(l[2], dab)),
+                         bi_sub(bi_mul(l[1], cda), bi_mul(l[0], bcd)));
+  return bi_sign(det);
 }
 
 // det [[p0 p1 |p|^2 1]] over 4 points (coordinates already permuted)
@@ -447,12 +465,20 @@ WT_HDN int exact_insphere_short(const float* P) {
 }
 WT_HDN int exact_orient3d_wide(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_T, P, 12) }
 WT_HDN int exact_insphere_wide(const float* P) { WT_EXACT_DISPATCH(exact_insphere_T, P, 15) }
+// The short tier cannot apply with a super vertex among the points (its
+// coordinates are -100 or 400, the differences to it span more than 30
+// bits): such calls go to the wide forms at once, without a failed try.
+WT_HD bool near_unit(const float* P, int k) {
+  bool ok = true;
+  for (int i = 0; i < k; ++i) ok = ok && P[i] > -4.0f && P[i] < 4.0f;
+  return ok;
+}
 WT_HD int exact_orient3d(const float* P) {
-  int s = exact_orient3d_short(P);
+  int s = near_unit(P, 12) ? exact_orient3d_short(P) : 2;
   return s != 2 ? s : exact_orient3d_wide(P);
 }
 WT_HD int exact_insphere(const float* P) {
-  int s = exact_insphere_short(P);
+  int s = near_unit(P, 15) ? exact_insphere_short(P) : 2;
   return s != 2 ? s : exact_insphere_wide(P);
 }
 // the lifted orient3d of four points known to be coplanar (the SoS cascade
@@ -464,7 +490,7 @@ WT_HDN int exact_orient3d_lifted_coplanar_short(const float* P) {
 }
 WT_HDN int exact_orient3d_lifted_wide(const float* P) { WT_EXACT_DISPATCH(exact_orient3d_lifted_T, P, 12) }
 WT_HD int exact_orient3d_lifted_coplanar(const float* P) {
-  int s = exact_orient3d_lifted_coplanar_short(P);
+  int s = near_unit(P, 12) ? exact_orient3d_lifted_coplanar_short(P) : 2;
   return s != 2 ? s : exact_orient3d_lifted_wide(P);
 }
 WT_HDN int exact_orient2d_lifted(const float* P) { WT_EXACT_DISPATCH(exact_orient2d_lifted_T, P, 9) }
@@ -717,7 +743,10 @@ WT_HD int orient3d_raw(const float* a, const float* b, const float* c, const flo
     }
   }
   float P[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
-  int s = exact_orient3d(P);
+  // the short tier pays where the filters have left the near-degenerate
+  // cases (a lattice's co-planar and co-spherical sets); on every call of
+  // the exact-only mode the wide form alone is faster
+  int s = mode == PRED_FILTERED ? exact_orient3d(P) : exact_orient3d_wide(P);
   stat_inc(st, s != 0 ? 2 : 3);
   return s;
 }
@@ -740,7 +769,7 @@ WT_HD int insphere_raw(const float* a, const float* b, const float* c, const flo
   }
   float P[15] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2],
                  d[0], d[1], d[2], e[0], e[1], e[2]};
-  int s = exact_insphere(P);
+  int s = mode == PRED_FILTERED ? exact_insphere(P) : exact_insphere_wide(P);
   stat_inc(st, s != 0 ? 6 : 7);
   return s;
 }
