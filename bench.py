@@ -118,6 +118,10 @@ def main():
                     help="only the 10,000-point Hilbert workload, exact only then filtered: 2 warmup "
                          "and 5 timed resident calls each, one JSON line per mode (the program "
                          "profiled for that workload)")
+    ap.add_argument("--stress-only", default="", metavar="NAMES",
+                    help="only the stress inputs named (comma-separated: lattice_128, sphere_1M): "
+                         "one untimed and 2 timed resident calls each, one JSON line per input "
+                         "(the program profiled for those workloads)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the streaming copy kernel over MIB MiB (5 launches), "
                          "the known-size reference for HBM byte counters")
@@ -143,6 +147,28 @@ def main():
         ctx.free(dst)
         print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(dt * 1e3, 3),
                           "GBs_read_plus_write": round(2 * nbytes / dt / 1e9, 1)}), flush=True)
+        return
+
+    if args.stress_only:
+        ctx = wtet_amd.Context(0)
+        ctx.set_flip_graphs(not args.no_graphs)
+        inputs = {"lattice_128": lambda: W.lattice(128), "sphere_1M": lambda: W.sphere(1_000_000, 2)}
+        for name in args.stress_only.split(","):
+            pts_s = inputs[name]()
+            d_s = ctx.upload(pts_s)
+            ctx.delaunay_resident(d_s, len(pts_s))
+            t = []
+            for _ in range(2):
+                t0 = time.perf_counter()
+                ctx.delaunay_resident(d_s, len(pts_s))
+                t.append((time.perf_counter() - t0) * 1e3)
+            s1 = ctx.stats()
+            ctx.free(d_s)
+            print(json.dumps({"workload": name, "points": len(pts_s), "ms": [round(x, 2) for x in t],
+                              "ms_device": round(s1["ms_device"], 3), "ms_repair": round(s1["ms_repair"], 3),
+                              "rounds": s1["rounds"], "flip_iterations": s1["flip_iterations"],
+                              "flips": s1["sum_flips"], "checked_tets": s1["sum_active"],
+                              "violations_gpu": s1["violations_gpu"]}), flush=True)
         return
 
     if args.config1_only:
