@@ -1455,6 +1455,17 @@ void delaunay_impl(void* a_) {
     uint32_t nviol = 0, mout = 0;
     run_device(c, d_xyz, {0u, a.n}, a.cfg, K, rec, nviol, resident, mout, cstart);
     uint32_t cap = c->ring < 2 ? c->region_cap : 0u;
+    // A trajectory that left stuck faces by the thousand (over a quarter of
+    // a percent of the points) merges them into regions past the cap: the
+    // wider ring at once, instead of after a repair that would stop there
+    // (1M sphere points: 830-1,020 stuck faces in most runs, 3,100-3,800 in
+    // the others, whose repair ran 400-600 ms before its region passed the
+    // cap)
+    if (cap && a.cfg.enable_splaying && nviol > std::max<uint32_t>(2000u, a.n / 400u)) {
+      ++retries;
+      c->ring += 1;
+      continue;
+    }
     try {
       finish_single(c, a.n, a.cfg, K, rec, nviol, mout, a.out, a.dout, t0, cap);
       break;
