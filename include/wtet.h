@@ -242,8 +242,10 @@ wtet_status wtet_set_insertion_ring(wtet_ctx *ctx, int ring);
  * triangulation does not depend on it.  No counterpart in the reference. */
 wtet_status wtet_set_insertion_warmup(wtet_ctx *ctx, uint32_t rounds);
 /* A resident result's host repair works on the rows within `rings`
- * face-steps (1..16, default 7) of each stuck face, copied to the host; a
- * repair that needs more falls back to every row.  No counterpart in the
+ * face-steps (1..16, default 7; three more, at most 16, when the stuck
+ * faces number over one per 2,000 points) of each stuck face, copied to
+ * the host; rows it reaches beyond them are fetched on demand (at most 32
+ * times), past that every row is copied.  No counterpart in the
  * reference. */
 wtet_status wtet_set_repair_rings(wtet_ctx *ctx, int rings);
 /* A single-cloud call whose host repair grows a region past `tets` tets

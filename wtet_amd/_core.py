@@ -396,7 +396,8 @@ class Context:
             raise ValueError(self.error())
 
     def set_repair_rings(self, rings: int):
-        """Rows within `rings` face-steps of each stuck face go to the host repair."""
+        """Rows within `rings` face-steps of each stuck face go to the host repair
+        (three more when the stuck faces number over one per 2,000 points)."""
         if lib().wtet_set_repair_rings(self._h, int(rings)) != 0:
             raise ValueError("rings must be in 1..16")
 

@@ -1130,7 +1130,7 @@ uint32_t listed_rows(wtet_ctx* c, uint32_t M) {
   return n;
 }
 
-void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
+void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M, int rings) {
   hipStream_t s = c->stream;
   size_t cap = (size_t)M + M / 32 + 65536;
   size_t c1 = c->h_cap, c2 = c->h_cap;
@@ -1142,7 +1142,7 @@ void stage_neighbourhood(wtet_ctx* c, uint32_t nrec, uint32_t M) {
   ensure(c, c->nbr_list, (size_t)M * 4 + 16);
   ensure(c, c->nbr_misc, 4 * (kRepairRingsMax + 4));
   uint32_t* cnt = P<uint32_t>(c->nbr_misc);
-  launch_neighbourhood(P<uint32_t>(c->rec), nrec, M, tail, c->repair_rings, P<uint32_t>(c->fout),
+  launch_neighbourhood(P<uint32_t>(c->rec), nrec, M, tail, rings, P<uint32_t>(c->fout),
                        P<uint32_t>(c->nbr_mark), P<uint32_t>(c->nbr_list), cnt, cnt + 1, false, s);
   HIP_CHECK(hipGetLastError());
   uint32_t n = listed_rows(c, M);
@@ -1276,7 +1276,13 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
     if (left > 0 || rec.size() < nviol) {
       download_rows(c, mout, left);  // point insertion needs every row
     } else {
-      stage_neighbourhood(c, (uint32_t)rec.size(), mout);
+      // Stuck faces denser than one per 2,000 points come in knots whose
+      // regions reach past the staged rings (1M sphere points: 830-1,020
+      // faces, 5-15 fetches at 7 rings, none at 10; repair median 24.8 ->
+      // 14.5 ms); sparser ones (1M uniform: 120-200) keep the narrower
+      // staging, which is 0.7 ms cheaper there
+      const int rings = std::min(c->repair_rings + ((uint64_t)nviol * 2000u > n ? 3 : 0), kRepairRingsMax);
+      stage_neighbourhood(c, (uint32_t)rec.size(), mout, rings);
       sparse = true;
     }
   }
