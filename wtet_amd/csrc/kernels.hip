@@ -474,6 +474,13 @@ void launch_round_next(uint32_t* ctr, hipStream_t s) { k_round_next<<<1, 64, 0, 
 // staged predicates on those; FULL runs them on every point (exact mode).
 enum KForm : int { LEAN = 0, FULL = 1, SLOW = 2 };
 
+// Grid of the point kernels' SLOW form, for the points the LEAN form
+// deferred (their count is on the device): a quarter of the LEAN grid, at
+// most 1,024 blocks.  Each deferred point keeps its thread in the full
+// predicates for microseconds; degenerate input defers many (a 128^3
+// lattice's relocation: 26.0 ms at a sixteenth and 256 blocks, 18.5 ms).
+static uint32_t slow_grid(uint32_t g) { return std::min(g / 4u + 16u, 1024u); }
+
 // The survivors of the round's pick (ctr[C_SURV]) in the round ctr[C_ROUND].
 template <int FORM>
 __global__ void __launch_bounds__(kBlock) k_split_points(Mesh M, const uint32_t* uvert,
@@ -500,7 +507,7 @@ void launch_split_points(const Mesh& M, const uint32_t* uvert, uint32_t* utet, u
   if (M.mode == PRED_FILTERED) {
     launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
     k_split_points<LEAN><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
-    k_split_points<SLOW><<<std::min(g / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, st, slow);
+    k_split_points<SLOW><<<slow_grid(g), kBlock, 0, s>>>(M, uvert, utet, st, slow);
   } else {
     k_split_points<FULL><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
   }
@@ -781,7 +788,7 @@ void launch_relocate(const Mesh& M, const uint32_t* uvert, uint32_t* utet, uint3
   if (M.mode == PRED_FILTERED) {
     launch_set_ctr(M.ctr, C_SLOW_P, 0u, -1, 0u, s);
     k_relocate<LEAN><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
-    k_relocate<SLOW><<<std::min(g / 16u + 16u, 256u), kBlock, 0, s>>>(M, uvert, utet, st, slow);
+    k_relocate<SLOW><<<slow_grid(g), kBlock, 0, s>>>(M, uvert, utet, st, slow);
   } else {
     k_relocate<FULL><<<g, kBlock, 0, s>>>(M, uvert, utet, st, slow);
   }
