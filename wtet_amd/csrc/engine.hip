@@ -401,14 +401,15 @@ uint32_t grid_for(const wtet_ctx* c, uint64_t a, uint32_t lo) {
   return g;
 }
 
-// The deferred check's grid: threads for four times the deferred tets an
-// iteration is expected to hold (the first iterations of a round hold more
-// than the average, and each deferred tet keeps its thread busy for
-// microseconds), up to four times the check's grid.  Degenerate input
-// defers most tets: a 128^3 lattice's deferred checks took 188 ms at one
-// thread per expected tet, 165 ms at four.
+// The deferred check's grid: four lanes per tet (one per face), for four
+// times the deferred tets an iteration is expected to hold (the first
+// iterations of a round hold more than the average, and each deferred tet
+// keeps its lanes busy for microseconds), up to eight times the check's
+// grid.  Degenerate input defers most tets: a 128^3 lattice's deferred
+// checks took 188 ms at one thread per expected tet, 165 ms at four, 154 ms
+// with four lanes per tet at four times the expected count.
 uint32_t deferred_grid(const wtet_ctx* c, uint64_t dper, uint32_t grid) {
-  return std::min(grid_for(c, 4 * dper, 1), std::min(grid * 4, 4096u));
+  return std::min(grid_for(c, 16 * dper, 1), std::min(grid * 8, 4096u));
 }
 
 // k flip iterations without the host: replays a captured graph (one per

@@ -577,15 +577,68 @@ __global__ void __launch_bounds__(kBlock) k_flip_check(Mesh M, FlipLists L, uint
   const uint32_t* cur = L.act[M.ctr[C_PAR] & 1u];
   uint32_t stride = gridDim.x * blockDim.x;
   PredStats ls = {};
-  if (FORM == SLOW) {
-    uint32_t n = M.ctr[C_SLOW_T];
-    for (uint32_t k = gid(); k < n; k += stride)
-      (void)flip_check_body_t<false>(M, cur, L.prop, L.slow[k], it, &ls);
-  } else {
-    uint32_t A = M.ctr[C_A];
-    for (uint32_t j = gid(); j < A; j += stride) {
-      bool defer = flip_check_body_t<FORM == LEAN>(M, cur, L.prop, j, it, &ls);
-      if (defer) L.slow[append_slot(&M.ctr[C_SLOW_T])] = j;
+  // LEAN defers what its filters leave to k_flip_check_deferred; FULL runs
+  // the full predicates on every entry
+  uint32_t A = M.ctr[C_A];
+  for (uint32_t j = gid(); j < A; j += stride) {
+    bool defer = flip_check_body_t<FORM == LEAN>(M, cur, L.prop, j, it, &ls);
+    if (defer) L.slow[append_slot(&M.ctr[C_SLOW_T])] = j;
+  }
+  flush_stats(ls, st);
+}
+
+// The full check of the tets the lean check deferred, four lanes per tet,
+// one per face: each lane runs its face's staged insphere and, where the
+// face's apex lies inside, the three orient3d that classify the flip; the
+// lowest face that proposes is the tet's proposal, as flip_check_body_t's
+// loop over the faces would choose it.  A tet's faces then reach the exact
+// and SoS stages together rather than one after another in divergent
+// passes.  A tet's lanes are consecutive in one wave (k = 4 * entry + face,
+// and a wave's first k is a multiple of 64).
+__global__ void __launch_bounds__(kBlock) k_flip_check_deferred(Mesh M, FlipLists L, uint64_t* st) {
+  if (!M.ctr[C_RUN]) return;
+  const uint32_t it = M.ctr[C_IT];
+  const uint32_t* cur = L.act[M.ctr[C_PAR] & 1u];
+  const uint32_t n4 = 4u * M.ctr[C_SLOW_T], stride = gridDim.x * blockDim.x;
+  PredStats ls = {};
+  for (uint32_t k = gid(); k < n4; k += stride) {
+    const uint32_t j = L.slow[k >> 2];
+    const int f = (int)(k & 3u);
+    const uint32_t t = cur[j];
+    const uint32_t info = INFO(M, t);
+    const uint4 tv = TV(M, t);
+    const uint4 tov = TOV(M, t);
+    const bool live = (info & kAlive) && tv.x != M.inf && tv.y != M.inf && tv.z != M.inf &&
+                      tv.w != M.inf;
+    bool qual = false;
+    uint32_t kind = 0u, bad = 0u, third = kInvalid;
+    if (live) {
+      float P[4][3], Q[4][3];
+      uint32_t R[4], RQ[4];
+      for (int q = 0; q < 4; ++q) load_pt(M, get4(tv, q), P[q], &R[q]);
+      for (int q = 0; q < 4; ++q) load_pt(M, get4(tov, q), Q[q], &RQ[q]);
+      qual = flip_face_full(M, t, tov, P, R, Q, RQ, f, &ls, &kind, &bad, &third);
+    }
+    const uint64_t qb = __ballot(qual);
+    const uint32_t g0 = __lane_id() & ~3u;
+    const uint32_t grp = (uint32_t)(qb >> g0) & 0xFu;
+    const uint32_t cf = grp ? (uint32_t)__builtin_ctz(grp) : 0u;
+    const uint32_t pk = (uint32_t)f | kind << 2 | bad << 3;
+    const uint32_t pk_c = (uint32_t)__shfl((int)pk, (int)(g0 + cf));
+    const uint32_t third_c = (uint32_t)__shfl((int)third, (int)(g0 + cf));
+    if (f == 0) {
+      // each entry is in the list once and only its lanes touch its info
+      INFO(M, t) = info & ~kQueued;
+      if (grp) {
+        uint32_t nt = opp_tet(*opp_word(M, t, cf));
+        uint64_t key = flip_key(it, t);
+        at_min_u64(&M.fvote[t], key);
+        at_min_u64(&M.fvote[nt], key);
+        if ((pk_c >> 2) & 1u) at_min_u64(&M.fvote[third_c], key);
+        L.prop[j] = make_uint4(t, pk_c, nt, third_c);
+      } else {
+        L.prop[j].x = kInvalid;
+      }
     }
   }
   flush_stats(ls, st);
@@ -753,7 +806,7 @@ void launch_flip_check(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_
 }
 void launch_flip_check_deferred(const Mesh& M, const FlipLists& L, uint32_t grid, uint64_t* st,
                                 hipStream_t s) {
-  if (M.mode == PRED_FILTERED) k_flip_check<SLOW><<<grid, kBlock, 0, s>>>(M, L, st);
+  if (M.mode == PRED_FILTERED) k_flip_check_deferred<<<grid, kBlock, 0, s>>>(M, L, st);
 }
 void launch_flip_decide_exec(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s) {
   k_flip_decide_exec<<<grid, kBlock, 0, s>>>(M, L);

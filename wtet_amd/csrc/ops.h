@@ -465,6 +465,57 @@ WT_HD bool flip_check_body_t(const Mesh& M, const uint32_t* act, uint4* prop, ui
   return false;
 }
 
+// point k (0..3, chosen at run time) of a tet's four: selects instead of an
+// indexed local array, which would live in scratch memory
+WT_HD void pick_pt(const float (*P)[3], const uint32_t* R, int k, float* out, uint32_t* r) {
+  for (int c = 0; c < 3; ++c) out[c] = k == 0 ? P[0][c] : (k == 1 ? P[1][c] : (k == 2 ? P[2][c] : P[3][c]));
+  *r = k == 0 ? R[0] : (k == 1 ? R[1] : (k == 2 ? R[2] : R[3]));
+}
+
+// One face f (chosen at run time) of flip_check_body_t's loop with the full
+// predicates: true when the face proposes a flip, with its kind, the
+// offending edge (3-2) and the third tet (3-2) as that loop sets them.  P, R:
+// the tet's points and ranks; Q, RQ: the vertices across its faces.
+WT_HD bool flip_face_full(const Mesh& M, uint32_t t, const uint4& tov, const float (*P)[3],
+                          const uint32_t* R, const float (*Q)[3], const uint32_t* RQ, int f,
+                          PredStats* st, uint32_t* kind, uint32_t* bad, uint32_t* third) {
+  uint32_t o = get4(tov, f);
+  if (o == M.inf) return false;
+  float O[3];
+  uint32_t ro;
+  pick_pt(Q, RQ, f, O, &ro);
+  int ins = o4s_check<false>(M, P[0], P[1], P[2], P[3], O, R[0], R[1], R[2], R[3], ro, st);
+  if (ins > 0) return false;
+  // o lies inside the circumsphere: classify the flip
+  float A[3], V[3][3];
+  uint32_t ra, RV[3];
+  pick_pt(P, R, f, A, &ra);
+  int pos[3] = {face_vi(f, 0), face_vi(f, 1), face_vi(f, 2)};
+  for (int k = 0; k < 3; ++k) pick_pt(P, R, pos[k], V[k], &RV[k]);
+  int b = -1, nbad = 0;
+  for (int k = 0; k < 3; ++k) {
+    int w = (k + 1) % 3;
+    if (o3s<false>(M, V[k], V[w], A, O, RV[k], RV[w], ra, ro, st) > 0) {
+      b = k;
+      ++nbad;
+    }
+  }
+  if (nbad == 0) {
+    *kind = 0u;
+    *bad = 0u;
+    *third = kInvalid;
+    return true;
+  }
+  if (nbad > 1) return false;
+  // 3-2 needs the edge to have degree 3 (see flip_check_body_t)
+  int pthird = b == 0 ? pos[2] : (b == 1 ? pos[0] : pos[1]);
+  if (get4(tov, pthird) != o) return false;
+  *kind = 1u;
+  *bad = (uint32_t)b;
+  *third = opp_tet(*opp_word(M, t, (uint32_t)pthird));
+  return true;
+}
+
 WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t j,
                            uint32_t it, PredStats* st) {
   (void)flip_check_body_t<false>(M, act, prop, j, it, st);
