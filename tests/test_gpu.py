@@ -61,7 +61,18 @@ def _pred_cases(rng, npts, k):
     S[:, 0, :] = sup[rng.integers(0, 4, k)]
     T = (rng.random((k, npts, 3), dtype=np.float32) * np.float32(1e-30)).astype(np.float32)
     T[:, :2] = rng.random((k, 2, 3), dtype=np.float32)
-    return np.concatenate([A, B, C, D, S, T])
+    # corners of small cubes of a 128^3 lattice: co-spherical and coplanar
+    # subsets, so SoS runs deep (the lifted steps on coplanar points) and
+    # the short-difference exact tier applies
+    base = rng.integers(0, 120, (k, 1, 3)).astype(np.float32)
+    step = rng.integers(1, 8, (k, 1, 1)).astype(np.float32)
+    E = ((base + rng.integers(0, 2, (k, npts, 3)).astype(np.float32) * step) / np.float32(127)).astype(np.float32)
+    # all points in one axis plane: every insphere vanishes and SoS is
+    # decided by the lifted steps on coplanar points (depths 2-4)
+    F = (rng.integers(0, 128, (k, npts, 3)).astype(np.float32) / np.float32(127)).astype(np.float32)
+    ax = rng.integers(0, 3, k)
+    F[np.arange(k), :, ax] = F[np.arange(k), :1, ax]
+    return np.concatenate([A, B, C, D, S, T, E, F])
 
 
 @pytest.mark.parametrize("mode", [0, 1])
