@@ -1281,8 +1281,11 @@ void finish_single(wtet_ctx* c, uint32_t n, const wtet_config& cfg, const Counte
       // regions reach past the staged rings (1M sphere points: 830-1,020
       // faces, 5-15 fetches at 7 rings, none at 10; repair median 24.8 ->
       // 14.5 ms); sparser ones (1M uniform: 120-200) keep the narrower
-      // staging, which is 0.7 ms cheaper there
-      const int rings = std::min(c->repair_rings + ((uint64_t)nviol * 2000u > n ? 3 : 0), kRepairRingsMax);
+      // staging, which is 0.7 ms cheaper there.  A rerun with the wider
+      // insertion ring (ring 2) leaves few stuck faces in large regions:
+      // it stages the wider rings too.
+      const bool wide = (uint64_t)nviol * 2000u > n || c->ring >= 2;
+      const int rings = std::min(c->repair_rings + (wide ? 3 : 0), kRepairRingsMax);
       stage_neighbourhood(c, (uint32_t)rec.size(), mout, rings);
       sparse = true;
     }
