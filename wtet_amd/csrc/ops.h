@@ -516,10 +516,6 @@ WT_HD bool flip_face_full(const Mesh& M, uint32_t t, const uint4& tov, const flo
   return true;
 }
 
-WT_HD void flip_check_body(const Mesh& M, const uint32_t* act, uint4* prop, uint32_t j,
-                           uint32_t it, PredStats* st) {
-  (void)flip_check_body_t<false>(M, act, prop, j, it, st);
-}
 
 // ---------------------------------------------------------------------------
 // execute a winning flip (src/shaders/flip.wgsl:531-845).  New tets reuse
