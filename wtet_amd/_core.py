@@ -62,7 +62,20 @@ class _ResultOwner:
             pass
 
 
-def _owned_rows(ptr, m: int, owner: _ResultOwner) -> np.ndarray:
+class _BatchOwner:
+    """Holds a wtet_batch_result until no array over its memory is left."""
+
+    def __init__(self, res):
+        self.res = res
+
+    def __del__(self):
+        try:
+            lib().wtet_batch_result_free(ctypes.byref(self.res))
+        except Exception:  # interpreter shutdown: the process frees it
+            pass
+
+
+def _owned_rows(ptr, m: int, owner) -> np.ndarray:
     """(m, 4) uint32 view of library memory that keeps `owner` alive."""
     buf = (ctypes.c_uint32 * (4 * m)).from_address(ctypes.addressof(ptr.contents))
     buf._owner = owner
@@ -608,17 +621,26 @@ class Context:
             if st in (1, 2):
                 raise ValueError(msg)
             raise RuntimeError(f"wtet_delaunay_batch failed ({_STATUS.get(st, st)}): {msg}")
+        # tets and adjacency: the library's arrays in place (no copy of the
+        # rows), freed when the last array over them is gone
+        m, C, k = res.num_tets, res.num_clouds, res.num_failed
         try:
-            m, C, k = res.num_tets, res.num_clouds, res.num_failed
             arr = lambda p, n, dt=np.uint32: (np.ctypeslib.as_array(p, shape=(n,)).copy() if n  # noqa: E731
                                               else np.zeros((0,), dt))
-            tets = arr(res.tets, 4 * m).reshape(-1, 4)
-            adj = arr(res.adjacency, 4 * m).reshape(-1, 4)
             toff = arr(res.tet_offsets, C + 1, np.uint64)
             failed = arr(res.failed_verts, k)
             foff = arr(res.failed_offsets, C + 1, np.uint64)
-        finally:
+        except BaseException:
             lib().wtet_batch_result_free(ctypes.byref(res))
+            raise
+        if m:
+            owner = _BatchOwner(res)
+            tets = _owned_rows(res.tets, m, owner)
+            adj = _owned_rows(res.adjacency, m, owner)
+        else:
+            lib().wtet_batch_result_free(ctypes.byref(res))
+            tets = np.zeros((0, 4), np.uint32)
+            adj = np.zeros((0, 4), np.uint32)
         return BatchResult(tets, adj, toff, failed, foff, self.stats())
 
     def delaunay_batch(self, clouds, offsets=None, config: "Config | None" = None) -> "BatchResult":
