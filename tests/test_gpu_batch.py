@@ -85,6 +85,25 @@ def test_batch_of_one_equals_single_call(oracle, ctx):
     assert b.tets.shape[0] == s.num_tets
 
 
+def test_batch_result_views_outlive_the_result(oracle, ctx):
+    """Batch tets and adjacency use the library's memory without a copy: a
+    cloud's views stay valid and unchanged after the batch result is gone
+    and other batches have allocated and freed theirs."""
+    import gc
+    clouds = [W.uniform(800 + 300 * k, 140 + k) for k in range(4)]
+    res = ctx.delaunay_batch(clouds)
+    c2 = res[2]
+    tets, adj = c2.tets, c2.adjacency
+    want_t, want_a = tets.copy(), adj.copy()
+    del res, c2
+    gc.collect()
+    for s in range(3):
+        ctx.delaunay_batch([W.uniform(900 + 100 * s, 150 + s), W.uniform(700, 160 + s)])
+        gc.collect()
+    assert np.array_equal(tets, want_t) and np.array_equal(adj, want_a)
+    assert np.array_equal(oracle.canonical_real(want_t, len(clouds[2])), _oracle_set(oracle, clouds[2]))
+
+
 def test_batch_identical_clouds(oracle, ctx):
     """The same coordinates in several clouds never interact."""
     p = W.uniform(500, 42)
