@@ -76,6 +76,10 @@ struct wtet_ctx {
   };
   std::vector<GraphEntry> graphs;        // flip iterations: one per (buffers, grid, iterations)
   std::vector<GraphEntry> round_graphs;  // rounds: one per (buffers, grid buckets, variant)
+  // keys launched directly once and not captured yet (a graph is captured
+  // the second time its key comes up: a one-off call does not pay for
+  // captures it never replays)
+  std::vector<std::vector<uint64_t>> graph_keys_seen;
   uint32_t graph_captures = 0;
   // stage-level session (wtet_stage_*), null when none
   wtet_stage_session* stage = nullptr;
@@ -467,6 +471,18 @@ void launch_cached_graph(wtet_ctx* c, std::vector<wtet_ctx::GraphEntry>& cache,
   for (auto& g : cache)
     if (g.key == key) exec = g.exec;
   if (!exec) {
+    // first sight of the key: launch directly and remember it
+    auto& seen = c->graph_keys_seen;
+    auto same_buffers = [&](const std::vector<uint64_t>& k) {
+      return k.size() >= buffers.size() && std::equal(buffers.begin(), buffers.end(), k.begin());
+    };
+    seen.erase(std::remove_if(seen.begin(), seen.end(), [&](const std::vector<uint64_t>& k) { return !same_buffers(k); }),
+               seen.end());
+    if (std::find(seen.begin(), seen.end(), key) == seen.end()) {
+      seen.push_back(key);
+      enqueue();
+      return;
+    }
     for (size_t i = 0; i < cache.size();) {
       const std::vector<uint64_t>& k = cache[i].key;
       if (k.size() < buffers.size() || !std::equal(buffers.begin(), buffers.end(), k.begin())) {
