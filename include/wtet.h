@@ -269,7 +269,9 @@ wtet_status wtet_set_audit_records(wtet_ctx *ctx, uint32_t records);
  * -- is captured in a HIP graph keyed by its grid sizes (powers of two over
  * the host's bounds; every kernel reads its counts from device counters)
  * and replayed for every round and call that fits it, with one host read
- * per round (on by default; off: the same kernels launched directly).  The list kernels are persistent grid-stride kernels
+ * per round (on by default; off: the same kernels launched directly).  A
+ * graph is captured the second time its key comes up: the first time its
+ * kernels are launched directly.  The list kernels are persistent grid-stride kernels
  * whose grid is sized per round from the host's bound on the first list,
  * for `items` entries per thread (default 2; powers of two from 16 to 4096
  * workgroups, one graph per size).  No counterpart in the reference, which
@@ -278,9 +280,10 @@ wtet_status wtet_set_flip_graphs(wtet_ctx *ctx, int on);
 wtet_status wtet_set_flip_items(wtet_ctx *ctx, uint32_t items);
 wtet_status wtet_get_flip_items(const wtet_ctx *ctx, uint32_t *items, uint32_t *graph_captures);
 /* Autotuner: triangulates the given clouds (device points) with 1, 2, 4
- * and 8 entries per thread, each setting once to capture its graphs and
- * three times timed; keeps the setting with the lowest median device time
- * for this context and reports that median. */
+ * and 8 entries per thread, each setting once untimed and three times
+ * timed (the first timed call captures the setting's graphs); keeps the
+ * setting with the lowest median device time for this context and reports
+ * that median. */
 wtet_status wtet_autotune(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
                           uint32_t num_clouds, const wtet_config *cfg, uint32_t *best_items,
                           double *best_ms);
