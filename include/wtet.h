@@ -280,10 +280,9 @@ wtet_status wtet_set_flip_graphs(wtet_ctx *ctx, int on);
 wtet_status wtet_set_flip_items(wtet_ctx *ctx, uint32_t items);
 wtet_status wtet_get_flip_items(const wtet_ctx *ctx, uint32_t *items, uint32_t *graph_captures);
 /* Autotuner: triangulates the given clouds (device points) with 1, 2, 4
- * and 8 entries per thread, each setting once untimed and three times
- * timed (the first timed call captures the setting's graphs); keeps the
- * setting with the lowest median device time for this context and reports
- * that median. */
+ * and 8 entries per thread, each setting twice untimed (the second call
+ * captures its graphs) and three times timed; keeps the setting with the
+ * lowest median device time for this context and reports that median. */
 wtet_status wtet_autotune(wtet_ctx *ctx, const float *d_xyz, const uint64_t *offsets,
                           uint32_t num_clouds, const wtet_config *cfg, uint32_t *best_items,
                           double *best_ms);

@@ -438,7 +438,8 @@ class Context:
 
     def autotune(self, ptr: int, offsets, config: "Config | None" = None):
         """Time 1, 2, 4 and 8 list entries per thread on these device points
-        (median of three calls each, after one that captures the graphs);
+        (median of three calls each, after two untimed ones, the second of
+        which captures the graphs);
         keeps the fastest.  Returns (entries per thread, median device ms)."""
         off = _check_offsets(offsets)
         self._prepare(config)

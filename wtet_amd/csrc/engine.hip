@@ -2568,16 +2568,17 @@ wtet_status wtet_autotune(wtet_ctx* c, const float* d_xyz, const uint64_t* offse
   for (uint32_t g : cand) {
     c->items = g;
     wtet_batch_device_result r;
-    // the first call at a setting captures its graphs; the median of the
-    // next three is its time (one run swings by a few percent)
+    // the first call at a setting launches its new graph keys directly and
+    // the second captures them; the median of the next three is its time
+    // (one run swings by a few percent)
     double ms[3] = {0, 0, 0};
-    for (int rep = 0; rep < 4; ++rep) {
+    for (int rep = 0; rep < 5; ++rep) {
       wtet_status st = wtet_delaunay_batch_resident(c, d_xyz, offsets, num_clouds, cfg, &r);
       if (st != WTET_OK) {
         c->items = keep;
         return st;
       }
-      if (rep > 0) ms[rep - 1] = c->stats.ms_device;
+      if (rep > 1) ms[rep - 2] = c->stats.ms_device;
     }
     std::sort(ms, ms + 3);
     if (ms[1] < bms) {
