@@ -645,8 +645,10 @@ __global__ void __launch_bounds__(kBlock) k_flip_check_deferred(Mesh M, FlipList
 }
 
 // Exclusive prefixes of three small counts in one pass (a, b in {0, 1},
-// c in 0..3), packed 10 bits each (block totals stay below 1024).  Every
-// thread of the block must call it; w is LDS of kBlock/64.
+// c in 0..3), packed 10, 10 and 12 bits (block totals of a and b stay
+// below 1024, of c below 4096, for blocks of at most 512 threads).  Every
+// thread of the block of BS threads must call it; w is LDS of BS/64.
+template <int BS>
 __device__ __forceinline__ uint32_t block_prefix3(uint32_t a, uint32_t b, uint32_t c, uint32_t* tot,
                                                   uint32_t* w) {
   uint64_t lt = (1ull << __lane_id()) - 1ull;
@@ -658,7 +660,7 @@ __device__ __forceinline__ uint32_t block_prefix3(uint32_t a, uint32_t b, uint32
   __syncthreads();
   uint32_t off = 0, t = 0;
 #pragma unroll
-  for (int k = 0; k < kBlock / 64; ++k) {
+  for (int k = 0; k < BS / 64; ++k) {
     off += k < wave ? w[k] : 0u;
     t += w[k];
   }
@@ -673,13 +675,21 @@ __device__ __forceinline__ uint32_t block_prefix3(uint32_t a, uint32_t b, uint32
 // next active entries) -- the counters share a cache line, so per-wave
 // atomics from every wave would serialize -- and every winner executes its
 // own flip from registers.
-__global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L) {
-  __shared__ uint32_t wsum[kBlock / 64];
+// Blocks of 512 threads: each tile of proposals claims its flips, fresh
+// slots and queue entries with one atomic per counter, and those
+// same-address atomics complete one at a time at the memory side; 512
+// halves them against 256 (1M uniform device 26.0-26.1 -> 25.7-26.0 ms).
+constexpr uint32_t kDecBlock = 512;
+static_assert(kDecBlock <= 512, "block_prefix3 packs totals of at most 512 threads");
+
+__global__ void __launch_bounds__(kDecBlock) k_flip_decide_exec(Mesh M, FlipLists L) {
+  constexpr uint32_t kBS = kDecBlock;
+  __shared__ uint32_t wsum[kBS / 64];
   __shared__ uint32_t base[3];
   if (!M.ctr[C_RUN]) return;
   uint32_t A = M.ctr[C_A], it = M.ctr[C_IT], round = M.ctr[C_ROUND];
   uint32_t* act_next = L.act[(M.ctr[C_PAR] & 1u) ^ 1u];
-  for (uint32_t b0 = blockIdx.x * kBlock; b0 < A; b0 += gridDim.x * kBlock) {
+  for (uint32_t b0 = blockIdx.x * kBS; b0 < A; b0 += gridDim.x * kBS) {
     uint32_t j = b0 + threadIdx.x;
     uint4 pr = make_uint4(kInvalid, 0u, 0u, 0u);
     if (j < A) pr = L.prop[j];
@@ -688,7 +698,7 @@ __global__ void __launch_bounds__(kBlock) k_flip_decide_exec(Mesh M, FlipLists L
     bool win = flip_decide_load(M, pr, it, qid, &nq, &nslot, F);
     uint32_t nqa = nq + nslot;  // the fresh slot is queued too
     uint32_t tot;
-    uint32_t r = block_prefix3(win ? 1u : 0u, nslot, nqa, &tot, wsum);
+    uint32_t r = block_prefix3<kBS>(win ? 1u : 0u, nslot, nqa, &tot, wsum);
     uint32_t rf = r & 1023u, rs = (r >> 10) & 1023u, rq = r >> 20;
     uint32_t tf = tot & 1023u, ts = (tot >> 10) & 1023u, tq = tot >> 20;
     if (threadIdx.x == 0) {
@@ -808,8 +818,10 @@ void launch_flip_check_deferred(const Mesh& M, const FlipLists& L, uint32_t grid
                                 hipStream_t s) {
   if (M.mode == PRED_FILTERED) k_flip_check_deferred<<<grid, kBlock, 0, s>>>(M, L, st);
 }
+// grid: blocks of kBlock threads, as the other list kernels; the same
+// threads in blocks of kDecBlock
 void launch_flip_decide_exec(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s) {
-  k_flip_decide_exec<<<grid, kBlock, 0, s>>>(M, L);
+  k_flip_decide_exec<<<std::max<uint32_t>(grid * kBlock / kDecBlock, 1u), kDecBlock, 0, s>>>(M, L);
 }
 void launch_flip_update(const Mesh& M, const FlipLists& L, uint32_t grid, hipStream_t s) {
   k_flip_update<<<grid, kBlock, 0, s>>>(M, L);
